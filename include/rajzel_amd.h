@@ -1,0 +1,152 @@
+/*
+ * rajzel_amd.h -- C ABI of the MI355X (gfx950) implementation of RAJZEL's
+ * adjoint-state first-arrival traveltime (FATT) gradient path.
+ *
+ * The path replaced is the reference's per-evaluation shot loop
+ *   grad_obj (grad_obj.c:10-79)
+ *     -> eikonal (eikonal.c:10-76, sweep.c:10-104)
+ *     -> calc_FA / calc_FA_res (calc_FA.c:10-19, calc_FA_res.c:10-77)
+ *     -> eikonal_adj (eikonal_adj.c:10-59, sweep_adj.c:10-77,
+ *                     init_travel_adj.c:12-60, init_recflag.c:12-35)
+ *     -> ass_grad (ass_grad.c:11-49, cp_grad_frame.c:10-53)
+ *   summed over shots (grad_obj.c:33-68, sum_grad_MPI.c:19-47).
+ *
+ * Conventions
+ *   - Fields are float32, row-major NY x NX with x fastest: the contiguous
+ *     block &A[1][1] of the reference's matrix() (util.c:177-204).
+ *   - Grid indices are 0-based: reference index (iround(pos/DH), fd.h:18)
+ *     minus one.  rz_grid_index() reproduces that rounding.
+ *   - Pointers flagged "device" are HIP device pointers; all others are host.
+ *   - Every call returns an rz_status; on failure rz_last_error() names the
+ *     cause.  Nothing here aborts the process (the reference's err() does;
+ *     the reference-surface shim maps failures onto it).
+ *   - Results are bitwise those of the reference's single-rank float path
+ *     (one MPI rank), see DESIGN.md for the exceptions.
+ *
+ * A context is single-threaded: calls on one context must not overlap.
+ */
+#ifndef RAJZEL_AMD_H
+#define RAJZEL_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rz_status {
+    RZ_OK = 0,
+    RZ_ERR_INVALID = 1,        /* bad argument, geometry or slowness          */
+    RZ_ERR_HIP = 2,            /* HIP runtime error                           */
+    RZ_ERR_NOMEM = 3,          /* device allocation failed                    */
+    RZ_ERR_NOT_CONVERGED = 4,  /* a shot hit max_iter (results still written) */
+    RZ_ERR_STATE = 5           /* call out of order (e.g. no slowness yet)    */
+} rz_status;
+
+typedef struct rz_ctx rz_ctx;
+
+/* One model (grid + acquisition).  Globals of the reference it replaces:
+ * NX, NY, DH, TIME, TTNORM, EPS_ADJ (globvar.h:12-15,96); srcpos/recpos
+ * (sources.c:68-71, receiver.c:55-58).  max_iter bounds the outer
+ * fast-sweeping iterations (the reference has no bound, eikonal.c:44,
+ * eikonal_adj.c:37); 0 selects 1000. */
+typedef struct rz_model_desc {
+    int nx, ny;                /* >= 5 each                                  */
+    float dh;                  /* DH (m), > 0                                */
+    float time;                /* TIME: T starts at 2*TIME, lambda at 10*TIME */
+    float ttnorm;              /* TTNORM: stop when mean |dT| < TTNORM        */
+    float eps_adj;             /* EPS_ADJ water level of sweep_adj.c:60       */
+    int max_iter;
+    int nshots;
+    const int *src_ix, *src_iy;  /* nshots source cells (0-based)           */
+    int ntr;
+    const int *rec_ix, *rec_iy;  /* ntr receiver cells (0-based), shared by all shots */
+} rz_model_desc;
+
+/* 0-based grid index of a position in metres: (int)floor(pos/DH + 0.5) - 1
+ * with pos/DH a float division (fd.h:18, eikonal.c:31-32, receiver.c:55). */
+int rz_grid_index(float pos_m, float dh);
+
+/* Create a context on HIP device `device` holding `nmodels` models.
+ * On failure *out is NULL and rz_last_error(NULL) gives the reason.
+ *   rows_per_lane  1, 2, 4 or 8 rows of a band per lane; 0 = chosen
+ *   hbm_budget     bytes of device memory for per-shot fields; 0 = enough
+ *                  for every shot at once.  Below that, shots run in
+ *                  batches that fit and forward fields are recomputed per
+ *                  batch (results are unchanged). */
+int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models, int rows_per_lane,
+              size_t hbm_budget);
+void rz_destroy(rz_ctx *ctx);
+const char *rz_last_error(const rz_ctx *ctx);
+
+/* Use an existing HIP stream (hipStream_t) for all work; NULL = the
+ * context's own stream. */
+int rz_set_stream(rz_ctx *ctx, void *hip_stream);
+
+/* Slowness S = 1/Vp (calc_S.c:18-24), NY*NX.  Replaces the reference's S
+ * matrix passed to grad_obj (fd.h:72).  Rejected unless finite and > 0. */
+int rz_set_slowness(rz_ctx *ctx, int model, const float *S, int on_device);
+
+/* Observed picks, nshots*ntr, shot-major; negative = muted
+ * (calc_FA_res.c:39).  Replaces the <DATA_DIR>_<shot>.dat files read by
+ * calc_FA_res.c:28-37. */
+int rz_set_picks(rz_ctx *ctx, int model, const float *tobs, int on_device);
+
+/* Initial contents (ntr) of the residual array Tres that grad_obj's caller
+ * passes in (fd.h:72); muted picks inherit it (calc_FA_res.c:39-70).
+ * Default zero. */
+int rz_set_tres_carry(rz_ctx *ctx, int model, const float *tres0);
+
+/* Forward only, every shot of every model: eikonal() per shot (eikonal.c:
+ * 10-76) with receiver times (calc_FA.c:15-17); the driver loop of
+ * forward.c:19-24. */
+int rz_forward(rz_ctx *ctx);
+
+/* One objective + gradient evaluation of every model: grad_obj()
+ * (grad_obj.c:10-79) with precond() left to the caller.  All shots in one
+ * launch per phase. */
+int rz_gradient(rz_ctx *ctx);
+
+/* The same evaluation with one shot per launch (the plain GPU form, the
+ * reference's shot loop with each routine a launch).  Same results. */
+int rz_gradient_per_shot(rz_ctx *ctx);
+
+/* Results of the last rz_forward / rz_gradient. */
+int rz_get_tmod(rz_ctx *ctx, int model, float *out, int on_device);      /* nshots*ntr */
+int rz_get_tres(rz_ctx *ctx, int model, float *out, int on_device);      /* nshots*ntr */
+int rz_get_grad(rz_ctx *ctx, int model, float *out, int on_device);      /* NY*NX      */
+/* L2 = 0.5 * sum of per-shot sums (grad_obj.c:61); l2_shot may be NULL. */
+int rz_get_misfit(rz_ctx *ctx, int model, double *L2, double *l2_shot);
+/* Outer iterations per shot; -1 = hit max_iter.  Either pointer may be NULL. */
+int rz_get_iterations(rz_ctx *ctx, int model, int *it_f, int *it_a);
+/* Tres array after the evaluation (the last shot's residuals). */
+int rz_get_tres_carry(rz_ctx *ctx, int model, float *out);
+/* Field of one shot, NY*NX: which = 0 traveltime T, 1 adjoint lambda.
+ * Only for shots of the last batch (all shots without an HBM budget). */
+int rz_get_field(rz_ctx *ctx, int model, int shot, int which, float *out, int on_device);
+
+/* Per-shot entry points with host arrays, used by the reference-surface
+ * shim (eikonal(), eikonal_adj(), ass_grad()).  `model` fixes the grid and
+ * receivers; its slowness must be set.
+ *   rz_eikonal_shot      eikonal.c:10-76 for a source cell: T, receiver
+ *                        times, outer iterations.
+ *   rz_eikonal_adj_shot  eikonal_adj.c:10-59 for a given T and Tres.
+ *   rz_ass_grad_shot     ass_grad.c:11-49 (no per-shot taper):
+ *                        grad_shot = frame-copied lambda*S^3, grad += it. */
+int rz_eikonal_shot(rz_ctx *ctx, int model, int src_ix, int src_iy, float *T, float *tmod, int *iters);
+int rz_eikonal_adj_shot(rz_ctx *ctx, int model, const float *T, const float *tres, float *lam, int *iters);
+int rz_ass_grad_shot(rz_ctx *ctx, int model, const float *lam, float *grad_shot, float *grad);
+
+/* Block until the context's stream is idle. */
+int rz_sync(rz_ctx *ctx);
+
+/* Rows per lane in use and bytes of device memory held. */
+int rz_rows_per_lane(const rz_ctx *ctx);
+size_t rz_device_bytes(const rz_ctx *ctx);
+/* Shots per launch of rz_gradient (all shots, or fewer under a budget). */
+int rz_batch_shots(const rz_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAJZEL_AMD_H */

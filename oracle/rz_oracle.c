@@ -1,0 +1,347 @@
+/*
+ * rz_oracle.c -- CPU restatement of RAJZEL's adjoint-state FATT gradient path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the package's
+ * smoke() and the cpu_baseline leg of bench.py may load it; the product path
+ * (rajzel_amd/) never links, loads or calls it.
+ *
+ * Arrays are flat, 0-based, row-major NY x NX (x fastest), i.e. the block
+ * &A[1][1] of the reference's Numerical-Recipes matrices (util.c:177-204).
+ * Grid indices (source / receiver) are 0-based: reference index minus one.
+ *
+ * The float build (default) reproduces the reference arithmetic operation by
+ * operation, including the mixed float/double expressions of sweep.c:89-95,
+ * the float L1 accumulation of norm1.c:20-29 and the double product of
+ * ass_grad.c:23.  Compiled with -DRZO_F64 the same code runs every quantity
+ * in double: that build is the "FP64 CPU run" the float results are compared
+ * with.  Build with -ffp-contract=off (no FMA contraction; the reference's
+ * x86-64 build has none).
+ *
+ * Pinning (how each function is checked, besides the compiled reference in
+ * oracle/_ref):
+ *   rzo_sweep / rzo_eikonal   homogeneous closed form along grid lines,
+ *                             causality invariants, order independence,
+ *                             fast-marching fixed point (tests/test_oracle_math.py)
+ *   rzo_sweep_adj             direct sparse solve of the Leung-Qian system
+ *                             (tests/test_oracle_math.py)
+ *   rzo_ass_grad, frame,      elementwise closed forms
+ *   rzo_calc_fa_res
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef RZO_F64
+typedef double real;
+#define RMIN(a, b) fmin((a), (b))
+#define NAME(x) x##_f64
+#else
+typedef float real;
+#define RMIN(a, b) fminf((a), (b))
+#define NAME(x) x
+#endif
+
+#define IDX(k, h) ((size_t)(k) * (size_t)nx + (size_t)(h))
+
+/* INF of fd.h:29-31 (an int constant compared against float fields). */
+#define RZO_INF 9999999
+
+/* ---------------------------------------------------------------------------
+ * One Gauss-Seidel fast-sweeping pass of the Godunov upwind update
+ * (sweep.c:10-104).  sx, sy = +1/-1 are the traversal directions (ndx, ndy).
+ * The reference walks x in the outer loop and y in the inner loop; a
+ * neighbour outside the grid is replaced by the cell itself (sweep.c:37-85).
+ * ------------------------------------------------------------------------- */
+void NAME(rzo_sweep)(int nx, int ny, float dh_, const real *S, real *T, int sx, int sy)
+{
+    const real dh = (real)dh_;
+    for (int ii = 0; ii < nx; ++ii) {
+        const int h = sx > 0 ? ii : nx - 1 - ii;
+        for (int jj = 0; jj < ny; ++jj) {
+            const int k = sy > 0 ? jj : ny - 1 - jj;
+            const real self = T[IDX(k, h)];
+            const real xl = h > 0 ? T[IDX(k, h - 1)] : self;
+            const real xr = h < nx - 1 ? T[IDX(k, h + 1)] : self;
+            const real yu = k > 0 ? T[IDX(k - 1, h)] : self;
+            const real yd = k < ny - 1 ? T[IDX(k + 1, h)] : self;
+            const real a = RMIN(xl, xr);
+            const real b = RMIN(yu, yd);
+            const real s = S[IDX(k, h)];
+            const real sdh = s * dh;              /* float product (sweep.c:89) */
+            const real d = a - b;                 /* float difference */
+            real tt;
+            if (fabs((double)d) >= (double)sdh) {
+                tt = RMIN(a, b) + sdh;            /* sweep.c:90 */
+            } else {
+                /* sweep.c:94: (a+b) is a float sum; the discriminant and the
+                 * square root are evaluated in double, the result is rounded
+                 * back to the field type on assignment. */
+                const double sd = (double)s;
+                const double hd = (double)dh;
+                const double disc = (2.0 * (sd * sd)) * (hd * hd) - (double)d * (double)d;
+                tt = (real)(((double)(a + b) + sqrt(disc)) / 2.0);
+            }
+            T[IDX(k, h)] = RMIN(self, tt);        /* sweep.c:97 */
+        }
+    }
+}
+
+/* norm1.c:17-31: float accumulator, x outer / y inner, mean over NX*NY. */
+static real NAME(l1_mean)(int nx, int ny, const real *A, const real *B)
+{
+    real sum = 0;
+    for (int h = 0; h < nx; ++h)
+        for (int k = 0; k < ny; ++k)
+            sum = (real)((double)sum + fabs((double)(A[IDX(k, h)] - B[IDX(k, h)])));
+    return sum / (real)(nx * ny);
+}
+
+/* ---------------------------------------------------------------------------
+ * Forward Eikonal solve for one shot (eikonal.c:10-76 without file output).
+ * T must hold nx*ny values; it is initialised here (zero_travel.c:19-25,
+ * eikonal.c:36).  Returns the number of outer iterations executed (each one
+ * = 4 sweeps), or -1 if maxit was reached before convergence.
+ * Sweep order (x+,y+),(x-,y+),(x-,y-),(x+,y-) as eikonal.c:50-53.
+ * ------------------------------------------------------------------------- */
+int NAME(rzo_eikonal)(int nx, int ny, float dh, float time, float ttnorm, int maxit,
+                      const real *S, int isx, int isy, real *T)
+{
+    const size_t n = (size_t)nx * (size_t)ny;
+    real *told = (real *)malloc(n * sizeof(real));
+    if (!told) return -2;
+    for (size_t i = 0; i < n; ++i) T[i] = (real)(2.0 * (double)time);
+    T[IDX(isy, isx)] = 0;
+    real lnorm = (real)(10.0 * (double)time);
+    int it = 0;
+    while (lnorm >= (real)ttnorm) {
+        if (maxit > 0 && it >= maxit) { it = -1; break; }
+        memcpy(told, T, n * sizeof(real));
+        NAME(rzo_sweep)(nx, ny, dh, S, T, +1, +1);
+        NAME(rzo_sweep)(nx, ny, dh, S, T, -1, +1);
+        NAME(rzo_sweep)(nx, ny, dh, S, T, -1, -1);
+        NAME(rzo_sweep)(nx, ny, dh, S, T, +1, -1);
+        lnorm = NAME(l1_mean)(nx, ny, T, told);
+        ++it;
+    }
+    free(told);
+    return it;
+}
+
+/* calc_FA.c:15-17 */
+void NAME(rzo_calc_fa)(int nx, int ny, const real *T, int ntr, const int *rx, const int *ry, real *tmod)
+{
+    (void)ny;
+    for (int i = 0; i < ntr; ++i) tmod[i] = T[IDX(ry[i], rx[i])];
+}
+
+/* ---------------------------------------------------------------------------
+ * Residual, misfit and adjoint-source scaling for one shot
+ * (calc_FA_res.c:10-77, with the picks passed in instead of read from
+ * <DATA_DIR>_<shot>.dat).  tres is in/out: a muted pick (tobs < 0) leaves
+ * tres[i] unchanged, exactly as the reference does (calc_FA_res.c:39).  The
+ * reference's l2 accumulator is uninitialised (calc_FA_res.c:21); here it
+ * starts at zero.  Returns sum of tres^2 (float squares, double sum).
+ * ------------------------------------------------------------------------- */
+double NAME(rzo_calc_fa_res)(int nx, int ny, float dh_, const real *T, int ntr, const int *rx,
+                             const int *ry, const real *tmod, const real *tobs, real *tres)
+{
+    const real dh = (real)dh_;
+    double l2 = 0.0;
+    for (int i = 0; i < ntr; ++i) {
+        if (!(tobs[i] >= 0)) continue;
+        real r = tmod[i] - tobs[i];
+        l2 += (double)(r * r);
+        const int k = ry[i], h = rx[i];
+        real g;
+        int have = 1;
+        if (k <= ny - 2)      g = (T[IDX(k + 1, h)] - T[IDX(k, h)]) / dh;   /* forward diff */
+        else if (k == ny - 1) g = (T[IDX(k, h)] - T[IDX(k - 1, h)]) / dh;   /* last row */
+        else { g = 0; have = 0; }
+        if (have) r = (fabs((double)g) > 0.0) ? r / g : (real)0;
+        tres[i] = r;
+    }
+    return l2;
+}
+
+/* init_recflag.c:12-35 as an O(ntr) scatter: last receiver in a cell wins;
+ * receivers outside the grid mark nothing.  Values are 1-based receiver
+ * numbers, 0 = no receiver. */
+void NAME(rzo_init_recflag)(int nx, int ny, int ntr, const int *rx, const int *ry, int *flag)
+{
+    memset(flag, 0, (size_t)nx * (size_t)ny * sizeof(int));
+    for (int i = 0; i < ntr; ++i)
+        if (rx[i] >= 0 && rx[i] < nx && ry[i] >= 0 && ry[i] < ny) flag[IDX(ry[i], rx[i])] = i + 1;
+}
+
+/* init_travel_adj.c:12-60 as an O(ntr) scatter: zero everywhere, 10*TIME on
+ * the interior, tres of the last receiver at interior receiver cells. */
+void NAME(rzo_init_travel_adj)(int nx, int ny, float time, int ntr, const int *rx, const int *ry,
+                               const real *tres, real *lam)
+{
+    const real big = (real)(10.0 * (double)time);
+    for (int k = 0; k < ny; ++k)
+        for (int h = 0; h < nx; ++h)
+            lam[IDX(k, h)] = (k > 0 && k < ny - 1 && h > 0 && h < nx - 1) ? big : (real)0;
+    for (int i = 0; i < ntr; ++i)
+        if (rx[i] > 0 && rx[i] < nx - 1 && ry[i] > 0 && ry[i] < ny - 1) lam[IDX(ry[i], rx[i])] = tres[i];
+}
+
+/* ---------------------------------------------------------------------------
+ * One adjoint sweep (sweep_adj.c:10-77): Leung & Qian (2006) eq. 3.6 upwind
+ * discretisation on the interior, receiver cells skipped, lambda only ever
+ * decreases, EPS_ADJ water level, INF guard.  Pure field-type arithmetic in
+ * source order; (x +- |x|)/2 is exact and kept as written.
+ * ------------------------------------------------------------------------- */
+void NAME(rzo_sweep_adj)(int nx, int ny, float dh_, float eps_adj, const real *T, const int *flag,
+                         real *lam, int sx, int sy)
+{
+    const real dh = (real)dh_;
+    for (int ii = 0; ii < nx - 2; ++ii) {
+        const int h = sx > 0 ? 1 + ii : nx - 2 - ii;
+        for (int jj = 0; jj < ny - 2; ++jj) {
+            const int k = sy > 0 ? 1 + jj : ny - 2 - jj;
+            if (flag[IDX(k, h)] != 0) continue;
+            const real tc = T[IDX(k, h)];
+            const real txp = T[IDX(k, h + 1)], txm = T[IDX(k, h - 1)];
+            const real typ = T[IDX(k + 1, h)], tym = T[IDX(k - 1, h)];
+            const real ap = -(txp - tc) / dh;
+            const real am = -(tc - txm) / dh;
+            const real bp = -(typ - tc) / dh;
+            const real bm = -(tc - tym) / dh;
+            const real app = (real)((ap + fabs(ap)) / 2.0), apm = (real)((ap - fabs(ap)) / 2.0);
+            const real amp = (real)((am + fabs(am)) / 2.0), amm = (real)((am - fabs(am)) / 2.0);
+            const real bpp = (real)((bp + fabs(bp)) / 2.0), bpm = (real)((bp - fabs(bp)) / 2.0);
+            const real bmp = (real)((bm + fabs(bm)) / 2.0), bmm = (real)((bm - fabs(bm)) / 2.0);
+            const real lhs = (app - amm) / dh + (bpp - bmm) / dh;
+            const real rhs = (amp * lam[IDX(k, h - 1)] - apm * lam[IDX(k, h + 1)]) / dh
+                           + (bmp * lam[IDX(k - 1, h)] - bpm * lam[IDX(k + 1, h)]) / dh;
+            const real lt = (lhs > (real)eps_adj) ? rhs / lhs : (real)0;
+            real v = RMIN(lam[IDX(k, h)], lt);
+            if (tc >= RZO_INF || typ >= RZO_INF || tym >= RZO_INF || txp >= RZO_INF || txm >= RZO_INF) v = 0;
+            lam[IDX(k, h)] = v;
+        }
+    }
+}
+
+/* Adjoint solve for one shot (eikonal_adj.c:10-59).  Sweep order
+ * (x+,y+),(x+,y-),(x-,y+),(x-,y-) as eikonal_adj.c:43-46.  lam is
+ * initialised here.  Returns outer iterations, or -1 at maxit. */
+int NAME(rzo_eikonal_adj)(int nx, int ny, float dh, float time, float ttnorm, float eps_adj, int maxit,
+                          const real *T, int ntr, const int *rx, const int *ry, const real *tres,
+                          const int *flag, real *lam)
+{
+    const size_t n = (size_t)nx * (size_t)ny;
+    real *old = (real *)malloc(n * sizeof(real));
+    if (!old) return -2;
+    NAME(rzo_init_travel_adj)(nx, ny, time, ntr, rx, ry, tres, lam);
+    real lnorm = (real)(10.0 * (double)time);
+    int it = 0;
+    while (lnorm >= (real)ttnorm) {
+        if (maxit > 0 && it >= maxit) { it = -1; break; }
+        memcpy(old, lam, n * sizeof(real));
+        NAME(rzo_sweep_adj)(nx, ny, dh, eps_adj, T, flag, lam, +1, +1);
+        NAME(rzo_sweep_adj)(nx, ny, dh, eps_adj, T, flag, lam, +1, -1);
+        NAME(rzo_sweep_adj)(nx, ny, dh, eps_adj, T, flag, lam, -1, +1);
+        NAME(rzo_sweep_adj)(nx, ny, dh, eps_adj, T, flag, lam, -1, -1);
+        lnorm = NAME(l1_mean)(nx, ny, lam, old);
+        ++it;
+    }
+    free(old);
+    return it;
+}
+
+/* cp_grad_frame.c:10-53: two-cell frame copied from row 3 / NY-2 and then
+ * from column 3 / NX-2 (1-based). */
+void NAME(rzo_cp_grad_frame)(int nx, int ny, real *A)
+{
+    for (int h = 0; h < nx; ++h) {
+        A[IDX(1, h)] = A[IDX(2, h)];
+        A[IDX(0, h)] = A[IDX(1, h)];
+    }
+    for (int h = 0; h < nx; ++h) {
+        A[IDX(ny - 2, h)] = A[IDX(ny - 3, h)];
+        A[IDX(ny - 1, h)] = A[IDX(ny - 2, h)];
+    }
+    for (int k = 0; k < ny; ++k) {
+        A[IDX(k, 1)] = A[IDX(k, 2)];
+        A[IDX(k, 0)] = A[IDX(k, 1)];
+    }
+    for (int k = 0; k < ny; ++k) {
+        A[IDX(k, nx - 2)] = A[IDX(k, nx - 3)];
+        A[IDX(k, nx - 1)] = A[IDX(k, nx - 2)];
+    }
+}
+
+/* ass_grad.c:11-49 without the optional per-shot tapers (SWS_TAPER_CIRCULAR_
+ * PER_SHOT = 0): grad_shot = lam*S^3 on the interior, frame copy, and
+ * grad += grad_shot over the whole grid.  pow(S,3.0) is the double product
+ * S*S*S with one rounding (S*S is exact in double). */
+void NAME(rzo_ass_grad)(int nx, int ny, const real *S, const real *lam, real *gshot, real *grad)
+{
+    memset(gshot, 0, (size_t)nx * (size_t)ny * sizeof(real));
+    for (int h = 1; h < nx - 1; ++h)
+        for (int k = 1; k < ny - 1; ++k) {
+            const double s = (double)S[IDX(k, h)];
+            gshot[IDX(k, h)] = (real)((double)lam[IDX(k, h)] * (s * s * s));
+        }
+    NAME(rzo_cp_grad_frame)(nx, ny, gshot);
+    const size_t n = (size_t)nx * (size_t)ny;
+    for (size_t i = 0; i < n; ++i) grad[i] += gshot[i];
+}
+
+/* ---------------------------------------------------------------------------
+ * One objective + gradient evaluation over shots 0..nshots-1 on one rank
+ * (grad_obj.c:10-79 without precond, which is the identity for
+ * SPATFILTER=0 and no tapers).
+ *   src_x/src_y   0-based source indices per shot
+ *   tobs          nshots*ntr picks (negative = muted)
+ *   tres          ntr, in/out: carry of the reference's residual array
+ *   grad          out, nx*ny
+ *   T, lam        out: fields of the LAST shot (the reference leaves them so)
+ *   tmod_all, tres_all, l2_shot, it_f, it_a   optional per-shot outputs
+ * Returns L2 = 0.5 * sum over shots of the per-shot sums.
+ * ------------------------------------------------------------------------- */
+double NAME(rzo_grad_obj)(int nx, int ny, float dh, float time, float ttnorm, float eps_adj, int maxit,
+                          const real *S, int nshots, const int *src_x, const int *src_y,
+                          int ntr, const int *rx, const int *ry, const real *tobs, real *tres,
+                          real *grad, real *T, real *lam,
+                          real *tmod_all, real *tres_all, double *l2_shot, int *it_f, int *it_a)
+{
+    const size_t n = (size_t)nx * (size_t)ny;
+    real *gshot = (real *)malloc(n * sizeof(real));
+    int *flag = (int *)malloc(n * sizeof(int));
+    real *tmod = (real *)malloc((size_t)(ntr > 0 ? ntr : 1) * sizeof(real));
+    if (!gshot || !flag || !tmod) { free(gshot); free(flag); free(tmod); return NAN; }
+    memset(grad, 0, n * sizeof(real));
+    NAME(rzo_init_recflag)(nx, ny, ntr, rx, ry, flag);
+    double l2sum = 0.0;
+    for (int s = 0; s < nshots; ++s) {
+        const int itf = NAME(rzo_eikonal)(nx, ny, dh, time, ttnorm, maxit, S, src_x[s], src_y[s], T);
+        NAME(rzo_calc_fa)(nx, ny, T, ntr, rx, ry, tmod);
+        const double l2 = NAME(rzo_calc_fa_res)(nx, ny, dh, T, ntr, rx, ry, tmod, tobs + (size_t)s * ntr, tres);
+        l2sum += l2;
+        const int ita = NAME(rzo_eikonal_adj)(nx, ny, dh, time, ttnorm, eps_adj, maxit, T, ntr, rx, ry, tres, flag, lam);
+        NAME(rzo_ass_grad)(nx, ny, S, lam, gshot, grad);
+        if (tmod_all) memcpy(tmod_all + (size_t)s * ntr, tmod, (size_t)ntr * sizeof(real));
+        if (tres_all) memcpy(tres_all + (size_t)s * ntr, tres, (size_t)ntr * sizeof(real));
+        if (l2_shot) l2_shot[s] = l2;
+        if (it_f) it_f[s] = itf;
+        if (it_a) it_a[s] = ita;
+    }
+    NAME(rzo_cp_grad_frame)(nx, ny, grad);
+    free(gshot); free(flag); free(tmod);
+    return 0.5 * l2sum;
+}
+
+/* Forward-only driver over all shots (forward.c:19-24) returning receiver
+ * times; iteration counts per shot in it_f (optional). */
+void NAME(rzo_forward)(int nx, int ny, float dh, float time, float ttnorm, int maxit, const real *S,
+                       int nshots, const int *src_x, const int *src_y, int ntr, const int *rx,
+                       const int *ry, real *tmod_all, real *T, int *it_f)
+{
+    for (int s = 0; s < nshots; ++s) {
+        const int itf = NAME(rzo_eikonal)(nx, ny, dh, time, ttnorm, maxit, S, src_x[s], src_y[s], T);
+        NAME(rzo_calc_fa)(nx, ny, T, ntr, rx, ry, tmod_all + (size_t)s * ntr);
+        if (it_f) it_f[s] = itf;
+    }
+}

@@ -1,0 +1,709 @@
+// C ABI of include/rajzel_amd.h: context, device buffers, batching and the
+// per-evaluation launch sequence.  Kernels live in rz_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/rajzel_amd.h"
+#include "rz_internal.h"
+
+using namespace rz;
+
+namespace {
+
+struct ModelHost {
+    rz_model_desc d{};
+    int nb = 0;
+    long long bstride = 0;
+    std::vector<int> src_k, src_c, rec_k, rec_c, uq_k, uq_c, uq_r;
+    long long rec_off = 0;   // offset of this model's (nshots, ntr) blocks
+    int shot_base = 0;       // global index of shot 0
+    long long grad_off = 0;  // offset in the gradient buffer
+    int carry_off = 0;       // offset in the carry buffer
+    bool have_S = false, have_picks = false;
+    std::vector<float> tres0;
+    // device
+    float *S = nullptr, *SA = nullptr, *SB = nullptr;
+    int *rec_k_d = nullptr, *rec_c_d = nullptr, *uq_k_d = nullptr, *uq_c_d = nullptr, *uq_r_d = nullptr;
+};
+
+}  // namespace
+
+struct rz_ctx {
+    int device = 0;
+    hipStream_t own = nullptr, stream = nullptr;
+    int R = 4, nb_max = 1;
+    std::vector<ModelHost> m;
+    int total_shots = 0;
+    long long total_rec = 0, total_cells = 0;
+    int total_tr = 0;
+    // device buffers
+    ModelDev *md_d = nullptr;
+    ShotDev *shots_d = nullptr;
+    int *slot_of_d = nullptr;
+    long long *rec_off_d = nullptr, *rec_off0_d = nullptr;
+    int *shot_base_d = nullptr, *shot_base0_d = nullptr;
+    float *tobs_d = nullptr, *tmod_d = nullptr, *tnb_d = nullptr, *tres_d = nullptr, *carry_d = nullptr;
+    float *grad_d = nullptr;
+    double *l2_shot_d = nullptr, *l2_sum_d = nullptr;
+    int *it_f_d = nullptr, *it_a_d = nullptr;
+    int *err_d = nullptr;
+    // single-shot scratch
+    float *s_tmod = nullptr, *s_tnb = nullptr, *s_tres = nullptr, *s_row = nullptr, *s_row2 = nullptr;
+    int *s_it = nullptr;
+    ShotDev *s_shot = nullptr;
+    int *s_slot0 = nullptr;
+    // field slots
+    int nslots = 0;
+    long long slot_floats = 0;
+    float *TA = nullptr, *TB = nullptr, *LA = nullptr, *LB = nullptr;
+    // host mirror of the shot table
+    std::vector<ShotDev> shots_h;
+    int last_b0 = -1, last_b1 = -1;   // global shot range resident in the slots
+    bool have_grad = false, have_fwd = false;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+    std::string err;
+};
+
+namespace {
+
+int fail(rz_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail((ctx), RZ_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                 \
+    } while (0)
+
+template <typename T>
+int dalloc(rz_ctx *c, T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, RZ_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    }
+    c->allocs.push_back(q);
+    c->bytes += n * sizeof(T);
+    *p = static_cast<T *>(q);
+    return RZ_OK;
+}
+
+#define TRY(expr)                 \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != RZ_OK) return rc_; \
+    } while (0)
+
+int bands_for(int ny, int R) { return (ny + kWave * R - 1) / (kWave * R); }
+
+int choose_R(int ny_max)
+{
+    for (int R : {1, 2, 4, 8})
+        if (bands_for(ny_max, R) <= 4) return R;
+    return 8;
+}
+
+ModelDev make_dev(const rz_ctx *c, const ModelHost &h)
+{
+    ModelDev d{};
+    d.nx = h.d.nx; d.ny = h.d.ny; d.nb = h.nb; d.R = c->R;
+    d.dh = h.d.dh; d.time = h.d.time; d.ttnorm = h.d.ttnorm; d.eps_adj = h.d.eps_adj;
+    d.two_dh2 = 2.0 * ((double)h.d.dh * (double)h.d.dh);
+    d.bstride = h.bstride;
+    d.fstride = (long long)h.nb * h.bstride;
+    d.SA = h.SA; d.SB = h.SB; d.S = h.S;
+    d.ntr = h.d.ntr; d.nuq = (int)h.uq_k.size();
+    d.rec_k = h.rec_k_d; d.rec_c = h.rec_c_d;
+    d.uq_k = h.uq_k_d; d.uq_c = h.uq_c_d; d.uq_r = h.uq_r_d;
+    d.max_iter = h.d.max_iter > 0 ? h.d.max_iter : 1000;
+    return d;
+}
+
+int check_model(rz_ctx *c, int model)
+{
+    if (!c) return RZ_ERR_INVALID;
+    if (model < 0 || model >= (int)c->m.size()) return fail(c, RZ_ERR_INVALID, "model %d out of range", model);
+    HIPCHK(c, hipSetDevice(c->device));
+    return RZ_OK;
+}
+
+Slots slots_of(const rz_ctx *c)
+{
+    Slots s;
+    s.TA = c->TA; s.TB = c->TB; s.LA = c->LA; s.LB = c->LB;
+    s.slot_stride = c->slot_floats;
+    return s;
+}
+
+ShotOut outs_of(const rz_ctx *c)
+{
+    ShotOut o;
+    o.tmod = c->tmod_d; o.tnb = c->tnb_d; o.it_f = c->it_f_d; o.it_a = c->it_a_d;
+    return o;
+}
+
+int ready(rz_ctx *c, bool need_picks)
+{
+    for (size_t i = 0; i < c->m.size(); ++i) {
+        if (!c->m[i].have_S) return fail(c, RZ_ERR_STATE, "model %zu: slowness not set", i);
+        if (need_picks && !c->m[i].have_picks && c->m[i].d.nshots > 0 && c->m[i].d.ntr > 0)
+            return fail(c, RZ_ERR_STATE, "model %zu: picks not set", i);
+    }
+    return RZ_OK;
+}
+
+int check_convergence(rz_ctx *c, bool adj)
+{
+    std::vector<int> itf(c->total_shots), ita(c->total_shots);
+    if (c->total_shots == 0) return RZ_OK;
+    HIPCHK(c, hipMemcpyAsync(itf.data(), c->it_f_d, itf.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (adj)
+        HIPCHK(c, hipMemcpyAsync(ita.data(), c->it_a_d, ita.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->total_shots; ++i)
+        if (itf[i] < 0 || (adj && ita[i] < 0))
+            return fail(c, RZ_ERR_NOT_CONVERGED, "shot %d did not converge within max_iter", i);
+    return RZ_OK;
+}
+
+// Run forward (and, with grad, residual + adjoint + gradient) over the
+// global shot order in batches of at most `cap` shots.
+int run(rz_ctx *c, bool grad, int cap)
+{
+    TRY(ready(c, grad));
+    HIPCHK(c, hipSetDevice(c->device));
+    const Slots sl = slots_of(c);
+    const ShotOut so = outs_of(c);
+    if (grad) {
+        std::vector<float> carry0((size_t)std::max(c->total_tr, 1), 0.f);
+        for (auto &h : c->m)
+            if (!h.tres0.empty()) std::copy(h.tres0.begin(), h.tres0.end(), carry0.begin() + h.carry_off);
+        if (c->total_tr > 0) {
+            HIPCHK(c, hipMemcpyAsync(c->carry_d, carry0.data(), c->total_tr * sizeof(float), hipMemcpyHostToDevice,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));   // carry0 is a local
+        }
+        HIPCHK(c, hipMemsetAsync(c->l2_sum_d, 0, c->m.size() * sizeof(double), c->stream));
+        // models without shots still report a zero gradient
+        HIPCHK(c, hipMemsetAsync(c->grad_d, 0, (size_t)c->total_cells * sizeof(float), c->stream));
+    }
+    // Shot g lives in slot g % nslots; any batch of at most nslots
+    // consecutive shots therefore uses distinct slots.
+    const int per = std::max(1, std::min(cap, c->nslots));
+    for (int b0 = 0; b0 < c->total_shots; b0 += per) {
+        const int b1 = std::min(c->total_shots, b0 + per);
+        HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+                                 c->shot_base_d, so, c->stream));
+        if (grad) {
+            for (size_t mi = 0; mi < c->m.size(); ++mi) {
+                const ModelHost &h = c->m[mi];
+                const int s0 = std::max(b0, h.shot_base) - h.shot_base;
+                const int s1 = std::min(b1, h.shot_base + h.d.nshots) - h.shot_base;
+                if (s1 <= s0 || h.d.ntr <= 0) continue;
+                HIPCHK(c, launch_residual(c->md_d, (int)mi, h.d.ntr, s0, s1, h.rec_off, c->tmod_d, c->tnb_d,
+                                          c->tobs_d, c->carry_d + h.carry_off, c->tres_d, c->l2_shot_d + h.shot_base,
+                                          c->l2_sum_d + mi, c->stream));
+            }
+            HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+                                     c->shot_base_d, c->tres_d, so, c->stream));
+            for (size_t mi = 0; mi < c->m.size(); ++mi) {
+                const ModelHost &h = c->m[mi];
+                const int s0 = std::max(b0, h.shot_base) - h.shot_base;
+                const int s1 = std::min(b1, h.shot_base + h.d.nshots) - h.shot_base;
+                if (s1 <= s0) continue;
+                ModelDev mh = make_dev(c, h);
+                HIPCHK(c, launch_gradient(&mh, (int)mi, c->md_d, c->slot_of_d + h.shot_base + s0, s1 - s0, sl,
+                                          c->grad_d + h.grad_off, s0 > 0 ? 1 : 0, c->stream));
+            }
+        }
+        c->last_b0 = b0;
+        c->last_b1 = b1;
+    }
+    c->have_fwd = true;
+    c->have_grad = grad;
+    return check_convergence(c, grad);
+}
+
+int copy_out(rz_ctx *c, void *dst, const void *src, size_t bytes, int on_device)
+{
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rz_grid_index(float pos_m, float dh)
+{
+    const float q = pos_m / dh;
+    return (int)std::floor((double)q + 0.5) - 1;
+}
+
+static thread_local std::string g_create_err;
+
+const char *rz_last_error(const rz_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+void rz_destroy(rz_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+}
+
+int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models, int rows_per_lane,
+              size_t hbm_budget)
+{
+    if (!out) return RZ_ERR_INVALID;
+    *out = nullptr;
+    rz_ctx *c = new rz_ctx();
+    auto bail = [&](int rc) {
+        // rz_last_error(NULL) reports why creation failed
+        g_create_err = c->err;
+        rz_destroy(c);
+        return rc;
+    };
+    c->device = device;
+    if (nmodels <= 0 || !models) return bail(fail(c, RZ_ERR_INVALID, "need at least one model"));
+    int ny_max = 0;
+    for (int i = 0; i < nmodels; ++i) {
+        const rz_model_desc &d = models[i];
+        if (d.nx < 5 || d.ny < 5) return bail(fail(c, RZ_ERR_INVALID, "model %d: NX, NY must be >= 5", i));
+        if (!(d.dh > 0.f) || !std::isfinite(d.dh)) return bail(fail(c, RZ_ERR_INVALID, "model %d: bad DH", i));
+        if (!std::isfinite(d.time) || !std::isfinite(d.ttnorm) || !std::isfinite(d.eps_adj))
+            return bail(fail(c, RZ_ERR_INVALID, "model %d: TIME/TTNORM/EPS_ADJ must be finite", i));
+        if (d.nshots < 0 || d.ntr < 0 || d.max_iter < 0)
+            return bail(fail(c, RZ_ERR_INVALID, "model %d: negative count", i));
+        if ((long long)d.nx * d.ny > (1LL << 28))
+            return bail(fail(c, RZ_ERR_INVALID, "model %d: grid too large", i));
+        ny_max = std::max(ny_max, d.ny);
+    }
+    c->R = rows_per_lane ? rows_per_lane : choose_R(ny_max);
+    if (c->R != 1 && c->R != 2 && c->R != 4 && c->R != 8)
+        return bail(fail(c, RZ_ERR_INVALID, "rows_per_lane must be 1, 2, 4 or 8"));
+    if (bands_for(ny_max, c->R) > kMaxBands)
+        return bail(fail(c, RZ_ERR_INVALID, "NY=%d needs more than %d bands at %d rows per lane", ny_max, kMaxBands,
+                         c->R));
+
+    long long rec_off = 0, cells = 0;
+    int shots = 0, carry = 0;
+    long long field_max = 0;
+    for (int i = 0; i < nmodels; ++i) {
+        const rz_model_desc &d = models[i];
+        ModelHost h;
+        h.d = d;
+        h.d.src_ix = h.d.src_iy = h.d.rec_ix = h.d.rec_iy = nullptr;
+        h.nb = bands_for(d.ny, c->R);
+        h.bstride = (long long)(d.nx + 63) * kWave * c->R;
+        field_max = std::max(field_max, (long long)h.nb * h.bstride);
+        for (int s = 0; s < d.nshots; ++s) {
+            const int x = d.src_ix[s], y = d.src_iy[s];
+            if (x < 0 || x >= d.nx || y < 0 || y >= d.ny)
+                return bail(fail(c, RZ_ERR_INVALID, "model %d shot %d: source cell (%d,%d) outside the grid", i, s,
+                                 x, y));
+            h.src_c.push_back(x);
+            h.src_k.push_back(y);
+        }
+        std::unordered_map<long long, int> last;
+        for (int r = 0; r < d.ntr; ++r) {
+            const int x = d.rec_ix[r], y = d.rec_iy[r];
+            if (x < 0 || x >= d.nx || y < 0 || y >= d.ny)
+                return bail(fail(c, RZ_ERR_INVALID, "model %d receiver %d: cell (%d,%d) outside the grid", i, r,
+                                 x, y));
+            h.rec_c.push_back(x);
+            h.rec_k.push_back(y);
+            if (x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1) last[(long long)y * d.nx + x] = r;
+        }
+        // interior receiver cells, last receiver wins (init_travel_adj.c:49-55)
+        std::vector<std::pair<long long, int>> uq(last.begin(), last.end());
+        std::sort(uq.begin(), uq.end());
+        for (auto &p : uq) {
+            h.uq_k.push_back((int)(p.first / d.nx));
+            h.uq_c.push_back((int)(p.first % d.nx));
+            h.uq_r.push_back(p.second);
+        }
+        h.rec_off = rec_off;
+        h.shot_base = shots;
+        h.grad_off = cells;
+        h.carry_off = carry;
+        rec_off += (long long)d.nshots * d.ntr;
+        shots += d.nshots;
+        cells += (long long)d.nx * d.ny;
+        carry += d.ntr;
+        c->nb_max = std::max(c->nb_max, h.nb);
+        c->m.push_back(std::move(h));
+    }
+    c->total_shots = shots;
+    c->total_rec = rec_off;
+    c->total_cells = cells;
+    c->total_tr = carry;
+    c->slot_floats = (field_max + 63) / 64 * 64;
+
+    int rc;
+    if (hipSetDevice(device) != hipSuccess) return bail(fail(c, RZ_ERR_HIP, "hipSetDevice(%d) failed", device));
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(c, RZ_ERR_HIP, "hipStreamCreate failed"));
+    c->stream = c->own;
+
+    // per-model buffers
+    for (auto &h : c->m) {
+        const size_t n = (size_t)h.d.nx * h.d.ny, f = (size_t)h.nb * h.bstride;
+        if ((rc = dalloc(c, &h.S, n)) || (rc = dalloc(c, &h.SA, f)) || (rc = dalloc(c, &h.SB, f)) ||
+            (rc = dalloc(c, &h.rec_k_d, h.rec_k.size())) || (rc = dalloc(c, &h.rec_c_d, h.rec_c.size())) ||
+            (rc = dalloc(c, &h.uq_k_d, h.uq_k.size())) || (rc = dalloc(c, &h.uq_c_d, h.uq_c.size())) ||
+            (rc = dalloc(c, &h.uq_r_d, h.uq_r.size())))
+            return bail(rc);
+        auto up = [&](int *dst, const std::vector<int> &v) {
+            return v.empty() ? hipSuccess : hipMemcpy(dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice);
+        };
+        if (up(h.rec_k_d, h.rec_k) || up(h.rec_c_d, h.rec_c) || up(h.uq_k_d, h.uq_k) || up(h.uq_c_d, h.uq_c) ||
+            up(h.uq_r_d, h.uq_r))
+            return bail(fail(c, RZ_ERR_HIP, "geometry upload failed"));
+    }
+    const size_t nm = c->m.size();
+    if ((rc = dalloc(c, &c->md_d, nm)) || (rc = dalloc(c, &c->shots_d, shots)) ||
+        (rc = dalloc(c, &c->slot_of_d, shots)) || (rc = dalloc(c, &c->rec_off_d, nm)) ||
+        (rc = dalloc(c, &c->rec_off0_d, nm)) || (rc = dalloc(c, &c->shot_base_d, nm)) ||
+        (rc = dalloc(c, &c->shot_base0_d, nm)) || (rc = dalloc(c, &c->tobs_d, rec_off)) ||
+        (rc = dalloc(c, &c->tmod_d, rec_off)) || (rc = dalloc(c, &c->tnb_d, rec_off)) ||
+        (rc = dalloc(c, &c->tres_d, rec_off)) || (rc = dalloc(c, &c->carry_d, carry)) ||
+        (rc = dalloc(c, &c->grad_d, cells)) || (rc = dalloc(c, &c->l2_shot_d, shots)) ||
+        (rc = dalloc(c, &c->l2_sum_d, nm)) || (rc = dalloc(c, &c->it_f_d, shots)) ||
+        (rc = dalloc(c, &c->it_a_d, shots)) || (rc = dalloc(c, &c->err_d, 1)))
+        return bail(rc);
+    int ntr_max = 0, ncell_max = 0;
+    for (auto &h : c->m) {
+        ntr_max = std::max(ntr_max, h.d.ntr);
+        ncell_max = std::max(ncell_max, h.d.nx * h.d.ny);
+    }
+    if ((rc = dalloc(c, &c->s_tmod, ntr_max)) || (rc = dalloc(c, &c->s_tnb, ntr_max)) ||
+        (rc = dalloc(c, &c->s_tres, ntr_max)) || (rc = dalloc(c, &c->s_row, ncell_max)) ||
+        (rc = dalloc(c, &c->s_row2, ncell_max)) || (rc = dalloc(c, &c->s_it, 2)) ||
+        (rc = dalloc(c, &c->s_shot, 1)) || (rc = dalloc(c, &c->s_slot0, 1)))
+        return bail(rc);
+
+    // field slots: all shots, or as many as the budget allows
+    const size_t per_slot = (size_t)c->slot_floats * 4 * sizeof(float);
+    int nslots = std::max(1, shots);
+    if (hbm_budget > 0) {
+        const size_t fit = hbm_budget / per_slot;
+        if (fit < 1) return bail(fail(c, RZ_ERR_NOMEM, "HBM budget below one shot's fields (%zu bytes)", per_slot));
+        nslots = (int)std::min<size_t>((size_t)nslots, fit);
+    }
+    c->nslots = nslots;
+    const size_t sf = (size_t)c->slot_floats * nslots;
+    if ((rc = dalloc(c, &c->TA, sf)) || (rc = dalloc(c, &c->TB, sf)) || (rc = dalloc(c, &c->LA, sf)) ||
+        (rc = dalloc(c, &c->LB, sf)))
+        return bail(rc);
+
+    // tables
+    std::vector<ModelDev> mdh;
+    std::vector<long long> ro, ro0(nm, 0);
+    std::vector<int> sb, sb0(nm, 0);
+    for (size_t i = 0; i < nm; ++i) {
+        const ModelHost &h = c->m[i];
+        mdh.push_back(make_dev(c, h));
+        ro.push_back(h.rec_off);
+        sb.push_back(h.shot_base);
+        for (int s = 0; s < h.d.nshots; ++s) {
+            ShotDev sd{};
+            sd.model = (int)i;
+            sd.shot = s;
+            sd.src_k = h.src_k[s];
+            sd.src_c = h.src_c[s];
+            sd.slot = (h.shot_base + s) % nslots;
+            c->shots_h.push_back(sd);
+        }
+    }
+    std::vector<int> slot_of(shots);
+    for (int g = 0; g < shots; ++g) slot_of[g] = c->shots_h[g].slot;
+    const int zero = 0;
+    ShotDev s1{};
+    if (hipMemcpy(c->md_d, mdh.data(), nm * sizeof(ModelDev), hipMemcpyHostToDevice) ||
+        (shots && hipMemcpy(c->shots_d, c->shots_h.data(), shots * sizeof(ShotDev), hipMemcpyHostToDevice)) ||
+        (shots && hipMemcpy(c->slot_of_d, slot_of.data(), shots * sizeof(int), hipMemcpyHostToDevice)) ||
+        hipMemcpy(c->rec_off_d, ro.data(), nm * sizeof(long long), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->rec_off0_d, ro0.data(), nm * sizeof(long long), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->shot_base_d, sb.data(), nm * sizeof(int), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->shot_base0_d, sb0.data(), nm * sizeof(int), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->s_slot0, &zero, sizeof(int), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->s_shot, &s1, sizeof(ShotDev), hipMemcpyHostToDevice))
+        return bail(fail(c, RZ_ERR_HIP, "table upload failed"));
+    *out = c;
+    return RZ_OK;
+}
+
+int rz_set_stream(rz_ctx *c, void *s)
+{
+    if (!c) return RZ_ERR_INVALID;
+    c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    return RZ_OK;
+}
+
+int rz_set_slowness(rz_ctx *c, int model, const float *S, int on_device)
+{
+    TRY(check_model(c, model));
+    if (!S) return fail(c, RZ_ERR_INVALID, "null slowness");
+    ModelHost &h = c->m[model];
+    const size_t n = (size_t)h.d.nx * h.d.ny;
+    HIPCHK(c, hipMemcpyAsync(h.S, S, n * sizeof(float), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemsetAsync(c->err_d, 0, sizeof(int), c->stream));
+    ModelDev mh = make_dev(c, h);
+    HIPCHK(c, launch_prepare_slowness(&mh, model, c->md_d, h.SA, h.SB, c->err_d, c->stream));
+    int bad = 0;
+    HIPCHK(c, hipMemcpyAsync(&bad, c->err_d, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bad) {
+        h.have_S = false;
+        return fail(c, RZ_ERR_INVALID, "model %d: slowness must be finite and > 0 everywhere", model);
+    }
+    h.have_S = true;
+    return RZ_OK;
+}
+
+int rz_set_picks(rz_ctx *c, int model, const float *tobs, int on_device)
+{
+    TRY(check_model(c, model));
+    ModelHost &h = c->m[model];
+    const size_t n = (size_t)h.d.nshots * h.d.ntr;
+    if (n == 0) { h.have_picks = true; return RZ_OK; }
+    if (!tobs) return fail(c, RZ_ERR_INVALID, "null picks");
+    HIPCHK(c, hipMemcpyAsync(c->tobs_d + h.rec_off, tobs, n * sizeof(float),
+                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    h.have_picks = true;
+    return RZ_OK;
+}
+
+int rz_set_tres_carry(rz_ctx *c, int model, const float *tres0)
+{
+    TRY(check_model(c, model));
+    ModelHost &h = c->m[model];
+    if (!tres0) { h.tres0.clear(); return RZ_OK; }
+    h.tres0.assign(tres0, tres0 + h.d.ntr);
+    return RZ_OK;
+}
+
+int rz_forward(rz_ctx *c)
+{
+    if (!c) return RZ_ERR_INVALID;
+    return run(c, false, c->nslots);
+}
+
+int rz_gradient(rz_ctx *c)
+{
+    if (!c) return RZ_ERR_INVALID;
+    return run(c, true, c->nslots);
+}
+
+int rz_gradient_per_shot(rz_ctx *c)
+{
+    if (!c) return RZ_ERR_INVALID;
+    return run(c, true, 1);
+}
+
+int rz_get_tmod(rz_ctx *c, int model, float *out, int on_device)
+{
+    TRY(check_model(c, model));
+    if (!c->have_fwd) return fail(c, RZ_ERR_STATE, "no forward has run");
+    const ModelHost &h = c->m[model];
+    return copy_out(c, out, c->tmod_d + h.rec_off, (size_t)h.d.nshots * h.d.ntr * sizeof(float), on_device);
+}
+
+int rz_get_tres(rz_ctx *c, int model, float *out, int on_device)
+{
+    TRY(check_model(c, model));
+    if (!c->have_grad) return fail(c, RZ_ERR_STATE, "no gradient has run");
+    const ModelHost &h = c->m[model];
+    return copy_out(c, out, c->tres_d + h.rec_off, (size_t)h.d.nshots * h.d.ntr * sizeof(float), on_device);
+}
+
+int rz_get_grad(rz_ctx *c, int model, float *out, int on_device)
+{
+    TRY(check_model(c, model));
+    if (!c->have_grad) return fail(c, RZ_ERR_STATE, "no gradient has run");
+    const ModelHost &h = c->m[model];
+    return copy_out(c, out, c->grad_d + h.grad_off, (size_t)h.d.nx * h.d.ny * sizeof(float), on_device);
+}
+
+int rz_get_misfit(rz_ctx *c, int model, double *L2, double *l2_shot)
+{
+    TRY(check_model(c, model));
+    if (!c->have_grad) return fail(c, RZ_ERR_STATE, "no gradient has run");
+    const ModelHost &h = c->m[model];
+    double sum = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&sum, c->l2_sum_d + model, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (l2_shot && h.d.nshots)
+        HIPCHK(c, hipMemcpyAsync(l2_shot, c->l2_shot_d + h.shot_base, h.d.nshots * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (L2) *L2 = 0.5 * sum;   // grad_obj.c:61
+    return RZ_OK;
+}
+
+int rz_get_iterations(rz_ctx *c, int model, int *it_f, int *it_a)
+{
+    TRY(check_model(c, model));
+    const ModelHost &h = c->m[model];
+    if (it_f && h.d.nshots) TRY(copy_out(c, it_f, c->it_f_d + h.shot_base, h.d.nshots * sizeof(int), 0));
+    if (it_a && h.d.nshots) TRY(copy_out(c, it_a, c->it_a_d + h.shot_base, h.d.nshots * sizeof(int), 0));
+    return RZ_OK;
+}
+
+int rz_get_tres_carry(rz_ctx *c, int model, float *out)
+{
+    TRY(check_model(c, model));
+    const ModelHost &h = c->m[model];
+    if (h.d.ntr == 0) return RZ_OK;
+    return copy_out(c, out, c->carry_d + h.carry_off, h.d.ntr * sizeof(float), 0);
+}
+
+int rz_get_field(rz_ctx *c, int model, int shot, int which, float *out, int on_device)
+{
+    TRY(check_model(c, model));
+    const ModelHost &h = c->m[model];
+    if (shot < 0 || shot >= h.d.nshots) return fail(c, RZ_ERR_INVALID, "shot out of range");
+    if (which != 0 && which != 1) return fail(c, RZ_ERR_INVALID, "which must be 0 (T) or 1 (lambda)");
+    if (which == 1 && !c->have_grad) return fail(c, RZ_ERR_STATE, "no gradient has run");
+    const int g = h.shot_base + shot;
+    if (g < c->last_b0 || g >= c->last_b1)
+        return fail(c, RZ_ERR_STATE, "fields of shot %d are not resident (HBM budget batching)", shot);
+    const int slot = g % c->nslots;
+    ModelDev mh = make_dev(c, h);
+    const float *f = (which == 0 ? c->TA : c->LA) + (long long)slot * c->slot_floats;
+    const size_t n = (size_t)h.d.nx * h.d.ny;
+    HIPCHK(c, launch_field_to_rowmajor(&mh, f, c->s_row, which == 0 ? 1 : 0, c->stream));
+    return copy_out(c, out, c->s_row, n * sizeof(float), on_device);
+}
+
+int rz_eikonal_shot(rz_ctx *c, int model, int src_ix, int src_iy, float *T, float *tmod, int *iters)
+{
+    TRY(check_model(c, model));
+    ModelHost &h = c->m[model];
+    if (!h.have_S) return fail(c, RZ_ERR_STATE, "slowness not set");
+    if (src_ix < 0 || src_ix >= h.d.nx || src_iy < 0 || src_iy >= h.d.ny)
+        return fail(c, RZ_ERR_INVALID, "source cell (%d,%d) outside the grid", src_ix, src_iy);
+    ShotDev sd{};
+    sd.model = model; sd.shot = 0; sd.src_k = src_iy; sd.src_c = src_ix; sd.slot = 0;
+    HIPCHK(c, hipMemcpyAsync(c->s_shot, &sd, sizeof sd, hipMemcpyHostToDevice, c->stream));
+    ShotOut o;
+    o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1;
+    HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d, o,
+                             c->stream));
+    ModelDev mh = make_dev(c, h);
+    HIPCHK(c, launch_field_to_rowmajor(&mh, c->TA, c->s_row, 1, c->stream));
+    int it = 0;
+    const size_t n = (size_t)h.d.nx * h.d.ny;
+    if (T) HIPCHK(c, hipMemcpyAsync(T, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (tmod && h.d.ntr)
+        HIPCHK(c, hipMemcpyAsync(tmod, c->s_tmod, h.d.ntr * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&it, c->s_it, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->last_b0 = c->last_b1 = -1;
+    if (iters) *iters = it;
+    if (it < 0) return fail(c, RZ_ERR_NOT_CONVERGED, "forward did not converge within max_iter");
+    return RZ_OK;
+}
+
+int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres, float *lam, int *iters)
+{
+    TRY(check_model(c, model));
+    ModelHost &h = c->m[model];
+    if (!T) return fail(c, RZ_ERR_INVALID, "null T");
+    const size_t n = (size_t)h.d.nx * h.d.ny;
+    for (size_t i = 0; i < n; ++i)
+        if (!(T[i] >= 0.f)) return fail(c, RZ_ERR_INVALID, "T must be >= 0 (cell %zu)", i);
+    ModelDev mh = make_dev(c, h);
+    HIPCHK(c, hipMemcpyAsync(c->s_row, T, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    // receiver cells carry the sign bit in the adjoint's copy of T
+    std::vector<float> marked(T, T + n);
+    for (size_t u = 0; u < h.uq_k.size(); ++u) {
+        float &v = marked[(size_t)h.uq_k[u] * h.d.nx + h.uq_c[u]];
+        v = -v;
+        if (v == 0.f) v = -0.f;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->s_row2, marked.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_rowmajor_to_field(&mh, c->s_row2, c->TA, 0, c->stream));
+    HIPCHK(c, launch_rowmajor_to_field(&mh, c->s_row2, c->TB, 1, c->stream));
+    if (h.d.ntr)
+        HIPCHK(c, hipMemcpyAsync(c->s_tres, tres, h.d.ntr * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    ShotDev sd{};
+    sd.model = model; sd.slot = 0;
+    HIPCHK(c, hipMemcpyAsync(c->s_shot, &sd, sizeof sd, hipMemcpyHostToDevice, c->stream));
+    ShotOut o;
+    o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1;
+    HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d,
+                             c->s_tres, o, c->stream));
+    HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA, c->s_row, 0, c->stream));
+    int it = 0;
+    if (lam) HIPCHK(c, hipMemcpyAsync(lam, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&it, c->s_it + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->last_b0 = c->last_b1 = -1;
+    if (iters) *iters = it;
+    if (it < 0) return fail(c, RZ_ERR_NOT_CONVERGED, "adjoint did not converge within max_iter");
+    return RZ_OK;
+}
+
+int rz_ass_grad_shot(rz_ctx *c, int model, const float *lam, float *grad_shot, float *grad)
+{
+    TRY(check_model(c, model));
+    ModelHost &h = c->m[model];
+    if (!h.have_S) return fail(c, RZ_ERR_STATE, "slowness not set");
+    if (!lam) return fail(c, RZ_ERR_INVALID, "null lambda");
+    const size_t n = (size_t)h.d.nx * h.d.ny;
+    ModelDev mh = make_dev(c, h);
+    HIPCHK(c, hipMemcpyAsync(c->s_row, lam, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_rowmajor_to_field(&mh, c->s_row, c->LA, 0, c->stream));
+    HIPCHK(c, launch_gradient(&mh, model, c->md_d, c->s_slot0, 1, slots_of(c), c->s_row2, 0, c->stream));
+    std::vector<float> gs(n);
+    HIPCHK(c, hipMemcpyAsync(gs.data(), c->s_row2, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->last_b0 = c->last_b1 = -1;
+    if (grad_shot) std::memcpy(grad_shot, gs.data(), n * sizeof(float));
+    if (grad)
+        for (size_t i = 0; i < n; ++i) grad[i] += gs[i];   // ass_grad.c:40-46
+    return RZ_OK;
+}
+
+int rz_sync(rz_ctx *c)
+{
+    if (!c) return RZ_ERR_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RZ_OK;
+}
+
+int rz_rows_per_lane(const rz_ctx *c) { return c ? c->R : 0; }
+size_t rz_device_bytes(const rz_ctx *c) { return c ? c->bytes : 0; }
+int rz_batch_shots(const rz_ctx *c) { return c ? c->nslots : 0; }
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+//
+// Band-skewed field layout (one per shot field, and for the slowness):
+//   rows are split into bands of 64*R rows; band b holds rows [64Rb, 64R(b+1)).
+//   Inside a band, lane l (0..63) owns R consecutive rows 64Rb + lR + r.
+//   Family A stores cell (k, c) of lane l at key  c + l,
+//   family B stores it at key                     c - l + 63,
+//   address = b*bstride + key*64R + l*R + r,  bstride = (NX+63)*64R floats.
+// A sweep along (x+,y+) or (x-,y-) visits, at each step, the cells of one
+// A-key (one anti-diagonal of the band), a sweep along (x+,y-) or (x-,y+)
+// those of one B-key; each step of a wavefront therefore reads and writes
+// 64*R contiguous floats.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rz {
+
+constexpr int kWave = 64;
+constexpr int kPF = 8;                // prefetch depth == steps between barriers
+constexpr int kLag = kPF;             // extra lag between consecutive bands (>= kPF-1)
+constexpr int kBandOff = kWave + kLag;// step offset between consecutive bands
+constexpr int kRingQ = 32;            // LDS ring slots per band (>= kLag+1+kPF, power of 2)
+constexpr int kMaxBands = 8;          // waves per workgroup (<= 512 threads)
+constexpr int kMaxThreads = kMaxBands * kWave;
+
+// INF of the reference (fd.h:29-31): an int compared against float fields.
+constexpr float kInf = 9999999.0f;
+
+struct ModelDev {
+    int nx, ny, nb, R;
+    float dh, time, ttnorm, eps_adj;
+    double two_dh2;                // 2*DH^2 (exact in double)
+    long long bstride;             // floats per band
+    long long fstride;             // floats per field (nb * bstride)
+    const float *SA, *SB;          // slowness in layouts A and B
+    const float *S;                // slowness row-major (ny, nx)
+    int ntr, nuq;
+    const int *rec_k, *rec_c;      // receivers, 0-based
+    const int *uq_k, *uq_c, *uq_r; // interior receiver cells and their winning receiver
+    int max_iter;
+    int pad_;
+};
+
+struct ShotDev {
+    int model;          // index into models
+    int shot;           // shot index within the model
+    int src_k, src_c;   // 0-based source cell
+    int slot;           // field slot
+    int pad_;
+};
+
+struct Slots {
+    float *TA, *TB, *LA, *LB;      // base of slot 0; slot s at + s*slot_stride
+    long long slot_stride;         // floats
+};
+
+// Per-shot outputs.  tmod/tnb are (nshots, ntr) blocks packed per model at
+// rec_off[model]; it_f/it_a are indexed shot_base[model] + shot.
+struct ShotOut {
+    float *tmod;     // T at each receiver (calc_FA.c:15-17)
+    float *tnb;      // T at the receiver's slope neighbour row (calc_FA_res.c:52-69)
+    int *it_f, *it_a;
+};
+
+// Launchers (rz_kernels.hip).  All return hipError_t of the launch.
+hipError_t launch_prepare_slowness(const ModelDev *md_host, int model, const ModelDev *md_dev, float *SA,
+                                   float *SB, int *err_flag, hipStream_t st);
+hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
+                          const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st);
+hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
+                          const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
+                          hipStream_t st);
+hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s1, long long rec_off,
+                           const float *tmod, const float *tnb, const float *tobs, float *carry, float *tres,
+                           double *l2_shot, double *l2_sum, hipStream_t st);
+hipError_t launch_gradient(const ModelDev *md_host, int model, const ModelDev *md_dev, const int *slot_of,
+                           int nsh, Slots slots, float *grad, int accumulate, hipStream_t st);
+hipError_t launch_field_to_rowmajor(const ModelDev *md_host, const float *field, float *out, int strip,
+                                    hipStream_t st);
+hipError_t launch_rowmajor_to_field(const ModelDev *md_host, const float *in, float *field, int family,
+                                    hipStream_t st);
+
+}  // namespace rz

@@ -1,0 +1,284 @@
+"""ctypes binding of librajzel_amd.so (include/rajzel_amd.h).
+
+This is the only way Python reaches the GPU path: there is no fallback.  If
+the library is missing, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librajzel_amd.so")
+
+RZ_OK, RZ_ERR_INVALID, RZ_ERR_HIP, RZ_ERR_NOMEM, RZ_ERR_NOT_CONVERGED, RZ_ERR_STATE = range(6)
+
+# Every symbol declared in include/rajzel_amd.h.
+EXPORTS = (
+    "rz_grid_index", "rz_create", "rz_destroy", "rz_last_error", "rz_set_stream", "rz_set_slowness",
+    "rz_set_picks", "rz_set_tres_carry", "rz_forward", "rz_gradient", "rz_gradient_per_shot",
+    "rz_get_tmod", "rz_get_tres", "rz_get_grad", "rz_get_misfit", "rz_get_iterations",
+    "rz_get_tres_carry", "rz_get_field", "rz_eikonal_shot", "rz_eikonal_adj_shot", "rz_ass_grad_shot",
+    "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots",
+)
+
+
+class RzError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rajzel_amd error {code}: {msg}")
+        self.code = code
+
+
+class NotConverged(RzError):
+    pass
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("dh", C.c_float), ("time", C.c_float),
+        ("ttnorm", C.c_float), ("eps_adj", C.c_float), ("max_iter", C.c_int), ("nshots", C.c_int),
+        ("src_ix", C.POINTER(C.c_int)), ("src_iy", C.POINTER(C.c_int)), ("ntr", C.c_int),
+        ("rec_ix", C.POINTER(C.c_int)), ("rec_iy", C.POINTER(C.c_int)),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RzError(-1, f"{LIB_PATH} not built: run __graft_entry__.build() or "
+                                  "python rajzel_amd/build.py")
+            L = C.CDLL(LIB_PATH)
+            vp, i, f, d = C.c_void_p, C.c_int, C.c_float, C.c_double
+            sig = {
+                "rz_grid_index": (i, [f, f]),
+                "rz_create": (i, [C.POINTER(vp), i, i, C.POINTER(ModelDesc), i, C.c_size_t]),
+                "rz_destroy": (None, [vp]),
+                "rz_last_error": (C.c_char_p, [vp]),
+                "rz_set_stream": (i, [vp, vp]),
+                "rz_set_slowness": (i, [vp, i, vp, i]),
+                "rz_set_picks": (i, [vp, i, vp, i]),
+                "rz_set_tres_carry": (i, [vp, i, vp]),
+                "rz_forward": (i, [vp]),
+                "rz_gradient": (i, [vp]),
+                "rz_gradient_per_shot": (i, [vp]),
+                "rz_get_tmod": (i, [vp, i, vp, i]),
+                "rz_get_tres": (i, [vp, i, vp, i]),
+                "rz_get_grad": (i, [vp, i, vp, i]),
+                "rz_get_misfit": (i, [vp, i, C.POINTER(d), vp]),
+                "rz_get_iterations": (i, [vp, i, vp, vp]),
+                "rz_get_tres_carry": (i, [vp, i, vp]),
+                "rz_get_field": (i, [vp, i, i, i, vp, i]),
+                "rz_eikonal_shot": (i, [vp, i, i, i, vp, vp, C.POINTER(i)]),
+                "rz_eikonal_adj_shot": (i, [vp, i, vp, vp, vp, C.POINTER(i)]),
+                "rz_ass_grad_shot": (i, [vp, i, vp, vp, vp]),
+                "rz_sync": (i, [vp]),
+                "rz_rows_per_lane": (i, [vp]),
+                "rz_device_bytes": (C.c_size_t, [vp]),
+                "rz_batch_shots": (i, [vp]),
+            }
+            for name, (res, args) in sig.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+        return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def grid_index(pos_m, dh):
+    return lib().rz_grid_index(float(pos_m), float(dh))
+
+
+class Context:
+    """One context over a list of models (dicts or Survey-like objects).
+
+    Each model needs nx, ny, dh, src_ix, src_iy, rec_ix, rec_iy (0-based) and
+    optionally time, ttnorm, eps_adj, max_iter.
+    """
+
+    def __init__(self, models, device=0, rows_per_lane=0, hbm_budget=0):
+        L = lib()
+        self._keep = []
+        descs = (ModelDesc * len(models))()
+        self.shapes = []
+        for k, m in enumerate(models):
+            g = (lambda n, dflt=None: getattr(m, n, m.get(n, dflt) if isinstance(m, dict) else dflt))
+            sx = np.ascontiguousarray(g("src_ix"), np.int32)
+            sy = np.ascontiguousarray(g("src_iy"), np.int32)
+            rx = np.ascontiguousarray(g("rec_ix"), np.int32)
+            ry = np.ascontiguousarray(g("rec_iy"), np.int32)
+            self._keep += [sx, sy, rx, ry]
+            d = descs[k]
+            d.nx, d.ny, d.dh = int(g("nx")), int(g("ny")), float(g("dh"))
+            d.time = float(g("time", 6.0))
+            d.ttnorm = float(g("ttnorm", 1e-10))
+            d.eps_adj = float(g("eps_adj", 1e-9))
+            d.max_iter = int(g("max_iter", 0) or 0)
+            d.nshots, d.ntr = len(sx), len(rx)
+            ip = C.POINTER(C.c_int)
+            d.src_ix, d.src_iy = sx.ctypes.data_as(ip), sy.ctypes.data_as(ip)
+            d.rec_ix, d.rec_iy = rx.ctypes.data_as(ip), ry.ctypes.data_as(ip)
+            self.shapes.append((d.ny, d.nx, d.nshots, d.ntr))
+        h = C.c_void_p()
+        rc = L.rz_create(C.byref(h), device, len(models), descs, rows_per_lane, hbm_budget)
+        if rc != RZ_OK:
+            raise RzError(rc, L.rz_last_error(None).decode())
+        self.h = h
+        self._keep = None
+
+    # -- lifecycle -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rz_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != RZ_OK:
+            msg = lib().rz_last_error(self.h).decode()
+            raise (NotConverged if rc == RZ_ERR_NOT_CONVERGED else RzError)(rc, msg)
+
+    # -- inputs --------------------------------------------------------------
+    def set_stream(self, stream_handle):
+        self._chk(lib().rz_set_stream(self.h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    def set_slowness(self, model, S):
+        """S: numpy (ny, nx) float32, or a device pointer (int)."""
+        if isinstance(S, int):
+            self._chk(lib().rz_set_slowness(self.h, model, C.c_void_p(S), 1))
+            return
+        ny, nx = self.shapes[model][:2]
+        S = np.ascontiguousarray(S, np.float32)
+        assert S.shape == (ny, nx), (S.shape, (ny, nx))
+        self._chk(lib().rz_set_slowness(self.h, model, _ptr(S), 0))
+
+    def set_picks(self, model, tobs):
+        if isinstance(tobs, int):
+            self._chk(lib().rz_set_picks(self.h, model, C.c_void_p(tobs), 1))
+            return
+        ns, ntr = self.shapes[model][2:]
+        tobs = np.ascontiguousarray(tobs, np.float32).reshape(ns, ntr)
+        self._chk(lib().rz_set_picks(self.h, model, _ptr(tobs), 0))
+
+    def set_tres_carry(self, model, tres0):
+        t = None if tres0 is None else np.ascontiguousarray(tres0, np.float32)
+        self._chk(lib().rz_set_tres_carry(self.h, model, _ptr(t)))
+
+    # -- runs ----------------------------------------------------------------
+    def forward(self):
+        self._chk(lib().rz_forward(self.h))
+
+    def gradient(self, per_shot=False):
+        self._chk((lib().rz_gradient_per_shot if per_shot else lib().rz_gradient)(self.h))
+
+    def sync(self):
+        self._chk(lib().rz_sync(self.h))
+
+    # -- outputs -------------------------------------------------------------
+    def tmod(self, model=0):
+        ny, nx, ns, ntr = self.shapes[model]
+        out = np.empty((ns, ntr), np.float32)
+        self._chk(lib().rz_get_tmod(self.h, model, _ptr(out), 0))
+        return out
+
+    def tres(self, model=0):
+        ny, nx, ns, ntr = self.shapes[model]
+        out = np.empty((ns, ntr), np.float32)
+        self._chk(lib().rz_get_tres(self.h, model, _ptr(out), 0))
+        return out
+
+    def grad(self, model=0, device_ptr=None):
+        ny, nx = self.shapes[model][:2]
+        if device_ptr is not None:
+            self._chk(lib().rz_get_grad(self.h, model, C.c_void_p(device_ptr), 1))
+            return None
+        out = np.empty((ny, nx), np.float32)
+        self._chk(lib().rz_get_grad(self.h, model, _ptr(out), 0))
+        return out
+
+    def misfit(self, model=0):
+        ns = self.shapes[model][2]
+        L2 = C.c_double()
+        per = np.empty(max(ns, 1), np.float64)
+        self._chk(lib().rz_get_misfit(self.h, model, C.byref(L2), _ptr(per)))
+        return L2.value, per[:ns]
+
+    def iterations(self, model=0):
+        ns = self.shapes[model][2]
+        f = np.empty(max(ns, 1), np.int32)
+        a = np.empty(max(ns, 1), np.int32)
+        self._chk(lib().rz_get_iterations(self.h, model, _ptr(f), _ptr(a)))
+        return f[:ns], a[:ns]
+
+    def tres_carry(self, model=0):
+        ntr = self.shapes[model][3]
+        out = np.empty(max(ntr, 1), np.float32)
+        self._chk(lib().rz_get_tres_carry(self.h, model, _ptr(out)))
+        return out[:ntr]
+
+    def field(self, model, shot, which):
+        ny, nx = self.shapes[model][:2]
+        out = np.empty((ny, nx), np.float32)
+        self._chk(lib().rz_get_field(self.h, model, shot, 0 if which in (0, "T") else 1, _ptr(out), 0))
+        return out
+
+    # -- single-shot reference-surface entry points --------------------------
+    def eikonal_shot(self, model, src_ix, src_iy):
+        ny, nx, ns, ntr = self.shapes[model]
+        T = np.empty((ny, nx), np.float32)
+        tm = np.empty(max(ntr, 1), np.float32)
+        it = C.c_int()
+        self._chk(lib().rz_eikonal_shot(self.h, model, int(src_ix), int(src_iy), _ptr(T), _ptr(tm), C.byref(it)))
+        return T, tm[:ntr], it.value
+
+    def eikonal_adj_shot(self, model, T, tres):
+        ny, nx, ns, ntr = self.shapes[model]
+        T = np.ascontiguousarray(T, np.float32)
+        tr = np.ascontiguousarray(tres, np.float32)
+        lam = np.empty((ny, nx), np.float32)
+        it = C.c_int()
+        self._chk(lib().rz_eikonal_adj_shot(self.h, model, _ptr(T), _ptr(tr), _ptr(lam), C.byref(it)))
+        return lam, it.value
+
+    def ass_grad_shot(self, model, lam, grad):
+        lam = np.ascontiguousarray(lam, np.float32)
+        g = np.ascontiguousarray(grad, np.float32).copy()
+        gs = np.empty_like(lam)
+        self._chk(lib().rz_ass_grad_shot(self.h, model, _ptr(lam), _ptr(gs), _ptr(g)))
+        return gs, g
+
+    @property
+    def rows_per_lane(self):
+        return lib().rz_rows_per_lane(self.h)
+
+    @property
+    def device_bytes(self):
+        return lib().rz_device_bytes(self.h)
+
+    @property
+    def batch_shots(self):
+        return lib().rz_batch_shots(self.h)
