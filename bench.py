@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Throughput of the batched FATT gradient evaluation on one MI355X.
+
+A step is one objective + gradient evaluation (the reference's grad_obj):
+for every shot the Eikonal forward, residuals, the adjoint and the gradient
+assembly, summed over shots.  Workload: 1001 x 3001 grid (DH = 5 m), 200
+surface shots, 1500 receivers at row 2; current model = 12-layer model,
+picks = times in the model with 20 Gaussian anomalies + N(0, 1 ms) noise
+(synthetic, seeded).  Inputs are resident in HBM before the timed region.
+
+Printed: one JSON line with the shot-gradients/s of the batched launch,
+the plain one-shot-per-launch form on the same GPU, the HBM roofline of the
+dominant kernel (phase times from HIP events on the work stream) and the
+reference CPU path timed on a bounded sample of the same evaluation.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "shot-gradients/s on 1001x3001 grid: batched vs per-shot GPU; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def model_dict(sv):
+    return dict(nx=sv.nx, ny=sv.ny, dh=sv.dh, time=sv.time, ttnorm=sv.ttnorm, eps_adj=sv.eps_adj,
+                src_ix=sv.src_ix, src_iy=sv.src_iy, rec_ix=sv.rec_ix, rec_iy=sv.rec_iy)
+
+
+def subset(sv, n):
+    from rajzel_amd import workloads as W
+    s = W.Survey(name=sv.name, nx=sv.nx, ny=sv.ny, dh=sv.dh, vp=sv.vp, src_x_m=sv.src_x_m[:n],
+                 src_z_m=sv.src_z_m[:n], rec_x_m=sv.rec_x_m, rec_z_m=sv.rec_z_m, time=sv.time,
+                 ttnorm=sv.ttnorm, eps_adj=sv.eps_adj)
+    s.tobs = None if sv.tobs is None else sv.tobs[:n]
+    return s
+
+
+def algorithmic_bytes(nx, ny, it_f, it_a):
+    """Bytes per launch by phase: 12 B per cell per directional sweep (read S
+    or T, read and write the swept field), 4 sweeps per outer iteration, 4 B
+    per cell to initialise T and lambda, 8 B per cell per shot for the
+    gradient (read lambda and S)."""
+    cells = float(nx) * float(ny)
+    fwd = cells * float(np.sum(48 * np.asarray(it_f, np.float64) + 4))
+    adj = cells * float(np.sum(48 * np.asarray(it_a, np.float64) + 4))
+    grd = cells * 8.0 * len(it_f)
+    return fwd, adj, grd
+
+
+def cpu_baseline(sv, S, tobs, nsample, gpu_tmod, gpu_tres, gpu_itf, gpu_ita):
+    """Reference C path (oracle/_ref) on the first nsample shots, one thread."""
+    from oracle.bindings import Reference, Restatement
+    kind = "reference"
+    try:
+        ref = Reference()
+        ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+        t0 = time.perf_counter()
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            out = ref.grad_obj(S, sv.src_x_m[:nsample], sv.src_z_m[:nsample], sv.rec_ix, sv.rec_iy,
+                               tobs[:nsample], workdir=d)
+        dt = time.perf_counter() - t0
+    except OSError as e:  # the reference build needs libmpi at load time
+        log(f"reference CPU build unavailable ({e}); timing the restatement instead")
+        kind = "port"
+        ora = Restatement()
+        t0 = time.perf_counter()
+        out = ora.grad_obj(S, sv.src_ix[:nsample], sv.src_iy[:nsample], sv.rec_ix, sv.rec_iy,
+                           tobs[:nsample], sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+        dt = time.perf_counter() - t0
+    parity = {
+        "shots": nsample,
+        "tmod_bitwise": bool(np.array_equal(out["tmod"].view(np.uint32), gpu_tmod[:nsample].view(np.uint32))),
+        "tres_bitwise": bool(np.array_equal(out["tres"].view(np.uint32), gpu_tres[:nsample].view(np.uint32))),
+        "iterations_equal": bool(np.array_equal(out["it_f"], gpu_itf[:nsample])
+                                 and np.array_equal(out["it_a"], gpu_ita[:nsample])),
+        "ref_it_f": out["it_f"].tolist(), "ref_it_a": out["it_a"].tolist(),
+    }
+    return {
+        "value": nsample / dt, "unit": "shot-gradients/s", "cores": 1, "kind": kind,
+        "sample": f"first {nsample} of {sv.nshots} shots of the same evaluation "
+                  f"({sv.ny}x{sv.nx}, {sv.ntr} receivers), forward+adjoint+gradient, 1 thread, "
+                  f"{dt:.1f} s",
+    }, parity
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=200)
+    ap.add_argument("--plain-shots", type=int, default=10,
+                    help="shots timed in the one-shot-per-launch form (rate is per shot)")
+    ap.add_argument("--cpu-shots", type=int, default=3, help="shots of the reference CPU sample")
+    ap.add_argument("--rows-per-lane", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the gradient path shards by shot, and a multi-GPU run "
+              f"(--gpus {args.gpus}) is not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from rajzel_amd import native as N
+    from rajzel_amd import workloads as W
+
+    t_setup = time.perf_counter()
+    sv = W.c4_gradient(nshots=args.shots)
+    S = sv.S
+    S_true = W.slowness(sv.vp_true)
+    log(f"[bench] workload {sv.name}: {sv.nshots} shots, {sv.ntr} receivers "
+        f"({time.perf_counter() - t_setup:.1f} s to generate)")
+
+    ctx = N.Context([model_dict(sv)], rows_per_lane=args.rows_per_lane)
+    # picks: forward in the true model + seeded noise (setup, untimed)
+    ctx.set_slowness(0, S_true)
+    ctx.forward()
+    rng = np.random.default_rng(sv.meta["seed"] + 1000)
+    sv.tobs = (ctx.tmod(0) + rng.normal(0.0, sv.noise_s, (sv.nshots, sv.ntr))).astype(np.float32)
+    ctx.set_slowness(0, S)
+    ctx.set_picks(0, sv.tobs)
+    log(f"[bench] setup done, R={ctx.rows_per_lane}, {ctx.device_bytes / 2**30:.2f} GiB on device")
+
+    for _ in range(args.warmup):
+        ctx.gradient()
+    # All work runs on the context's stream; torch is not loaded here (it
+    # would bring a second HIP runtime into the process), so the timed
+    # region is bracketed by synchronising that stream.
+    ctx.set_profiling(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.gradient()
+    ctx.sync()
+    t1 = time.perf_counter()
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = sv.nshots / (ms / 1e3)
+    log(f"[bench] batched: {ms:.2f} ms/evaluation, {value:.1f} shot-gradients/s; phases {prof}")
+
+    itf, ita = ctx.iterations(0)
+    fwd_b, adj_b, grd_b = algorithmic_bytes(sv.nx, sv.ny, itf, ita)
+    phase_ms = {k: v[0] / max(v[1], 1) for k, v in prof.items()}
+    phase_bytes = {"forward": fwd_b, "adjoint": adj_b, "gradient": grd_b}
+    dom = max(("forward", "adjoint"), key=lambda k: phase_ms[k])
+    achieved = phase_bytes[dom] / (phase_ms[dom] * 1e-3) / 1e9
+    eval_gbs = (fwd_b + adj_b + grd_b) / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": f"k_{dom}",
+                "bytes_per_launch": phase_bytes[dom], "avg_launch_ms": phase_ms[dom]}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"FATT gradient evaluation, {sv.ny}x{sv.nx} grid, DH={sv.dh} m, "
+                               f"{sv.nshots} shots, {sv.ntr} receivers, 12-layer model",
+                   "nx": sv.nx, "ny": sv.ny, "shots": sv.nshots, "receivers": sv.ntr,
+                   "rows_per_lane": ctx.rows_per_lane},
+        "roofline": roofline,
+        "evaluation_hbm_frac": eval_gbs / HBM_PEAK_GBS,
+        "phase_ms": phase_ms,
+        "iterations": {"forward_mean": float(np.mean(itf)), "adjoint_mean": float(np.mean(ita)),
+                       "forward_max": int(np.max(itf)), "adjoint_max": int(np.max(ita))},
+    }
+
+    if not args.no_plain:
+        n = min(args.plain_shots, sv.nshots)
+        with N.Context([model_dict(subset(sv, n))], rows_per_lane=args.rows_per_lane) as cp:
+            cp.set_slowness(0, S)
+            cp.set_picks(0, sv.tobs[:n])
+            cp.gradient(per_shot=True)
+            cp.sync()
+            p0 = time.perf_counter()
+            cp.gradient(per_shot=True)
+            cp.sync()
+            p1 = time.perf_counter()
+            plain = n / (p1 - p0)
+            same = bool(np.array_equal(cp.tmod(0).view(np.uint32), ctx.tmod(0)[:n].view(np.uint32)))
+        out["plain_per_shot"] = {"value": plain, "unit": "shot-gradients/s", "shots_timed": n,
+                                 "tmod_bitwise_vs_batched": same}
+        out["speedup_vs_plain"] = value / plain
+        log(f"[bench] plain one-shot-per-launch: {plain:.1f} shot-gradients/s")
+
+    if not args.no_cpu:
+        gpu_tmod, gpu_tres = ctx.tmod(0), ctx.tres(0)
+        cb, parity = cpu_baseline(sv, S, sv.tobs, min(args.cpu_shots, sv.nshots), gpu_tmod, gpu_tres, itf, ita)
+        out["cpu_baseline"] = cb
+        out["parity_sample"] = parity
+        log(f"[bench] cpu baseline: {cb['value']:.3f} shot-gradients/s; parity {parity}")
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -137,6 +137,15 @@ int rz_eikonal_shot(rz_ctx *ctx, int model, int src_ix, int src_iy, float *T, fl
 int rz_eikonal_adj_shot(rz_ctx *ctx, int model, const float *T, const float *tres, float *lam, int *iters);
 int rz_ass_grad_shot(rz_ctx *ctx, int model, const float *lam, float *grad_shot, float *grad);
 
+/* Phase timing: with profiling on, rz_forward / rz_gradient record HIP
+ * events on the work stream around each phase; rz_get_profile returns the
+ * summed milliseconds and the number of launches per phase since the last
+ * rz_set_profiling call, in the order forward, residual, adjoint, gradient.
+ * No counterpart in the reference (it times only the whole run,
+ * rajzel.c:185-187). */
+int rz_set_profiling(rz_ctx *ctx, int on);
+int rz_get_profile(const rz_ctx *ctx, double *ms, long long *launches);
+
 /* Block until the context's stream is idle. */
 int rz_sync(rz_ctx *ctx);
 

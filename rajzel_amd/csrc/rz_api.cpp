@@ -71,6 +71,12 @@ struct rz_ctx {
     size_t bytes = 0;
     std::vector<void *> allocs;
     std::string err;
+    // phase timing with events on the work stream (rz_set_profiling)
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    int ev_used = 0;
+    double prof_ms[4] = {0, 0, 0, 0};   // forward, residual, adjoint, gradient
+    long long prof_launches[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -189,6 +195,43 @@ int check_convergence(rz_ctx *c, bool adj)
     return RZ_OK;
 }
 
+// Record an event on the work stream when profiling (index into c->ev).
+int mark(rz_ctx *c, int *idx)
+{
+    *idx = -1;
+    if (!c->prof) return RZ_OK;
+    if (c->ev_used == (int)c->ev.size()) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+    *idx = c->ev_used++;
+    HIPCHK(c, hipEventRecord(c->ev[*idx], c->stream));
+    return RZ_OK;
+}
+
+struct PhaseMarks {
+    int e[5];
+};
+
+// Add the elapsed times of recorded batches to the phase totals.
+int collect(rz_ctx *c, const std::vector<PhaseMarks> &marks, bool grad)
+{
+    if (!c->prof) return RZ_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (const PhaseMarks &pm : marks) {
+        const int nph = grad ? 4 : 1;
+        for (int p = 0; p < nph; ++p) {
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[pm.e[p]], c->ev[pm.e[p + 1]]));
+            c->prof_ms[p] += ms;
+            c->prof_launches[p] += 1;
+        }
+    }
+    c->ev_used = 0;
+    return RZ_OK;
+}
+
 // Run forward (and, with grad, residual + adjoint + gradient) over the
 // global shot order in batches of at most `cap` shots.
 int run(rz_ctx *c, bool grad, int cap)
@@ -213,10 +256,14 @@ int run(rz_ctx *c, bool grad, int cap)
     // Shot g lives in slot g % nslots; any batch of at most nslots
     // consecutive shots therefore uses distinct slots.
     const int per = std::max(1, std::min(cap, c->nslots));
+    std::vector<PhaseMarks> marks;
     for (int b0 = 0; b0 < c->total_shots; b0 += per) {
         const int b1 = std::min(c->total_shots, b0 + per);
+        PhaseMarks pm;
+        TRY(mark(c, &pm.e[0]));
         HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
                                  c->shot_base_d, so, c->stream));
+        TRY(mark(c, &pm.e[1]));
         if (grad) {
             for (size_t mi = 0; mi < c->m.size(); ++mi) {
                 const ModelHost &h = c->m[mi];
@@ -227,8 +274,10 @@ int run(rz_ctx *c, bool grad, int cap)
                                           c->tobs_d, c->carry_d + h.carry_off, c->tres_d, c->l2_shot_d + h.shot_base,
                                           c->l2_sum_d + mi, c->stream));
             }
+            TRY(mark(c, &pm.e[2]));
             HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
                                      c->shot_base_d, c->tres_d, so, c->stream));
+            TRY(mark(c, &pm.e[3]));
             for (size_t mi = 0; mi < c->m.size(); ++mi) {
                 const ModelHost &h = c->m[mi];
                 const int s0 = std::max(b0, h.shot_base) - h.shot_base;
@@ -238,10 +287,13 @@ int run(rz_ctx *c, bool grad, int cap)
                 HIPCHK(c, launch_gradient(&mh, (int)mi, c->md_d, c->slot_of_d + h.shot_base + s0, s1 - s0, sl,
                                           c->grad_d + h.grad_off, s0 > 0 ? 1 : 0, c->stream));
             }
+            TRY(mark(c, &pm.e[4]));
         }
+        if (c->prof) marks.push_back(pm);
         c->last_b0 = b0;
         c->last_b1 = b1;
     }
+    TRY(collect(c, marks, grad));
     c->have_fwd = true;
     c->have_grad = grad;
     return check_convergence(c, grad);
@@ -275,6 +327,7 @@ void rz_destroy(rz_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : c->allocs) (void)hipFree(p);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -699,6 +752,27 @@ int rz_sync(rz_ctx *c)
 {
     if (!c) return RZ_ERR_INVALID;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RZ_OK;
+}
+
+int rz_set_profiling(rz_ctx *c, int on)
+{
+    if (!c) return RZ_ERR_INVALID;
+    c->prof = on != 0;
+    for (int p = 0; p < 4; ++p) {
+        c->prof_ms[p] = 0.0;
+        c->prof_launches[p] = 0;
+    }
+    return RZ_OK;
+}
+
+int rz_get_profile(const rz_ctx *c, double *ms, long long *launches)
+{
+    if (!c) return RZ_ERR_INVALID;
+    for (int p = 0; p < 4; ++p) {
+        if (ms) ms[p] = c->prof_ms[p];
+        if (launches) launches[p] = c->prof_launches[p];
+    }
     return RZ_OK;
 }
 

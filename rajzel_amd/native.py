@@ -22,7 +22,8 @@ EXPORTS = (
     "rz_set_picks", "rz_set_tres_carry", "rz_forward", "rz_gradient", "rz_gradient_per_shot",
     "rz_get_tmod", "rz_get_tres", "rz_get_grad", "rz_get_misfit", "rz_get_iterations",
     "rz_get_tres_carry", "rz_get_field", "rz_eikonal_shot", "rz_eikonal_adj_shot", "rz_ass_grad_shot",
-    "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots",
+    "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
+    "rz_get_profile",
 )
 
 
@@ -85,6 +86,8 @@ def lib():
                 "rz_rows_per_lane": (i, [vp]),
                 "rz_device_bytes": (C.c_size_t, [vp]),
                 "rz_batch_shots": (i, [vp]),
+                "rz_set_profiling": (i, [vp, i]),
+                "rz_get_profile": (i, [vp, vp, vp]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -270,6 +273,16 @@ class Context:
         gs = np.empty_like(lam)
         self._chk(lib().rz_ass_grad_shot(self.h, model, _ptr(lam), _ptr(gs), _ptr(g)))
         return gs, g
+
+    def set_profiling(self, on=True):
+        self._chk(lib().rz_set_profiling(self.h, 1 if on else 0))
+
+    def profile(self):
+        """{phase: (total ms, launches)} since the last set_profiling()."""
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.int64)
+        self._chk(lib().rz_get_profile(self.h, _ptr(ms), _ptr(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("forward", "residual", "adjoint", "gradient"))}
 
     @property
     def rows_per_lane(self):
