@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdint>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -28,6 +29,7 @@ struct ModelHost {
     long long grad_off = 0;  // offset in the gradient buffer
     int carry_off = 0;       // offset in the carry buffer
     bool have_S = false, have_picks = false;
+    bool fastdiv = false;
     std::vector<float> tres0;
     // device
     float *S = nullptr, *SA = nullptr, *SB = nullptr;
@@ -40,6 +42,7 @@ struct rz_ctx {
     int device = 0;
     hipStream_t own = nullptr, stream = nullptr;
     int R = 4, nb_max = 1;
+    bool fastdiv = true;
     std::vector<ModelHost> m;
     int total_shots = 0;
     long long total_rec = 0, total_cells = 0;
@@ -125,6 +128,29 @@ int dalloc(rz_ctx *c, T **p, size_t n)
 
 int bands_for(int ny, int R) { return (ny + kWave * R - 1) / (kWave * R); }
 
+// The adjoint divides by DH as (float)(x * RN64(1/DH)).  That equals the IEEE
+// quotient x/DH for every float x (a float quotient is never within 2^-49
+// relative of a rounding tie; the double product errs by < 2^-52).  Confirm
+// on 2^20 inputs spread over all exponents, including subnormal ones, before
+// the kernels rely on it; otherwise they divide.
+bool fast_div_exact(float dh)
+{
+    const double r = 1.0 / (double)dh;
+    uint32_t state = 0x9E3779B9u;
+    for (int i = 0; i < (1 << 20); ++i) {
+        state ^= state << 13; state ^= state >> 17; state ^= state << 5;
+        uint32_t bits = state & 0x7F7FFFFFu;   // finite, both signs via the loop below
+        float x;
+        std::memcpy(&x, &bits, 4);
+        for (int sg = 0; sg < 2; ++sg, x = -x) {
+            const volatile float q = x / dh;
+            const float f = (float)((double)x * r);
+            if (std::memcmp((const void *)&q, &f, 4) != 0) return false;
+        }
+    }
+    return true;
+}
+
 int choose_R(int ny_max)
 {
     for (int R : {1, 2, 4, 8})
@@ -138,6 +164,7 @@ ModelDev make_dev(const rz_ctx *c, const ModelHost &h)
     d.nx = h.d.nx; d.ny = h.d.ny; d.nb = h.nb; d.R = c->R;
     d.dh = h.d.dh; d.time = h.d.time; d.ttnorm = h.d.ttnorm; d.eps_adj = h.d.eps_adj;
     d.two_dh2 = 2.0 * ((double)h.d.dh * (double)h.d.dh);
+    d.rdh = 1.0 / (double)h.d.dh;
     d.bstride = h.bstride;
     d.fstride = (long long)h.nb * h.bstride;
     d.SA = h.SA; d.SB = h.SB; d.S = h.S;
@@ -276,7 +303,7 @@ int run(rz_ctx *c, bool grad, int cap)
             }
             TRY(mark(c, &pm.e[2]));
             HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
-                                     c->shot_base_d, c->tres_d, so, c->stream));
+                                     c->shot_base_d, c->tres_d, so, c->fastdiv ? 1 : 0, c->stream));
             TRY(mark(c, &pm.e[3]));
             for (size_t mi = 0; mi < c->m.size(); ++mi) {
                 const ModelHost &h = c->m[mi];
@@ -362,9 +389,9 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     c->R = rows_per_lane ? rows_per_lane : choose_R(ny_max);
     if (c->R != 1 && c->R != 2 && c->R != 4 && c->R != 8)
         return bail(fail(c, RZ_ERR_INVALID, "rows_per_lane must be 1, 2, 4 or 8"));
-    if (bands_for(ny_max, c->R) > kMaxBands)
-        return bail(fail(c, RZ_ERR_INVALID, "NY=%d needs more than %d bands at %d rows per lane", ny_max, kMaxBands,
-                         c->R));
+    if (bands_for(ny_max, c->R) > max_bands_for(c->R))
+        return bail(fail(c, RZ_ERR_INVALID, "NY=%d needs more than %d bands at %d rows per lane", ny_max,
+                         max_bands_for(c->R), c->R));
 
     long long rec_off = 0, cells = 0;
     int shots = 0, carry = 0;
@@ -375,7 +402,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         h.d = d;
         h.d.src_ix = h.d.src_iy = h.d.rec_ix = h.d.rec_iy = nullptr;
         h.nb = bands_for(d.ny, c->R);
-        h.bstride = (long long)(d.nx + 63) * kWave * c->R;
+        h.bstride = (long long)(d.nx + 64) * kWave * c->R;   // keys 0..nx+62 + one dummy block
         field_max = std::max(field_max, (long long)h.nb * h.bstride);
         for (int s = 0; s < d.nshots; ++s) {
             const int x = d.src_ix[s], y = d.src_iy[s];
@@ -403,6 +430,8 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
             h.uq_c.push_back((int)(p.first % d.nx));
             h.uq_r.push_back(p.second);
         }
+        h.fastdiv = fast_div_exact(d.dh);
+        c->fastdiv = c->fastdiv && h.fastdiv;
         h.rec_off = rec_off;
         h.shot_base = shots;
         h.grad_off = cells;
@@ -715,7 +744,7 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     ShotOut o;
     o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1;
     HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d,
-                             c->s_tres, o, c->stream));
+                             c->s_tres, o, c->fastdiv ? 1 : 0, c->stream));
     HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA, c->s_row, 0, c->stream));
     int it = 0;
     if (lam) HIPCHK(c, hipMemcpyAsync(lam, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));

@@ -5,7 +5,9 @@
 //   Inside a band, lane l (0..63) owns R consecutive rows 64Rb + lR + r.
 //   Family A stores cell (k, c) of lane l at key  c + l,
 //   family B stores it at key                     c - l + 63,
-//   address = b*bstride + key*64R + l*R + r,  bstride = (NX+63)*64R floats.
+//   address = b*bstride + key*64R + l*R + r,  bstride = (NX+64)*64R floats
+//   (keys 0..NX+62 hold cells; key NX+63 is a dummy block that absorbs the
+//   stores of lanes outside the grid).
 // A sweep along (x+,y+) or (x-,y-) visits, at each step, the cells of one
 // A-key (one anti-diagonal of the band), a sweep along (x+,y-) or (x-,y+)
 // those of one B-key; each step of a wavefront therefore reads and writes
@@ -21,8 +23,10 @@ constexpr int kPF = 8;                // prefetch depth == steps between barrier
 constexpr int kLag = kPF;             // extra lag between consecutive bands (>= kPF-1)
 constexpr int kBandOff = kWave + kLag;// step offset between consecutive bands
 constexpr int kRingQ = 32;            // LDS ring slots per band (>= kLag+1+kPF, power of 2)
-constexpr int kMaxBands = 8;          // waves per workgroup (<= 512 threads)
-constexpr int kMaxThreads = kMaxBands * kWave;
+constexpr int kMaxBands = 16;         // waves per workgroup (<= 1024 threads)
+// Bands a workgroup may hold at R rows per lane: 16 for R <= 2, 8 above
+// (register budget of 128 resp. 256 VGPRs per lane).
+constexpr int max_bands_for(int R) { return R <= 2 ? 16 : 8; }
 
 // INF of the reference (fd.h:29-31): an int compared against float fields.
 constexpr float kInf = 9999999.0f;
@@ -31,6 +35,7 @@ struct ModelDev {
     int nx, ny, nb, R;
     float dh, time, ttnorm, eps_adj;
     double two_dh2;                // 2*DH^2 (exact in double)
+    double rdh;                    // RN64(1/DH)
     long long bstride;             // floats per band
     long long fstride;             // floats per field (nb * bstride)
     const float *SA, *SB;          // slowness in layouts A and B
@@ -70,7 +75,7 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
                           const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st);
 hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
                           const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
-                          hipStream_t st);
+                          int fastdiv, hipStream_t st);
 hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s1, long long rec_off,
                            const float *tmod, const float *tnb, const float *tobs, float *carry, float *tres,
                            double *l2_shot, double *l2_sum, hipStream_t st);

@@ -15,7 +15,7 @@
 // an LDS ring (new values) and single-lane global loads (old values).
 //
 // Arithmetic is the reference's, operation by operation (compile with
-// -ffp-contract=off): see godunov() and adj_cell().  Any topological order
+// -ffp-contract=off): see godunov(), qdh() and adj_sweep().  Any topological order
 // of a Gauss-Seidel sweep's dependency DAG gives the same bits, so each
 // sweep here equals the reference's sweep bit for bit.
 #include "rz_internal.h"
@@ -25,6 +25,21 @@ namespace rz {
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
+// Field pointers reach the sweeps through structs in memory, where the
+// compiler cannot infer their address space and would emit FLAT accesses;
+// a wait on a FLAT load needs vmcnt(0), which would also wait for every
+// prefetch in flight.  All field accesses go through global pointers.
+typedef __attribute__((address_space(1))) float gfloat;
+template <typename T>
+__device__ __forceinline__ const gfloat *gp(const T *p)
+{
+    return (const gfloat *)(p);
+}
+template <typename T>
+__device__ __forceinline__ gfloat *gpw(T *p)
+{
+    return (gfloat *)(p);
+}
 __device__ __forceinline__ float lane_from_below(float v)  // lane l gets lane l+1
 {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, false));
@@ -34,34 +49,44 @@ __device__ __forceinline__ float lane_from_above(float v)  // lane l gets lane l
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) vf2 gvf2;
+typedef __attribute__((address_space(1))) vf4 gvf4;
+
 template <int R>
-__device__ __forceinline__ void ldv(const float *p, float (&o)[R])
+__device__ __forceinline__ void ldv(const gfloat *p, float (&o)[R])
 {
     if constexpr (R == 1) {
         o[0] = p[0];
     } else if constexpr (R == 2) {
-        const float2 t = *reinterpret_cast<const float2 *>(p);
+        const vf2 t = *reinterpret_cast<const gvf2 *>(p);
         o[0] = t.x; o[1] = t.y;
     } else {
 #pragma unroll
         for (int i = 0; i < R; i += 4) {
-            const float4 t = *reinterpret_cast<const float4 *>(p + i);
+            const vf4 t = *reinterpret_cast<const gvf4 *>(p + i);
             o[i] = t.x; o[i + 1] = t.y; o[i + 2] = t.z; o[i + 3] = t.w;
         }
     }
 }
 
 template <int R>
-__device__ __forceinline__ void stv(float *p, const float (&v)[R])
+__device__ __forceinline__ void stv(gfloat *p, const float (&v)[R])
 {
     if constexpr (R == 1) {
         p[0] = v[0];
     } else if constexpr (R == 2) {
-        *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
+        vf2 t;
+        t.x = v[0]; t.y = v[1];
+        *reinterpret_cast<gvf2 *>(p) = t;
     } else {
 #pragma unroll
-        for (int i = 0; i < R; i += 4)
-            *reinterpret_cast<float4 *>(p + i) = make_float4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+        for (int i = 0; i < R; i += 4) {
+            vf4 t;
+            t.x = v[i]; t.y = v[i + 1]; t.z = v[i + 2]; t.w = v[i + 3];
+            *reinterpret_cast<gvf4 *>(p + i) = t;
+        }
     }
 }
 
@@ -82,19 +107,54 @@ __device__ __forceinline__ int addr_b(int k, int c, long long bstride)
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// min with the semantics of the reference's fminf (glibc x86-64: minss,
+// i.e. x < y ? x : y for non-NaN arguments, the second one on ties, so also
+// on +0/-0).  A compare and a select; no NaN canonicalisation.
+__device__ __forceinline__ float mn(float x, float y) { return x < y ? x : y; }
+
+// sqrt of a positive normal double by v_rsq_f64 and two Newton steps; within
+// a few ulp of the correctly rounded root (correctly rounded on every one of
+// 6.7e7 sampled inputs on gfx950).  Callers only use it behind a rounding
+// guard, so exactness never depends on that observation.
+__device__ __forceinline__ double sqrt_nr(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    const double d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
 // Godunov upwind update of sweep.c:89-95.  |a-b| and S*DH are float; the
 // 2-D root is evaluated in double: 2*S^2*DH^2 is S^2 * (2 DH^2) (both
 // products of exact values, hence the same rounding), (a-b)^2 is exact in
 // double, (a+b) is a float sum, /2.0 is exact, one rounding back to float.
+// The root comes from sqrt_nr(); the float rounding of the half-sum can only
+// differ from the one with the correctly rounded root when the double lies
+// within a few ulp of a float tie (low 29 mantissa bits near 0x10000000).
+// Such cells (~1 in 1.6e7) take the correctly rounded sqrt.
 __device__ __forceinline__ float godunov(float a, float b, float s, float dh, double two_dh2)
 {
     const float sdh = s * dh;
     const float d = a - b;
-    if (fabsf(d) >= sdh) return fminf(a, b) + sdh;
+    if (fabsf(d) >= sdh) return mn(a, b) + sdh;
     const double sd = (double)s;
     const double dd = (double)d;
     const double disc = (sd * sd) * two_dh2 - dd * dd;
-    return (float)(((double)(a + b) + __builtin_sqrt(disc)) * 0.5);
+    const double ab = (double)(a + b);
+    double half = (ab + sqrt_nr(disc)) * 0.5;
+    constexpr unsigned kGuard = 16;
+    const unsigned lo = (unsigned)__double_as_longlong(half) & 0x1FFFFFFFu;
+    if (lo - (0x10000000u - kGuard) <= 2 * kGuard) {
+        // The empty asm has a side effect, so the compiler cannot hoist the
+        // correctly rounded sqrt out of this rarely taken branch.
+        double x = disc;
+        asm volatile("" : "+v"(x));
+        half = (ab + __builtin_sqrt(x)) * 0.5;
+    }
+    return (float)half;
 }
 
 // Block-wide double sum in a fixed order (identical in every wave).
@@ -109,6 +169,18 @@ __device__ __forceinline__ double block_sum(double v, double *red, int nwaves)
     for (int i = 0; i < nwaves; ++i) t += red[i];
     __syncthreads();
     return t;
+}
+
+// Barrier between step epochs of a sweep.  Within a sweep no wave reads
+// global memory another wave writes: new values cross bands through the LDS
+// ring, old values through loads that have already been consumed (waited
+// for) before the barrier.  So only LDS needs ordering here, and the
+// prefetched loads stay in flight.  Sweep boundaries use __syncthreads().
+__device__ __forceinline__ void epoch_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 struct WaveCtx {
@@ -130,6 +202,11 @@ __device__ __forceinline__ int sweep_steps(int nx, int nb)
 //   Tsame  written in the same layout, may be Tin (if WSAME)
 //   INIT   read the initial field (2*TIME, 0 at the source) instead of Tin
 // Returns this lane's sum of (old - new) over its valid cells.
+//
+// Every step issues the same memory operations whatever the lane or step
+// validity (invalid lanes write to the band's dummy key block nx+63 and to a
+// dummy LDS word), so the compiler can wait on each prefetch with a partial
+// vmcnt instead of draining the queue.
 // ---------------------------------------------------------------------------
 template <int R, int SX, int SY, bool INIT, bool WOTHER, bool WSAME>
 __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const float *Tin,
@@ -139,17 +216,19 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
-    const int nkeys = nx + 63;
+    const int nkeys = nx + 63;          // data keys 0..nx+62; key nkeys is the dummy block
     const int gend = sweep_steps(nx, nb);
     if (!w.active) {
-        for (int g0 = 0; g0 < gend; g0 += kPF) __syncthreads();
+        for (int g0 = 0; g0 < gend; g0 += kPF) epoch_barrier();
+        __syncthreads();
         return 0.0;
     }
     const int lag = SY > 0 ? l : 63 - l;
     const int off = (SY > 0 ? beta : nb - 1 - beta) * kBandOff;
     const long long boff = (long long)beta * w.bstride;
-    const float *tband = Tin + boff;
-    const float *sband = Sf + boff;
+    const gfloat *tband = gp(Tin) + boff;
+    const gfloat *sband = gp(Sf) + boff;
+    gfloat *gTsame = gpw(Tsame) + boff, *gTother = gpw(Tother) + boff;
     const int k0 = beta * BR + l * R;
 
     bool rvalid[R], top[R], bot[R];
@@ -160,18 +239,15 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
         top[r] = (k == 0);
         bot[r] = (k == ny - 1);
     }
-    // The edge lane needs the old value of the row across the band boundary
-    // on the downstream side: below (SY>0, lane 63) or above (SY<0, lane 0).
-    const int edge_lane = SY > 0 ? 63 : 0;
+    // Old value across the band boundary on the downstream side: the row
+    // below (SY>0, needed by lane 63) or above (SY<0, lane 0).  Both edge
+    // lanes have lag 63, so the address is the same for the whole wave.
     const bool has_dn = SY > 0 ? (beta + 1 < nb) : (beta > 0);
-    const bool has_up = SY > 0 ? (beta > 0) : (beta + 1 < nb);
-    const bool do_edge = (l == edge_lane) && has_dn;
-    const float *eband = tband + (SY > 0 ? w.bstride : -w.bstride);
+    const gfloat *eband = has_dn ? tband + (SY > 0 ? w.bstride : -w.bstride) : tband;
     const int erow = SY > 0 ? (beta + 1) * BR : beta * BR - 1;
-    const int ering = (SY > 0 ? beta - 1 : beta + 1);       // producer band read here
-    const int ering_c = clampi(ering, 0, kMaxBands - 1);
-    const bool ring_rd_lane = SY > 0 ? (l == 0) : (l == 63);
-    const bool ring_wr = (SY > 0 ? (l == 63) : (l == 0)) && (SY > 0 ? (beta + 1 < nb) : (beta > 0));
+    const int ering = clampi(SY > 0 ? beta - 1 : beta + 1, 0, kMaxBands - 1);
+    const bool ring_writer = (SY > 0 ? (l == 63) : (l == 0)) && (SY > 0 ? (beta + 1 < nb) : (beta > 0));
+    float *ring_dummy = w.ring + kMaxBands * kRingQ + l;
 
     float qt[kPF][R], qs[kPF][R], qe[kPF];
 
@@ -180,48 +256,49 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
         const int tau_ = (TAU);                                                             \
         const int key_ = clampi(SX > 0 ? tau_ : nx + 62 - tau_, 0, nkeys - 1);              \
         ldv<R>(sband + key_ * BR + l * R, qs[SLOT]);                                        \
+        const int ce_ = clampi(SX > 0 ? tau_ - 63 : nx - 1 - tau_ + 63, 0, nx - 1);         \
         if constexpr (INIT) {                                                               \
             const int c_ = FA ? key_ - l : key_ + l - 63;                                   \
             _Pragma("unroll") for (int r_ = 0; r_ < R; ++r_)                                \
                 qt[SLOT][r_] = (k0 + r_ == src_k && c_ == src_c) ? 0.f : tinit;             \
+            qe[SLOT] = (erow == src_k && ce_ == src_c) ? 0.f : tinit;                       \
         } else {                                                                            \
             ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                    \
-        }                                                                                   \
-        if (do_edge) {                                                                      \
-            const int ce_ = clampi(SX > 0 ? tau_ - lag : nx - 1 - tau_ + lag, 0, nx - 1);   \
-            if constexpr (INIT) {                                                           \
-                qe[SLOT] = (erow == src_k && ce_ == src_c) ? 0.f : tinit;                   \
-            } else {                                                                        \
-                const int ek_ = SY > 0 ? (FA ? ce_ : ce_ + 63) : (FA ? ce_ + 63 : ce_);     \
-                const int eo_ = SY > 0 ? 0 : 63 * R + R - 1;                                \
-                qe[SLOT] = eband[ek_ * BR + eo_];                                           \
-            }                                                                               \
+            const int ek_ = SY > 0 ? (FA ? ce_ : ce_ + 63) : (FA ? ce_ + 63 : ce_);         \
+            qe[SLOT] = eband[ek_ * BR + (SY > 0 ? 0 : 63 * R + R - 1)];                     \
         }                                                                                   \
     } while (0)
 
 #pragma unroll
-    for (int u = 0; u < kPF; ++u) {
-        qe[u] = 0.f;
-        RZ_FWD_LOAD(u, u - off);
-    }
+    for (int u = 0; u < kPF; ++u) RZ_FWD_LOAD(u, u - off);
 
     float pnew[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) pnew[r] = tinit;
     double acc = 0.0;
+    float ring_next = 0.f;   // first consumer step is >= kBandOff steps away
 
     for (int g0 = 0; g0 < gend; g0 += kPF) {
 #pragma unroll
         for (int u = 0; u < kPF; ++u) {
             const int un = (u + 1) % kPF;
             const int tau = g0 + u - off;
-            if (tau >= 0 && tau < nkeys) {
-                const int c = SX > 0 ? tau - lag : nx - 1 - tau + lag;
-                const bool cvalid = (unsigned)c < (unsigned)nx;
+            const bool active = tau >= 0 && tau < nkeys;   // wave-uniform
+            const int c = SX > 0 ? tau - lag : nx - 1 - tau + lag;
+            const bool cvalid = active && (unsigned)c < (unsigned)nx;
+            // ring value of this step, read one step ahead (it was written
+            // eight or more steps before, in an earlier epoch)
+            const float ringv = ring_next;
+            {
+                const int cn = SX > 0 ? tau + 1 - lag : nx - 2 - tau + lag;
+                ring_next = w.ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
+            }
+            float nv[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) nv[r] = qt[u][r];
+            if (active) {
                 const bool first = SX > 0 ? (c == 0) : (c == nx - 1);
                 const bool last = SX > 0 ? (c == nx - 1) : (c == 0);
-                const int ckey = clampi(c, 0, nx - 1) & (kRingQ - 1);
-                const float ringv = w.ring[ering_c * kRingQ + ckey];
                 float up_new, dn_old;
                 if (SY > 0) {
                     up_new = lane_from_above(pnew[R - 1]);
@@ -234,8 +311,6 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
                     if (l == 63) up_new = ringv;
                     if (l == 0) dn_old = qe[u];
                 }
-                (void)has_up;
-                float nv[R];
                 float fsum = 0.f;
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
@@ -255,69 +330,63 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
                         if (bot[r]) yu = self;
                         if (top[r]) yd = self;
                     }
-                    const float a = fminf(xu, xd);
-                    const float b = fminf(yu, yd);
+                    const float a = mn(xu, xd);
+                    const float b = mn(yu, yd);
                     const float t = godunov(a, b, qs[u][r], dh, two_dh2);
-                    nv[r] = fminf(self, t);
+                    nv[r] = mn(self, t);
                     if (cvalid && rvalid[r]) fsum += self - nv[r];
                 }
                 acc += (double)fsum;
-                if (cvalid) {
-                    if constexpr (WSAME) {
-                        const int key = SX > 0 ? tau : nx + 62 - tau;
-                        stv<R>(Tsame + boff + key * BR + l * R, nv);
-                    }
-                    if constexpr (WOTHER) {
-                        const int ko = FA ? (c - l + 63) : (c + l);
-                        stv<R>(Tother + boff + ko * BR + l * R, nv);
-                    }
-                    if (ring_wr) w.ring[beta * kRingQ + (c & (kRingQ - 1))] = SY > 0 ? nv[R - 1] : nv[0];
-                }
 #pragma unroll
                 for (int r = 0; r < R; ++r) pnew[r] = nv[r];
             }
+            if constexpr (WSAME) {
+                const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
+                stv<R>(gTsame + ks * BR + l * R, nv);
+            }
+            if constexpr (WOTHER) {
+                const int ko = cvalid ? (FA ? (c - l + 63) : (c + l)) : nkeys;
+                stv<R>(gTother + ko * BR + l * R, nv);
+            }
+            float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
+            *rw = SY > 0 ? nv[R - 1] : nv[0];
             RZ_FWD_LOAD(u, tau + kPF);
         }
-        __syncthreads();
+        epoch_barrier();
     }
+    __syncthreads();
 #undef RZ_FWD_LOAD
-    (void)ring_rd_lane;
     return acc;
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint cell update (sweep_adj.c:33-68), float arithmetic in source order.
+// Division by DH (sweep_adj.c:33-54).  FD: (float)(x * RN64(1/DH)) in
+// double, which is the correctly rounded x/DH for every float x: a quotient
+// of two floats is never within 2^-49 (relative) of a float rounding tie,
+// while the double product is within 2^-52 of it.  The host also checks it
+// against the IEEE division for the model's DH before enabling it.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float adj_cell(float tc, float txp, float txm, float typ, float tym,
-                                          float lc, float lxm, float lxp, float lym, float lyp,
-                                          float dh, float eps)
+template <bool FD>
+__device__ __forceinline__ float qdh(float x, float dh, double rdh)
 {
-    const float ap = -(txp - tc) / dh;
-    const float am = -(tc - txm) / dh;
-    const float bp = -(typ - tc) / dh;
-    const float bm = -(tc - tym) / dh;
-    const float app = (ap + fabsf(ap)) * 0.5f, apm = (ap - fabsf(ap)) * 0.5f;
-    const float amp = (am + fabsf(am)) * 0.5f, amm = (am - fabsf(am)) * 0.5f;
-    const float bpp = (bp + fabsf(bp)) * 0.5f, bpm = (bp - fabsf(bp)) * 0.5f;
-    const float bmp = (bm + fabsf(bm)) * 0.5f, bmm = (bm - fabsf(bm)) * 0.5f;
-    const float lhs = (app - amm) / dh + (bpp - bmm) / dh;
-    const float rhs = (amp * lxm - apm * lxp) / dh + (bmp * lym - bpm * lyp) / dh;
-    const float lt = (lhs > eps) ? rhs / lhs : 0.f;
-    float v = fminf(lc, lt);
-    const float tmax = fmaxf(fmaxf(fmaxf(tc, typ), fmaxf(tym, txp)), txm);
-    if (tmax >= kInf) v = 0.f;
-    return v;
+    if constexpr (FD) return (float)((double)x * rdh);
+    else return x / dh;
 }
+
+// (x + |x|)/2 and (x - |x|)/2 of sweep_adj.c:39-49: exact, signed zeros kept.
+__device__ __forceinline__ float pos_part(float x) { return (x + fabsf(x)) * 0.5f; }
+__device__ __forceinline__ float neg_part(float x) { return (x - fabsf(x)) * 0.5f; }
 
 // ---------------------------------------------------------------------------
 // Adjoint sweep (sweep_adj.c:10-77) in direction (SX, SY).
 //   Tf    converged traveltime in this sweep's family layout; the sign bit
 //         marks receiver cells (skipped, sweep_adj.c:30)
-//   Lin   lambda in the same layout; outputs as in fwd_sweep.
+//   Lin   lambda in the same layout; outputs as in fwd_sweep, with the same
+//         fixed per-step set of memory operations.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool WOTHER, bool WSAME>
-__device__ double adj_sweep(const WaveCtx &w, float dh, float eps, const float *Tf, const float *Lin,
-                            float *Lother, float *Lsame)
+template <int R, int SX, int SY, bool WOTHER, bool WSAME, bool FD>
+__device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
+                            const float *Lin, float *Lother, float *Lsame)
 {
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
@@ -325,14 +394,16 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, float eps, const float *
     const int nkeys = nx + 63;
     const int gend = sweep_steps(nx, nb);
     if (!w.active) {
-        for (int g0 = 0; g0 < gend; g0 += kPF) __syncthreads();
+        for (int g0 = 0; g0 < gend; g0 += kPF) epoch_barrier();
+        __syncthreads();
         return 0.0;
     }
     const int lag = SY > 0 ? l : 63 - l;
     const int off = (SY > 0 ? beta : nb - 1 - beta) * kBandOff;
     const long long boff = (long long)beta * w.bstride;
-    const float *tband = Tf + boff;
-    const float *lband = Lin + boff;
+    const gfloat *tband = gp(Tf) + boff;
+    const gfloat *lband = gp(Lin) + boff;
+    gfloat *gLsame = gpw(Lsame) + boff, *gLother = gpw(Lother) + boff;
     const int k0 = beta * BR + l * R;
 
     bool rvalid[R], frow[R];
@@ -342,16 +413,19 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, float eps, const float *
         rvalid[r] = k < ny;
         frow[r] = (k == 0) || (k >= ny - 1);
     }
-    // Lane 0 reads the row above from band beta-1, lane 63 the row below
-    // from band beta+1 (T always; lambda on the downstream side only).
-    const bool up_lane = (l == 0) && (beta > 0);
-    const bool dn_lane = (l == 63) && (beta + 1 < nb);
-    const bool do_edge = up_lane || dn_lane;
-    const bool lam_edge = SY > 0 ? dn_lane : up_lane;
-    const int ering_c = clampi(SY > 0 ? beta - 1 : beta + 1, 0, kMaxBands - 1);
-    const bool ring_wr = (SY > 0 ? (l == 63) : (l == 0)) && (SY > 0 ? (beta + 1 < nb) : (beta > 0));
+    // Rows across the band boundaries: above lane 0 (band beta-1, lane 63,
+    // r = R-1) and below lane 63 (band beta+1, lane 0, r = 0); each address
+    // is the same for the whole wave.  T on both sides, lambda (old) on the
+    // downstream side.
+    const int lag0 = SY > 0 ? 0 : 63, lag63 = SY > 0 ? 63 : 0;
+    const bool has_up = beta > 0, has_dn = beta + 1 < nb;
+    const long long up_base = has_up ? -w.bstride + 63 * R + R - 1 : 0;
+    const long long dn_base = has_dn ? w.bstride : 0;
+    const int ering = clampi(SY > 0 ? beta - 1 : beta + 1, 0, kMaxBands - 1);
+    const bool ring_writer = (SY > 0 ? (l == 63) : (l == 0)) && (SY > 0 ? has_dn : has_up);
+    float *ring_dummy = w.ring + kMaxBands * kRingQ + l;
 
-    float qt[kPF][R], ql[kPF][R], qet[kPF], qel[kPF];
+    float qt[kPF][R], ql[kPF][R], qeu[kPF], qed[kPF], qel[kPF];
 
 #define RZ_ADJ_LOAD(SLOT, TAU)                                                              \
     do {                                                                                    \
@@ -359,39 +433,42 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, float eps, const float *
         const int key_ = clampi(SX > 0 ? tau_ : nx + 62 - tau_, 0, nkeys - 1);              \
         ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                        \
         ldv<R>(lband + key_ * BR + l * R, ql[SLOT]);                                        \
-        if (do_edge) {                                                                      \
-            const int ce_ = clampi(SX > 0 ? tau_ - lag : nx - 1 - tau_ + lag, 0, nx - 1);   \
-            long long eo_;                                                                  \
-            if (l == 0) eo_ = -w.bstride + (long long)((FA ? ce_ + 63 : ce_) * BR + 63 * R + R - 1); \
-            else        eo_ = w.bstride + (long long)((FA ? ce_ : ce_ + 63) * BR);         \
-            qet[SLOT] = tband[eo_];                                                         \
-            if (lam_edge) qel[SLOT] = lband[eo_];                                           \
-        }                                                                                   \
+        const int c0_ = clampi(SX > 0 ? tau_ - lag0 : nx - 1 - tau_ + lag0, 0, nx - 1);     \
+        const int c63_ = clampi(SX > 0 ? tau_ - lag63 : nx - 1 - tau_ + lag63, 0, nx - 1);  \
+        const long long eu_ = up_base + (long long)(FA ? c0_ + 63 : c0_) * BR;              \
+        const long long ed_ = dn_base + (long long)(FA ? c63_ : c63_ + 63) * BR;            \
+        qeu[SLOT] = tband[eu_];                                                             \
+        qed[SLOT] = tband[ed_];                                                             \
+        qel[SLOT] = lband[SY > 0 ? ed_ : eu_];                                              \
     } while (0)
 
 #pragma unroll
-    for (int u = 0; u < kPF; ++u) {
-        qet[u] = 0.f;
-        qel[u] = 0.f;
-        RZ_ADJ_LOAD(u, u - off);
-    }
+    for (int u = 0; u < kPF; ++u) RZ_ADJ_LOAD(u, u - off);
 
-    float tprev[R], pnew[R];
+    float tprev[R], pnew[R], xcarry[R], ybp[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) { tprev[r] = 0.f; pnew[r] = 0.f; }
+    for (int r = 0; r < R; ++r) { tprev[r] = 0.f; pnew[r] = 0.f; xcarry[r] = 0.f; ybp[r] = 0.f; }
     double acc = 0.0;
+    float ring_next = 0.f;
 
     for (int g0 = 0; g0 < gend; g0 += kPF) {
 #pragma unroll
         for (int u = 0; u < kPF; ++u) {
             const int un = (u + 1) % kPF;
             const int tau = g0 + u - off;
-            if (tau >= 0 && tau < nkeys) {
-                const int c = SX > 0 ? tau - lag : nx - 1 - tau + lag;
-                const bool cvalid = (unsigned)c < (unsigned)nx;
+            const bool active = tau >= 0 && tau < nkeys;   // wave-uniform
+            const int c = SX > 0 ? tau - lag : nx - 1 - tau + lag;
+            const bool cvalid = active && (unsigned)c < (unsigned)nx;
+            const float ringv = ring_next;   // read one step ahead, as in fwd_sweep
+            {
+                const int cn = SX > 0 ? tau + 1 - lag : nx - 2 - tau + lag;
+                ring_next = w.ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
+            }
+            float nv[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) nv[r] = ql[u][r];
+            if (active) {
                 const bool fcol = (c <= 0) || (c >= nx - 1);
-                const int ckey = clampi(c, 0, nx - 1) & (kRingQ - 1);
-                const float ringv = w.ring[ering_c * kRingQ + ckey];
                 // T of the rows across the lane boundary, at column c
                 float t_above, t_below;
                 if (SY > 0) {   // lane l-1 is one step ahead, lane l+1 one step behind
@@ -401,10 +478,10 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, float eps, const float *
                     t_above = lane_from_above(qt[un][R - 1]);
                     t_below = lane_from_below(tprev[0]);
                 }
-                if (l == 0) t_above = qet[u];
-                if (l == 63) t_below = qet[u];
+                if (l == 0) t_above = qeu[u];
+                if (l == 63) t_below = qed[u];
                 // lambda across the lane boundary: new upstream, old downstream
-                float l_up, l_dn;   // up = row k-1, dn = row k+1 (for r = 0 / r = R-1)
+                float l_up, l_dn;   // row k-1 for r = 0, row k+1 for r = R-1
                 if (SY > 0) {
                     l_up = lane_from_above(pnew[R - 1]);
                     l_dn = lane_from_below(ql[un][0]);
@@ -416,18 +493,46 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, float eps, const float *
                     if (l == 0) l_up = qel[u];
                     if (l == 63) l_dn = ringv;
                 }
-                float nv[R];
+                // T-only part of the cell equations (no dependence on lambda)
+                float lhs[R], amp[R], apm[R], bmp[R], bpm[R], xq[R];
+                bool keep[R], infv[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float traw = qt[u][r];
+                    const bool rec = __float_as_uint(traw) >> 31;
+                    const float tc = fabsf(traw);
+                    const float tl = fabsf(SX > 0 ? tprev[r] : qt[un][r]);   // T(k, c-1)
+                    const float tr = fabsf(SX > 0 ? qt[un][r] : tprev[r]);   // T(k, c+1)
+                    const float tu = fabsf(r == 0 ? t_above : qt[u][r > 0 ? r - 1 : 0]);
+                    const float td = fabsf(r == R - 1 ? t_below : qt[u][r < R - 1 ? r + 1 : r]);
+                    // ap(c) and am(c+1) are the same quotient; carried along x
+                    float ap, am;
+                    if (SX > 0) {
+                        am = xcarry[r];
+                        ap = qdh<FD>(-(tr - tc), dh, rdh);
+                        xq[r] = ap;
+                    } else {
+                        ap = xcarry[r];
+                        am = qdh<FD>(-(tc - tl), dh, rdh);
+                        xq[r] = am;
+                    }
+                    const float bp = qdh<FD>(-(td - tc), dh, rdh);
+                    ybp[r] = bp;
+                    // bm(k) = bp(k-1): from the row above in this lane
+                    const float bm = r > 0 ? ybp[r > 0 ? r - 1 : 0] : qdh<FD>(-(tc - tu), dh, rdh);
+                    lhs[r] = qdh<FD>(pos_part(ap) - neg_part(am), dh, rdh)
+                           + qdh<FD>(pos_part(bp) - neg_part(bm), dh, rdh);
+                    amp[r] = pos_part(am);
+                    apm[r] = neg_part(ap);
+                    bmp[r] = pos_part(bm);
+                    bpm[r] = neg_part(bp);
+                    keep[r] = !cvalid || fcol || frow[r] || rec;
+                    infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
+                }
                 float fsum = 0.f;
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
                     const int r = SY > 0 ? i : R - 1 - i;
-                    const float traw = qt[u][r];
-                    const bool rec = __float_as_uint(traw) >> 31;
-                    const float tc = fabsf(traw);
-                    const float tup = fabsf(SX > 0 ? tprev[r] : qt[un][r]);   // T(k, c-1)
-                    const float tdn = fabsf(SX > 0 ? qt[un][r] : tprev[r]);   // T(k, c+1)
-                    const float tym = fabsf(r == 0 ? t_above : qt[u][r > 0 ? r - 1 : 0]);
-                    const float typ = fabsf(r == R - 1 ? t_below : qt[u][r < R - 1 ? r + 1 : r]);
                     const float lc = ql[u][r];
                     const float lxm = SX > 0 ? pnew[r] : ql[un][r];            // lambda(k, c-1)
                     const float lxp = SX > 0 ? ql[un][r] : pnew[r];            // lambda(k, c+1)
@@ -439,33 +544,37 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, float eps, const float *
                         lym = (r == 0) ? l_up : ql[u][r > 0 ? r - 1 : 0];
                         lyp = (r == R - 1) ? l_dn : nv[r < R - 1 ? r + 1 : r];
                     }
-                    const float v = adj_cell(tc, tdn, tup, typ, tym, lc, lxm, lxp, lym, lyp, dh, eps);
-                    const bool keep = !cvalid || fcol || frow[r] || rec;
-                    nv[r] = keep ? lc : v;
+                    const float rhs = qdh<FD>(amp[r] * lxm - apm[r] * lxp, dh, rdh)
+                                    + qdh<FD>(bmp[r] * lym - bpm[r] * lyp, dh, rdh);
+                    const float lt = (lhs[r] > eps) ? rhs / lhs[r] : 0.f;          // sweep_adj.c:60-64
+                    float v = mn(lc, lt);                                          // sweep_adj.c:66
+                    if (infv[r]) v = 0.f;                                          // sweep_adj.c:68
+                    nv[r] = keep[r] ? lc : v;
                     if (cvalid && rvalid[r]) fsum += lc - nv[r];
                 }
                 acc += (double)fsum;
-                if (cvalid) {
-                    if constexpr (WSAME) {
-                        const int key = SX > 0 ? tau : nx + 62 - tau;
-                        stv<R>(Lsame + boff + key * BR + l * R, nv);
-                    }
-                    if constexpr (WOTHER) {
-                        const int ko = FA ? (c - l + 63) : (c + l);
-                        stv<R>(Lother + boff + ko * BR + l * R, nv);
-                    }
-                    if (ring_wr) w.ring[beta * kRingQ + (c & (kRingQ - 1))] = SY > 0 ? nv[R - 1] : nv[0];
-                }
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
+                    xcarry[r] = xq[r];
                     pnew[r] = nv[r];
                     tprev[r] = qt[u][r];
                 }
             }
+            if constexpr (WSAME) {
+                const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
+                stv<R>(gLsame + ks * BR + l * R, nv);
+            }
+            if constexpr (WOTHER) {
+                const int ko = cvalid ? (FA ? (c - l + 63) : (c + l)) : nkeys;
+                stv<R>(gLother + ko * BR + l * R, nv);
+            }
+            float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
+            *rw = SY > 0 ? nv[R - 1] : nv[0];
             RZ_ADJ_LOAD(u, tau + kPF);
         }
-        __syncthreads();
+        epoch_barrier();
     }
+    __syncthreads();
 #undef RZ_ADJ_LOAD
     return acc;
 }
@@ -474,12 +583,12 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, float eps, const float *
 // Forward kernel: one workgroup per shot (eikonal.c:10-76 + calc_FA.c).
 // ---------------------------------------------------------------------------
 template <int R>
-__global__ __launch_bounds__(kMaxThreads) void k_forward(const ModelDev *__restrict__ mds,
+__global__ __launch_bounds__(max_bands_for(R) * kWave) void k_forward(const ModelDev *__restrict__ mds,
                                                          const ShotDev *__restrict__ shots, Slots slots,
                                                          const long long *__restrict__ rec_off,
                                                          const int *__restrict__ shot_base, ShotOut out)
 {
-    __shared__ float ring[kMaxBands * kRingQ];
+    __shared__ float ring[kMaxBands * kRingQ + kWave];
     __shared__ double red[kMaxBands];
     const ShotDev sh = shots[blockIdx.x];
     const ModelDev md = mds[sh.model];
@@ -549,15 +658,15 @@ __global__ __launch_bounds__(kMaxThreads) void k_forward(const ModelDev *__restr
 // Adjoint kernel: one workgroup per shot (eikonal_adj.c:10-59 with
 // init_travel_adj.c as an O(ntr) scatter).
 // ---------------------------------------------------------------------------
-template <int R>
-__global__ __launch_bounds__(kMaxThreads) void k_adjoint(const ModelDev *__restrict__ mds,
+template <int R, bool FD>
+__global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const ModelDev *__restrict__ mds,
                                                          const ShotDev *__restrict__ shots, Slots slots,
                                                          const long long *__restrict__ rec_off,
                                                          const int *__restrict__ shot_base,
                                                          const float *__restrict__ tres, ShotOut out)
 {
     constexpr int BR = kWave * R;
-    __shared__ float ring[kMaxBands * kRingQ];
+    __shared__ float ring[kMaxBands * kRingQ + kWave];
     __shared__ double red[kMaxBands];
     const ShotDev sh = shots[blockIdx.x];
     const ModelDev md = mds[sh.model];
@@ -592,10 +701,10 @@ __global__ __launch_bounds__(kMaxThreads) void k_adjoint(const ModelDev *__restr
     int it = 0;
     while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
-        double acc = adj_sweep<R, +1, +1, true, false>(w, md.dh, md.eps_adj, TA, LA, LB, nullptr);
-        acc += adj_sweep<R, +1, -1, false, true>(w, md.dh, md.eps_adj, TB, LB, nullptr, LB);
-        acc += adj_sweep<R, -1, +1, true, false>(w, md.dh, md.eps_adj, TB, LB, LA, nullptr);
-        acc += adj_sweep<R, -1, -1, false, true>(w, md.dh, md.eps_adj, TA, LA, nullptr, LA);
+        double acc = adj_sweep<R, +1, +1, true, false, FD>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB, nullptr);
+        acc += adj_sweep<R, +1, -1, false, true, FD>(w, md.dh, md.rdh, md.eps_adj, TB, LB, nullptr, LB);
+        acc += adj_sweep<R, -1, +1, true, false, FD>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA, nullptr);
+        acc += adj_sweep<R, -1, -1, false, true, FD>(w, md.dh, md.rdh, md.eps_adj, TA, LA, nullptr, LA);
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
         ++it;
@@ -784,11 +893,16 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
 
 hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
                           const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
-                          hipStream_t st)
+                          int fastdiv, hipStream_t st)
 {
     if (nshots <= 0) return hipSuccess;
-    RZ_DISPATCH_R(R, hipLaunchKernelGGL(k_adjoint<RR>, dim3(nshots), dim3(nb_max * kWave), 0, st, md, shots,
-                                        slots, rec_off, shot_base, tres, out));
+    if (fastdiv) {
+        RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, true>), dim3(nshots), dim3(nb_max * kWave), 0, st, md,
+                                            shots, slots, rec_off, shot_base, tres, out));
+    } else {
+        RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, false>), dim3(nshots), dim3(nb_max * kWave), 0, st, md,
+                                            shots, slots, rec_off, shot_base, tres, out));
+    }
     return hipGetLastError();
 }
 

@@ -41,7 +41,8 @@ def test_forward_small_bitwise(ora, R):
 
 
 @pytest.mark.parametrize("R,shape", [(1, (150, 90)), (2, (300, 70)), (1, (64, 40)), (1, (65, 41)),
-                                     (4, (520, 33)), (1, (200, 5))])
+                                     (4, (520, 33)), (1, (200, 5)), (1, (1001, 37)), (2, (1001, 29)),
+                                     (8, (1100, 21))])
 def test_forward_multiband_bitwise(ora, R, shape):
     ny, nx = shape
     sv = _case(nx, ny, 2, seed=11)
@@ -82,7 +83,7 @@ def _check_grad(ora, sv, ctx, tres0=None):
     return O
 
 
-@pytest.mark.parametrize("R", [1, 4])
+@pytest.mark.parametrize("R", [1, 2, 4, 8])
 def test_gradient_small_bitwise(ora, R):
     sv = _grad_case(ora, 53, 37, 4, seed=3)
     N = _native()
@@ -98,12 +99,13 @@ def test_gradient_small_bitwise(ora, R):
         assert bits_equal(T_last, O["T"])
 
 
-def test_gradient_multiband_muted_carry(ora):
-    sv = _grad_case(ora, 70, 150, 5, seed=5, mute=[(0, 3), (1, 3), (2, 10), (4, 0)])
+@pytest.mark.parametrize("R,ny", [(1, 150), (1, 700), (2, 700)])
+def test_gradient_multiband_muted_carry(ora, R, ny):
+    sv = _grad_case(ora, 70, ny, 5, seed=5, mute=[(0, 3), (1, 3), (2, 10), (4, 0)])
     N = _native()
     tres0 = np.linspace(-1e-3, 1e-3, sv.ntr).astype(np.float32)
-    print("[case] gradient 150x70 R=1 muted", flush=True)
-    with N.Context([model_dict(sv, max_iter=50)], rows_per_lane=1) as ctx:
+    print(f"[case] gradient {ny}x70 R={R} muted", flush=True)
+    with N.Context([model_dict(sv, max_iter=50)], rows_per_lane=R) as ctx:
         ctx.set_slowness(0, sv.S)
         ctx.set_picks(0, sv.tobs)
         ctx.set_tres_carry(0, tres0)
@@ -124,10 +126,10 @@ def test_per_shot_and_budget_equal_batched(ora):
         g_p = ctx.grad(0)
         assert bits_equal(g_b, g_p)
         _check_grad(ora, sv, ctx)
-    # a budget that fits two shots' fields (4 fields of nb*(nx+63)*64R floats): 3 batches
+    # a budget that fits two shots' fields (4 fields of nb*(nx+64)*64R floats): 3 batches
     R = 1
     nb = -(-sv.ny // (64 * R))
-    slot = -(-(nb * (sv.nx + 63) * 64 * R) // 64) * 64 * 4 * 4
+    slot = -(-(nb * (sv.nx + 64) * 64 * R) // 64) * 64 * 4 * 4
     with N.Context([model_dict(sv, max_iter=50)], rows_per_lane=R, hbm_budget=2 * slot) as ctx:
         assert ctx.batch_shots == 2
         ctx.set_slowness(0, sv.S)
