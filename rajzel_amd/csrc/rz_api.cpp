@@ -43,6 +43,7 @@ struct rz_ctx {
     hipStream_t own = nullptr, stream = nullptr;
     int R = 4, nb_max = 1;
     bool fastdiv = true;
+    bool chkinf = false;   // INF guard of sweep_adj.c:68 can fire only if 2*TIME >= INF
     std::vector<ModelHost> m;
     int total_shots = 0;
     long long total_rec = 0, total_cells = 0;
@@ -183,6 +184,8 @@ int check_model(rz_ctx *c, int model)
     return RZ_OK;
 }
 
+int adj_variant(const rz_ctx *c) { return (c->fastdiv ? 1 : 0) | (c->chkinf ? 2 : 0); }
+
 Slots slots_of(const rz_ctx *c)
 {
     Slots s;
@@ -303,7 +306,7 @@ int run(rz_ctx *c, bool grad, int cap)
             }
             TRY(mark(c, &pm.e[2]));
             HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
-                                     c->shot_base_d, c->tres_d, so, c->fastdiv ? 1 : 0, c->stream));
+                                     c->shot_base_d, c->tres_d, so, adj_variant(c), c->stream));
             TRY(mark(c, &pm.e[3]));
             for (size_t mi = 0; mi < c->m.size(); ++mi) {
                 const ModelHost &h = c->m[mi];
@@ -432,6 +435,9 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         }
         h.fastdiv = fast_div_exact(d.dh);
         c->fastdiv = c->fastdiv && h.fastdiv;
+        // T starts at 2*TIME and only decreases, so the guard TT >= INF
+        // (INF = 9999999, fd.h:29-31) is dead unless 2*TIME reaches it.
+        c->chkinf = c->chkinf || (float)(2.0 * (double)d.time) >= kInf;
         h.rec_off = rec_off;
         h.shot_base = shots;
         h.grad_off = cells;
@@ -722,8 +728,11 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     ModelHost &h = c->m[model];
     if (!T) return fail(c, RZ_ERR_INVALID, "null T");
     const size_t n = (size_t)h.d.nx * h.d.ny;
-    for (size_t i = 0; i < n; ++i)
+    bool big = false;
+    for (size_t i = 0; i < n; ++i) {
         if (!(T[i] >= 0.f)) return fail(c, RZ_ERR_INVALID, "T must be >= 0 (cell %zu)", i);
+        big = big || T[i] >= kInf;
+    }
     ModelDev mh = make_dev(c, h);
     HIPCHK(c, hipMemcpyAsync(c->s_row, T, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     // receiver cells carry the sign bit in the adjoint's copy of T
@@ -744,7 +753,7 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     ShotOut o;
     o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1;
     HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d,
-                             c->s_tres, o, c->fastdiv ? 1 : 0, c->stream));
+                             c->s_tres, o, adj_variant(c) | (big ? 2 : 0), c->stream));
     HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA, c->s_row, 0, c->stream));
     int it = 0;
     if (lam) HIPCHK(c, hipMemcpyAsync(lam, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));

@@ -75,7 +75,7 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
                           const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st);
 hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
                           const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
-                          int fastdiv, hipStream_t st);
+                          int variant, hipStream_t st);   // variant bit 0: fast DH division, bit 1: INF guard
 hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s1, long long rec_off,
                            const float *tmod, const float *tnb, const float *tobs, float *carry, float *tres,
                            double *l2_shot, double *l2_sum, hipStream_t st);

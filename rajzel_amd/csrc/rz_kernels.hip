@@ -142,7 +142,9 @@ __device__ __forceinline__ float godunov(float a, float b, float s, float dh, do
     if (fabsf(d) >= sdh) return mn(a, b) + sdh;
     const double sd = (double)s;
     const double dd = (double)d;
-    const double disc = (sd * sd) * two_dh2 - dd * dd;
+    // (a-b)^2 is exact in double, so fma(-dd, dd, .) rounds once, like the
+    // reference's subtraction of pow((a-b),2.0).
+    const double disc = __builtin_fma(-dd, dd, (sd * sd) * two_dh2);
     const double ab = (double)(a + b);
     double half = (ab + sqrt_nr(disc)) * 0.5;
     constexpr unsigned kGuard = 16;
@@ -384,7 +386,7 @@ __device__ __forceinline__ float neg_part(float x) { return (x - fabsf(x)) * 0.5
 //   Lin   lambda in the same layout; outputs as in fwd_sweep, with the same
 //         fixed per-step set of memory operations.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool WOTHER, bool WSAME, bool FD>
+template <int R, int SX, int SY, bool WOTHER, bool WSAME, bool FD, bool CHKINF>
 __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
                             const float *Lin, float *Lother, float *Lsame)
 {
@@ -527,7 +529,8 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     bmp[r] = pos_part(bm);
                     bpm[r] = neg_part(bp);
                     keep[r] = !cvalid || fcol || frow[r] || rec;
-                    infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
+                    if constexpr (CHKINF) infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
+                    else infv[r] = false;
                 }
                 float fsum = 0.f;
 #pragma unroll
@@ -595,7 +598,8 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_forward(const Mode
     const int nwaves = blockDim.x >> 6;
     WaveCtx w;
     w.nx = md.nx; w.ny = md.ny; w.nb = md.nb;
-    w.beta = threadIdx.x >> 6; w.lane = threadIdx.x & 63;
+    w.beta = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform band
+    w.lane = threadIdx.x & 63;
     w.active = w.beta < md.nb;
     w.bstride = md.bstride;
     w.ring = ring;
@@ -658,7 +662,7 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_forward(const Mode
 // Adjoint kernel: one workgroup per shot (eikonal_adj.c:10-59 with
 // init_travel_adj.c as an O(ntr) scatter).
 // ---------------------------------------------------------------------------
-template <int R, bool FD>
+template <int R, bool FD, bool CHKINF>
 __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const ModelDev *__restrict__ mds,
                                                          const ShotDev *__restrict__ shots, Slots slots,
                                                          const long long *__restrict__ rec_off,
@@ -673,7 +677,8 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const Mode
     const int nwaves = blockDim.x >> 6;
     WaveCtx w;
     w.nx = md.nx; w.ny = md.ny; w.nb = md.nb;
-    w.beta = threadIdx.x >> 6; w.lane = threadIdx.x & 63;
+    w.beta = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform band
+    w.lane = threadIdx.x & 63;
     w.active = w.beta < md.nb;
     w.bstride = md.bstride;
     w.ring = ring;
@@ -701,10 +706,10 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const Mode
     int it = 0;
     while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
-        double acc = adj_sweep<R, +1, +1, true, false, FD>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB, nullptr);
-        acc += adj_sweep<R, +1, -1, false, true, FD>(w, md.dh, md.rdh, md.eps_adj, TB, LB, nullptr, LB);
-        acc += adj_sweep<R, -1, +1, true, false, FD>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA, nullptr);
-        acc += adj_sweep<R, -1, -1, false, true, FD>(w, md.dh, md.rdh, md.eps_adj, TA, LA, nullptr, LA);
+        double acc = adj_sweep<R, +1, +1, true, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB, nullptr);
+        acc += adj_sweep<R, +1, -1, false, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, nullptr, LB);
+        acc += adj_sweep<R, -1, +1, true, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA, nullptr);
+        acc += adj_sweep<R, -1, -1, false, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, nullptr, LA);
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
         ++it;
@@ -893,16 +898,20 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
 
 hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
                           const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
-                          int fastdiv, hipStream_t st)
+                          int variant, hipStream_t st)
 {
     if (nshots <= 0) return hipSuccess;
-    if (fastdiv) {
-        RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, true>), dim3(nshots), dim3(nb_max * kWave), 0, st, md,
-                                            shots, slots, rec_off, shot_base, tres, out));
-    } else {
-        RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, false>), dim3(nshots), dim3(nb_max * kWave), 0, st, md,
-                                            shots, slots, rec_off, shot_base, tres, out));
+    const dim3 g(nshots), b(nb_max * kWave);
+#define RZ_ADJ_LAUNCH(FD_, CI_)                                                                    \
+    RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, FD_, CI_>), g, b, 0, st, md, shots, slots, \
+                                        rec_off, shot_base, tres, out))
+    switch (variant & 3) {
+    case 0: RZ_ADJ_LAUNCH(false, false); break;
+    case 1: RZ_ADJ_LAUNCH(true, false); break;
+    case 2: RZ_ADJ_LAUNCH(false, true); break;
+    default: RZ_ADJ_LAUNCH(true, true); break;
     }
+#undef RZ_ADJ_LAUNCH
     return hipGetLastError();
 }
 

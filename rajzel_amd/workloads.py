@@ -216,18 +216,24 @@ def c3_instances(nmodels=32, shots_per_model=8):
 
 
 def small_case(nx=53, ny=37, dh=10.0, nshots=3, seed=7, ntr=None, rec_row=2):
-    """Tiny seeded case for golden vectors and fast parity tests."""
+    """Tiny seeded case for golden vectors and fast parity tests.
+
+    ``rec_row`` is the 1-based receiver row (depth rec_row*DH), or a sequence
+    of rows assigned to the receivers in turn (e.g. (2, ny-1) alternates a
+    surface line with a line at depth, a transmission geometry in which the
+    adjoint field fills the model)."""
     rng = np.random.default_rng(seed)
     X, Z = _grid_coords(nx, ny, dh)
     vtrue = 1500.0 + 0.5 * Z + _gaussians(rng, X, Z, 3, -300.0, 300.0, 3 * dh, 8 * dh, dh, nx, ny)
     cols = rng.integers(1, nx + 1, size=nshots)
     ntr = nx if ntr is None else ntr
     rc = np.linspace(1, nx, ntr).round().astype(np.int64)
+    rows = np.resize(np.atleast_1d(np.asarray(rec_row, np.int64)), ntr)
     return Survey(
         name=f"small_{ny}x{nx}", nx=nx, ny=ny, dh=dh,
         vp=gradient_model(nx, ny, dh, 1500.0, 0.5), vp_true=vtrue.astype(np.float32),
         src_x_m=(cols * dh).astype(np.float32), src_z_m=np.full(nshots, dh, np.float32),
-        rec_x_m=(rc * dh).astype(np.float32), rec_z_m=np.full(ntr, rec_row * dh, np.float32),
+        rec_x_m=(rc * dh).astype(np.float32), rec_z_m=(rows * dh).astype(np.float32),
         noise_s=0.002, meta={"seed": seed},
     )
 

@@ -1,0 +1,128 @@
+#!/usr/bin/env python3
+"""Write the golden vectors of tests/golden/*.npz from the reference itself.
+
+Every expected output here comes from the reference's own C sources compiled
+by oracle/Makefile (oracle/_ref/librzref.so, driven by oracle/ref_harness.c);
+inputs are seeded synthetic data.  Run from the repository root:
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Cases
+  sweeps.npz      one reference sweep() in each of the 4 directions from a
+                  random T on a 17x23 grid (borders and corners included)
+  sweeps_adj.npz  one reference sweep_adj() in each direction, with receiver
+                  cells (recflag) and cells at TT >= INF
+  ass_grad.npz    ass_grad() + cp_grad_frame() on random lambda, S, grad
+  eval_small.npz  grad_obj's shot loop, 37x53, 4 shots, one muted pick per
+                  shot 1..2 and a non-zero initial Tres (carry)
+                  (receivers alternate between row 2 and a line at depth)
+  eval_bands.npz  the same on a 150x70 grid (several row bands on the GPU)
+  eval_surface.npz  surface receivers only (row 2), as the reference's own
+                  example acquisition
+  c1_forward.npz  eikonal() of the 200x600 forward configuration: receiver
+                  times, iterations and the full traveltime grid
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle.bindings import Reference  # noqa: E402
+from rajzel_amd import workloads as W  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def sweeps(ref):
+    rng = np.random.default_rng(101)
+    ny, nx, dh = 17, 23, 10.0
+    S = (1.0 / rng.uniform(1500, 4000, (ny, nx))).astype(np.float32)
+    T0 = rng.uniform(0.0, 0.05, (ny, nx)).astype(np.float32)
+    ref.setup(nx, ny, dh, 6.0, 1e-10, 1e-9)
+    out = {"S": S, "T0": T0, "dh": np.float32(dh)}
+    for sx, sy in [(1, 1), (-1, 1), (-1, -1), (1, -1)]:
+        out[f"T_{sx:+d}{sy:+d}"] = ref.sweep(S, T0, sx, sy)
+    np.savez_compressed(os.path.join(OUT, "sweeps.npz"), **out)
+
+
+def sweeps_adj(ref):
+    rng = np.random.default_rng(202)
+    ny, nx, dh = 19, 21, 5.0
+    S = (1.0 / rng.uniform(1500, 4000, (ny, nx))).astype(np.float32)
+    ref.setup(nx, ny, dh, 6.0, 1e-10, 1e-9)
+    T, _, _ = ref.eikonal(S, 10 * dh, 1 * dh)
+    T = T.copy()
+    T[12:14, 15:17] = np.float32(2.0e7)       # >= INF
+    rec_ix = np.array([3, 7, 7, 12, 0, 20, 9], np.int32)   # one duplicate cell, frame cells
+    rec_iy = np.array([1, 1, 1, 4, 3, 5, 18], np.int32)
+    flag = ref.init_recflag(rec_ix, rec_iy)
+    lam0 = rng.uniform(-0.01, 0.01, (ny, nx)).astype(np.float32)
+    out = {"S": S, "T": T, "rec_ix": rec_ix, "rec_iy": rec_iy, "flag": flag, "lam0": lam0,
+           "dh": np.float32(dh), "eps_adj": np.float32(1e-9)}
+    for sx, sy in [(1, 1), (1, -1), (-1, 1), (-1, -1)]:
+        out[f"lam_{sx:+d}{sy:+d}"] = ref.sweep_adj(lam0, T, flag, sx, sy)
+    np.savez_compressed(os.path.join(OUT, "sweeps_adj.npz"), **out)
+
+
+def ass_grad(ref):
+    rng = np.random.default_rng(303)
+    ny, nx = 13, 16
+    S = (1.0 / rng.uniform(1500, 4000, (ny, nx))).astype(np.float32)
+    lam = rng.normal(0, 1e-2, (ny, nx)).astype(np.float32)
+    g0 = rng.normal(0, 1e-9, (ny, nx)).astype(np.float32)
+    ref.setup(nx, ny, 10.0)
+    gs, g = ref.ass_grad(S, lam, g0)
+    np.savez_compressed(os.path.join(OUT, "ass_grad.npz"), S=S, lam=lam, grad0=g0, grad_shot=gs, grad=g)
+
+
+def evaluation(ref, name, nx, ny, nshots, seed, mute, rec_row):
+    sv = W.small_case(nx=nx, ny=ny, nshots=nshots, seed=seed, rec_row=rec_row)
+    ref.setup(nx, ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    # picks: reference forward in the true model + noise, as written to and read back from pick files
+    St = W.slowness(sv.vp_true)
+    tm = np.stack([ref.eikonal(St, float(sv.src_x_m[s]), float(sv.src_z_m[s]), sv.rec_ix, sv.rec_iy)[1]
+                   for s in range(nshots)])
+    rng = np.random.default_rng(seed + 1)
+    tobs = (tm + rng.normal(0, 0.002, tm.shape)).astype(np.float32)
+    for (s, i) in mute:
+        tobs[s, i] = -1.0
+    tres0 = np.linspace(-1e-3, 1e-3, sv.ntr).astype(np.float32)
+    out = ref.grad_obj(sv.S, sv.src_x_m, sv.src_z_m, sv.rec_ix, sv.rec_iy, tobs, tres0=tres0)
+    np.savez_compressed(
+        os.path.join(OUT, f"{name}.npz"),
+        nx=nx, ny=ny, dh=np.float32(sv.dh), time=np.float32(sv.time), ttnorm=np.float32(sv.ttnorm),
+        eps_adj=np.float32(sv.eps_adj), S=sv.S, src_x_m=sv.src_x_m, src_z_m=sv.src_z_m,
+        src_ix=sv.src_ix, src_iy=sv.src_iy, rec_ix=sv.rec_ix, rec_iy=sv.rec_iy, tres0=tres0,
+        tobs=out["tobs"], grad=out["grad"], T_last=out["T"], lam_last=out["lam"], tmod=out["tmod"],
+        tres=out["tres"], tres_carry=out["tres_carry"], l2_shot=out["l2_shot"], L2=np.float64(out["L2"]),
+        it_f=out["it_f"], it_a=out["it_a"])
+
+
+def c1_forward(ref):
+    sv = W.c1_forward()
+    ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    T, tm, it = ref.eikonal(sv.S, float(sv.src_x_m[0]), float(sv.src_z_m[0]), sv.rec_ix, sv.rec_iy)
+    np.savez_compressed(os.path.join(OUT, "c1_forward.npz"), S=sv.S, src_ix=sv.src_ix,
+                        src_iy=sv.src_iy, rec_ix=sv.rec_ix, rec_iy=sv.rec_iy, dh=np.float32(sv.dh),
+                        T=T, tmod=tm, iters=np.int32(it))
+
+
+def main():
+    ref = Reference()
+    sweeps(ref)
+    sweeps_adj(ref)
+    ass_grad(ref)
+    evaluation(ref, "eval_small", 53, 37, 4, 31, [(1, 5), (2, 5), (2, 11)], (2, 36))
+    evaluation(ref, "eval_bands", 70, 150, 3, 41, [(0, 0), (2, 69)], (2, 120, 149))
+    evaluation(ref, "eval_surface", 53, 37, 3, 51, [(1, 7)], 2)
+    c1_forward(ref)
+    for f in sorted(os.listdir(OUT)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(OUT, f)))
+
+
+if __name__ == "__main__":
+    main()

@@ -18,7 +18,10 @@ def _native():
     return N
 
 
-def _case(nx, ny, nshots, seed, rec_row=2, ntr=None):
+def _case(nx, ny, nshots, seed, rec_row=None, ntr=None):
+    # receivers alternate between row 2 and a line near the bottom: the
+    # adjoint field then spans the model
+    rec_row = (2, ny - 1) if rec_row is None else rec_row
     return W.small_case(nx=nx, ny=ny, nshots=nshots, seed=seed, rec_row=rec_row, ntr=ntr)
 
 
@@ -172,3 +175,22 @@ def test_single_shot_entry_points(ora):
         gs_g, g_g = ctx.ass_grad_shot(0, lam_o, g0)
         gs_o, g_o = ora.ass_grad(sv.S, lam_o, g0)
         assert bits_equal(gs_g, gs_o) and bits_equal(g_g, g_o)
+
+
+def test_adjoint_inf_guard(ora):
+    """sweep_adj.c:68: lambda = 0 next to traveltimes >= INF (9999999)."""
+    sv = _grad_case(ora, 40, 30, 1, seed=23)
+    N = _native()
+    print("[case] adjoint INF guard", flush=True)
+    T, _ = ora.eikonal(sv.S, sv.src_ix[0], sv.src_iy[0], sv.dh)
+    T = T.copy()
+    T[10:14, 20:25] = np.float32(1.5e7)
+    T[25, 5] = np.float32(9999999.0)
+    tres = np.linspace(-2e-3, 2e-3, sv.ntr).astype(np.float32)
+    with N.Context([model_dict(sv, max_iter=50)], rows_per_lane=1) as ctx:
+        ctx.set_slowness(0, sv.S)
+        lam_g, it_g = ctx.eikonal_adj_shot(0, T, tres)
+    lam_o, it_o = \
+        ora.eikonal_adj(T, sv.rec_ix, sv.rec_iy, tres, sv.dh, maxit=50)
+    assert it_g == it_o and bits_equal(lam_g, lam_o), f"{ndiff(lam_g, lam_o)} cells differ"
+    assert np.any(lam_o[9:15, 19:26] == 0)

@@ -1,0 +1,214 @@
+"""Pins of the CPU restatement that do not come from the reference: the
+mathematics of the discretisation fixes them.
+
+* a single sweep is equivariant under transposition of the grid (x and y
+  enter the Godunov update symmetrically and every topological order of a
+  sweep's dependency DAG gives the same bits);
+* along the source row and column of a homogeneous medium the fixed point is
+  the float sum n * S*DH;
+* a heap-ordered fast-marching solve with the same local update reaches the
+  same discrete fixed point, bit for bit;
+* the converged adjoint equals a direct sparse solve of the Leung-Qian
+  equations (eq. 3.6) in double precision;
+* gradient assembly, frame copy and residual scaling have closed forms;
+* a planted sign error in the direct adjoint solve is detected (the check
+  has teeth).
+"""
+import heapq
+
+import numpy as np
+import pytest
+
+from rajzel_amd import workloads as W
+from tests.helpers import bits_equal, ndiff
+
+f32 = np.float32
+
+
+@pytest.mark.parametrize("sx,sy", [(1, 1), (-1, 1), (-1, -1), (1, -1)])
+def test_sweep_transpose_equivariance(ora, sx, sy):
+    rng = np.random.default_rng(11)
+    S = (1.0 / rng.uniform(1500, 4000, (19, 27))).astype(np.float32)
+    T0 = rng.uniform(0, 0.05, (19, 27)).astype(np.float32)
+    a = ora.sweep(S, T0, sx, sy, 10.0)
+    b = ora.sweep(np.ascontiguousarray(S.T), np.ascontiguousarray(T0.T), sy, sx, 10.0)
+    assert bits_equal(a, np.ascontiguousarray(b.T)), f"{ndiff(a, b.T)} cells differ"
+
+
+def test_homogeneous_axes_closed_form(ora):
+    nx, ny, dh = 41, 29, 7.5
+    S = np.full((ny, nx), f32(1.0 / 2500.0), np.float32)
+    isx, isy = 17, 11
+    T, _ = ora.eikonal(S, isx, isy, dh)
+    sdh = f32(S[0, 0] * f32(dh))
+    acc = [f32(0.0)]
+    for _ in range(max(nx, ny)):
+        acc.append(f32(acc[-1] + sdh))
+    acc = np.array(acc, np.float32)
+    for n in range(nx):
+        assert T[isy, n] == acc[abs(n - isx)]
+    for k in range(ny):
+        assert T[k, isx] == acc[abs(k - isy)]
+    assert T[isy, isx] == 0 and np.all(T > 0) == False and np.all(T >= 0)
+
+
+def _godunov(a, b, s, dh):
+    a, b, s, dh = f32(a), f32(b), f32(s), f32(dh)
+    sdh = f32(s * dh)
+    d = f32(a - b)
+    if abs(float(d)) >= float(sdh):
+        return f32(min(a, b) + sdh)
+    disc = (2.0 * (float(s) * float(s))) * (float(dh) * float(dh)) - float(d) * float(d)
+    return f32((float(f32(a + b)) + np.sqrt(disc)) / 2.0)
+
+
+def _fast_marching(S, isx, isy, dh, t0):
+    """Textbook fast marching with the Godunov local solve; not-yet-accepted
+    neighbours count as +inf."""
+    ny, nx = S.shape
+    T = np.full((ny, nx), f32(t0))
+    acc = np.zeros((ny, nx), bool)
+    T[isy, isx] = 0
+    heap = [(0.0, isy, isx)]
+    while heap:
+        _, k, h = heapq.heappop(heap)
+        if acc[k, h]:
+            continue
+        acc[k, h] = True
+        for kk, hh in ((k - 1, h), (k + 1, h), (k, h - 1), (k, h + 1)):
+            if not (0 <= kk < ny and 0 <= hh < nx) or acc[kk, hh]:
+                continue
+
+            def v(k2, h2):
+                ok = 0 <= k2 < ny and 0 <= h2 < nx and acc[k2, h2]
+                return T[k2, h2] if ok else f32(np.inf)
+            a = min(v(kk, hh - 1), v(kk, hh + 1))
+            b = min(v(kk - 1, hh), v(kk + 1, hh))
+            t = _godunov(a, b, S[kk, hh], dh) if np.isfinite(a) and np.isfinite(b) \
+                else f32(min(a, b) + f32(S[kk, hh] * f32(dh)))
+            if t < T[kk, hh]:
+                T[kk, hh] = t
+                heapq.heappush(heap, (float(t), kk, hh))
+    return T
+
+
+@pytest.mark.parametrize("seed,src", [(5, (12, 9)), (6, (0, 0)), (7, (24, 19))])
+def test_forward_equals_fast_marching(ora, seed, src):
+    sv = W.small_case(nx=25, ny=20, nshots=1, seed=seed)
+    S = W.slowness(sv.vp_true)
+    T, _ = ora.eikonal(S, src[0], src[1], sv.dh, ttnorm=1e-30)   # iterate to the exact fixed point
+    Tm = _fast_marching(S, src[0], src[1], sv.dh, 2 * sv.time)
+    assert bits_equal(T, Tm), f"{ndiff(T, Tm)} cells differ"
+
+
+def _leung_qian_direct(T, rx, ry, tres, dh, eps, flip_sign=False):
+    """Solve the Leung-Qian upwind equations (eq. 3.6) directly in double:
+    lhs*lam = rhs(neighbours) inside, lam = tres at receivers, 0 on the frame
+    and where lhs <= EPS_ADJ."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    ny, nx = T.shape
+    T = T.astype(np.float64)
+    n = nx * ny
+    rows, cols, vals = [], [], []
+    b = np.zeros(n)
+    rec = {}
+    for i, (x, y) in enumerate(zip(rx, ry)):
+        if 0 < x < nx - 1 and 0 < y < ny - 1:
+            rec[(y, x)] = float(tres[i])
+    P = (lambda x: max(x, 0.0))
+    M = (lambda x: min(x, 0.0))
+    for k in range(ny):
+        for h in range(nx):
+            i = k * nx + h
+            if k in (0, ny - 1) or h in (0, nx - 1) or (k, h) in rec:
+                rows.append(i); cols.append(i); vals.append(1.0)
+                b[i] = rec.get((k, h), 0.0)
+                continue
+            ap = -(T[k, h + 1] - T[k, h]) / dh
+            am = -(T[k, h] - T[k, h - 1]) / dh
+            bp = -(T[k + 1, h] - T[k, h]) / dh
+            bm = -(T[k, h] - T[k - 1, h]) / dh
+            if flip_sign:
+                ap, am = am, ap
+            lhs = (P(ap) - M(am)) / dh + (P(bp) - M(bm)) / dh
+            rows.append(i); cols.append(i); vals.append(lhs if lhs > eps else 1.0)
+            if lhs > eps:
+                for kk, hh, c in ((k, h - 1, P(am)), (k, h + 1, -M(ap)), (k - 1, h, P(bm)), (k + 1, h, -M(bp))):
+                    rows.append(i); cols.append(kk * nx + hh); vals.append(-c / dh)
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(n, n))
+    return spl.spsolve(A, b).reshape(ny, nx)
+
+
+def test_adjoint_equals_direct_solve(ora):
+    sv = W.small_case(nx=30, ny=24, nshots=1, seed=9, rec_row=20)
+    S = W.slowness(sv.vp_true)
+    T, _ = ora.eikonal(S, sv.src_ix[0], sv.src_iy[0], sv.dh, ttnorm=1e-30)
+    tres = np.random.default_rng(3).normal(0, 1e-3, sv.ntr).astype(np.float32)
+    lam, _ = ora.eikonal_adj(T, sv.rec_ix, sv.rec_iy, tres, sv.dh, ttnorm=1e-30, maxit=500)
+    direct = _leung_qian_direct(T, sv.rec_ix, sv.rec_iy, tres, sv.dh, 1e-9)
+    assert np.count_nonzero(lam) > 400
+    rel = np.linalg.norm(lam - direct) / np.linalg.norm(direct)
+    assert rel < 1e-6, rel
+    # a swapped pair of one-sided slopes must be caught by this comparison
+    wrong = _leung_qian_direct(T, sv.rec_ix, sv.rec_iy, tres, sv.dh, 1e-9, flip_sign=True)
+    assert np.linalg.norm(lam - wrong) / np.linalg.norm(wrong) > 1e-2
+
+
+def test_gradient_assembly_closed_form(ora):
+    rng = np.random.default_rng(4)
+    ny, nx = 11, 14
+    S = (1.0 / rng.uniform(1500, 4000, (ny, nx))).astype(np.float32)
+    lam = rng.normal(0, 1e-2, (ny, nx)).astype(np.float32)
+    g0 = rng.normal(0, 1e-9, (ny, nx)).astype(np.float32)
+    gs, g = ora.ass_grad(S, lam, g0)
+    s = S.astype(np.float64)
+    inner = (lam.astype(np.float64) * (s * s * s)).astype(np.float32)
+    kk = np.clip(np.arange(ny), 2, ny - 3)
+    cc = np.clip(np.arange(nx), 2, nx - 3)
+    expect = inner[np.ix_(kk, cc)]
+    assert bits_equal(gs, expect)
+    assert bits_equal(g, (g0 + expect).astype(np.float32))
+
+
+def test_residual_closed_form(ora):
+    rng = np.random.default_rng(5)
+    ny, nx, dh = 9, 12, 10.0
+    T = np.cumsum(rng.uniform(1e-4, 3e-3, (ny, nx)), axis=0).astype(np.float32)
+    T[4, 6] = T[3, 6]                      # zero vertical slope below row 3, column 6
+    rx = np.array([1, 2, 6, 7, 8], np.int32)
+    ry = np.array([0, 3, 3, ny - 1, 5], np.int32)
+    tmod = T[ry, rx]
+    tobs = (tmod + rng.normal(0, 1e-3, 5)).astype(np.float32)
+    tobs[4] = -1.0                         # muted: keeps the carried value
+    carry = np.arange(5, dtype=np.float32)
+    tres, l2 = ora.calc_fa_res(T, rx, ry, tmod, tobs, carry, dh)
+    expect = carry.copy()
+    l2e = 0.0
+    for i in range(5):
+        if tobs[i] < 0:
+            continue
+        r = f32(tmod[i] - tobs[i])
+        l2e += float(f32(r * r))
+        k, h = ry[i], rx[i]
+        g = f32(f32(T[k + 1, h] - T[k, h]) / f32(dh)) if k <= ny - 2 else f32(f32(T[k, h] - T[k - 1, h]) / f32(dh))
+        expect[i] = f32(r / g) if g != 0 else f32(0)
+    assert bits_equal(tres, expect) and l2 == l2e
+    assert tres[2] == 0 and tres[4] == carry[4]
+
+
+def test_fp64_build_close_to_fp32(ora, ora64):
+    """The float path against the all-double build of the same restatement:
+    traveltimes within 1e-5 s; the figures are recorded in DESIGN.md."""
+    sv = W.small_case(nx=60, ny=40, nshots=3, seed=12, rec_row=(2, 39))
+
+    def fwd(S, s):
+        return ora.forward(S, s.src_ix, s.src_iy, s.rec_ix, s.rec_iy, s.dh)[0]
+    W.synthesize_picks(sv, fwd)
+    a = ora.grad_obj(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.tobs, sv.dh)
+    b = ora64.grad_obj(sv.S.astype(np.float64), sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy,
+                       sv.tobs.astype(np.float64), sv.dh)
+    assert np.max(np.abs(a["T"] - b["T"])) < 1e-5
+    assert np.max(np.abs(a["tmod"] - b["tmod"])) < 1e-5
+    rel = np.linalg.norm(a["grad"] - b["grad"]) / np.linalg.norm(b["grad"])
+    assert rel < 1e-2, rel
