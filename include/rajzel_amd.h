@@ -79,9 +79,11 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
 void rz_destroy(rz_ctx *ctx);
 const char *rz_last_error(const rz_ctx *ctx);
 
-/* Use an existing HIP stream (hipStream_t) for all work; NULL = the
- * context's own stream. */
+/* Run all further work on the HIP stream `hip_stream` (a hipStream_t);
+ * NULL selects HIP's null stream.  Until the first call the context uses a
+ * non-blocking stream of its own, which rz_get_stream returns. */
 int rz_set_stream(rz_ctx *ctx, void *hip_stream);
+void *rz_get_stream(const rz_ctx *ctx);
 
 /* Slowness S = 1/Vp (calc_S.c:18-24), NY*NX.  Replaces the reference's S
  * matrix passed to grad_obj (fd.h:72).  Rejected unless finite and > 0. */
@@ -111,7 +113,22 @@ int rz_gradient(rz_ctx *ctx);
  * reference's shot loop with each routine a launch).  Same results. */
 int rz_gradient_per_shot(rz_ctx *ctx);
 
-/* Results of the last rz_forward / rz_gradient. */
+/* Adjoint source given directly: dT (nshots*ntr, shot-major) replaces the
+ * residual Tmod - Tobs of calc_FA_res.c:42 (no pick is muted); the slope
+ * scaling of calc_FA_res.c:52-69, the adjoint and the gradient assembly are
+ * unchanged.  With dT = dL/dTmod the gradient is the reference's adjoint-
+ * state gradient of L.  No counterpart in the reference (its residual always
+ * comes from picks); it serves autograd callers. */
+int rz_set_residuals(rz_ctx *ctx, int model, const float *dT, int on_device);
+
+/* Residual scaling + adjoint + gradient for the rz_set_residuals sources.
+ * Reuses the forward fields of the last rz_forward when every shot's fields
+ * are still resident and the slowness is unchanged; otherwise recomputes the
+ * forward per batch (HBM budget).  Receiver times of the recomputation are
+ * identical to the earlier ones. */
+int rz_adjoint(rz_ctx *ctx);
+
+/* Results of the last rz_forward / rz_gradient / rz_adjoint. */
 int rz_get_tmod(rz_ctx *ctx, int model, float *out, int on_device);      /* nshots*ntr */
 int rz_get_tres(rz_ctx *ctx, int model, float *out, int on_device);      /* nshots*ntr */
 int rz_get_grad(rz_ctx *ctx, int model, float *out, int on_device);      /* NY*NX      */

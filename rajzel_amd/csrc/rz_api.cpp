@@ -28,7 +28,7 @@ struct ModelHost {
     int shot_base = 0;       // global index of shot 0
     long long grad_off = 0;  // offset in the gradient buffer
     int carry_off = 0;       // offset in the carry buffer
-    bool have_S = false, have_picks = false;
+    bool have_S = false, have_picks = false, have_res = false;
     bool fastdiv = false;
     std::vector<float> tres0;
     // device
@@ -55,6 +55,9 @@ struct rz_ctx {
     long long *rec_off_d = nullptr, *rec_off0_d = nullptr;
     int *shot_base_d = nullptr, *shot_base0_d = nullptr;
     float *tobs_d = nullptr, *tmod_d = nullptr, *tnb_d = nullptr, *tres_d = nullptr, *carry_d = nullptr;
+    float *dres_d = nullptr;            // residuals given by rz_set_residuals
+    long long s_epoch = 0;              // bumped by every rz_set_slowness
+    long long fields_epoch = -1;        // slowness epoch of resident forward fields of all shots
     float *grad_d = nullptr;
     double *l2_shot_d = nullptr, *l2_sum_d = nullptr;
     int *it_f_d = nullptr, *it_a_d = nullptr;
@@ -264,9 +267,17 @@ int collect(rz_ctx *c, const std::vector<PhaseMarks> &marks, bool grad)
 
 // Run forward (and, with grad, residual + adjoint + gradient) over the
 // global shot order in batches of at most `cap` shots.
-int run(rz_ctx *c, bool grad, int cap)
+int run(rz_ctx *c, bool grad, int cap, bool direct = false)
 {
-    TRY(ready(c, grad));
+    if (direct) {
+        for (size_t i = 0; i < c->m.size(); ++i)
+            if (!c->m[i].have_res && c->m[i].d.nshots > 0 && c->m[i].d.ntr > 0)
+                return fail(c, RZ_ERR_STATE, "model %zu: residuals not set", i);
+    }
+    TRY(ready(c, grad && !direct));
+    // forward fields of every shot are still resident and current: the
+    // adjoint-only run does not recompute them
+    const bool reuse = direct && c->fields_epoch == c->s_epoch && c->nslots >= c->total_shots;
     HIPCHK(c, hipSetDevice(c->device));
     const Slots sl = slots_of(c);
     const ShotOut so = outs_of(c);
@@ -291,8 +302,9 @@ int run(rz_ctx *c, bool grad, int cap)
         const int b1 = std::min(c->total_shots, b0 + per);
         PhaseMarks pm;
         TRY(mark(c, &pm.e[0]));
-        HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
-                                 c->shot_base_d, so, c->stream));
+        if (!reuse)
+            HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+                                     c->shot_base_d, so, c->stream));
         TRY(mark(c, &pm.e[1]));
         if (grad) {
             for (size_t mi = 0; mi < c->m.size(); ++mi) {
@@ -301,8 +313,8 @@ int run(rz_ctx *c, bool grad, int cap)
                 const int s1 = std::min(b1, h.shot_base + h.d.nshots) - h.shot_base;
                 if (s1 <= s0 || h.d.ntr <= 0) continue;
                 HIPCHK(c, launch_residual(c->md_d, (int)mi, h.d.ntr, s0, s1, h.rec_off, c->tmod_d, c->tnb_d,
-                                          c->tobs_d, c->carry_d + h.carry_off, c->tres_d, c->l2_shot_d + h.shot_base,
-                                          c->l2_sum_d + mi, c->stream));
+                                          c->tobs_d, direct ? c->dres_d : nullptr, c->carry_d + h.carry_off,
+                                          c->tres_d, c->l2_shot_d + h.shot_base, c->l2_sum_d + mi, c->stream));
             }
             TRY(mark(c, &pm.e[2]));
             HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
@@ -326,6 +338,9 @@ int run(rz_ctx *c, bool grad, int cap)
     TRY(collect(c, marks, grad));
     c->have_fwd = true;
     c->have_grad = grad;
+    // all shots' forward fields stay in their slots only without batching
+    if (c->nslots >= c->total_shots && !(grad && cap < c->nslots)) c->fields_epoch = c->s_epoch;
+    else if (!reuse) c->fields_epoch = -1;
     return check_convergence(c, grad);
 }
 
@@ -355,7 +370,8 @@ void rz_destroy(rz_ctx *c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);   // also fine for the null stream
+    if (c->own && c->own != c->stream) (void)hipStreamSynchronize(c->own);
     for (void *p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
@@ -483,6 +499,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         (rc = dalloc(c, &c->shot_base0_d, nm)) || (rc = dalloc(c, &c->tobs_d, rec_off)) ||
         (rc = dalloc(c, &c->tmod_d, rec_off)) || (rc = dalloc(c, &c->tnb_d, rec_off)) ||
         (rc = dalloc(c, &c->tres_d, rec_off)) || (rc = dalloc(c, &c->carry_d, carry)) ||
+        (rc = dalloc(c, &c->dres_d, rec_off)) ||
         (rc = dalloc(c, &c->grad_d, cells)) || (rc = dalloc(c, &c->l2_shot_d, shots)) ||
         (rc = dalloc(c, &c->l2_sum_d, nm)) || (rc = dalloc(c, &c->it_f_d, shots)) ||
         (rc = dalloc(c, &c->it_a_d, shots)) || (rc = dalloc(c, &c->err_d, 1)))
@@ -552,9 +569,11 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
 int rz_set_stream(rz_ctx *c, void *s)
 {
     if (!c) return RZ_ERR_INVALID;
-    c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    c->stream = static_cast<hipStream_t>(s);   // NULL: HIP's null stream
     return RZ_OK;
 }
+
+void *rz_get_stream(const rz_ctx *c) { return c ? static_cast<void *>(c->stream) : nullptr; }
 
 int rz_set_slowness(rz_ctx *c, int model, const float *S, int on_device)
 {
@@ -575,6 +594,7 @@ int rz_set_slowness(rz_ctx *c, int model, const float *S, int on_device)
         return fail(c, RZ_ERR_INVALID, "model %d: slowness must be finite and > 0 everywhere", model);
     }
     h.have_S = true;
+    ++c->s_epoch;
     return RZ_OK;
 }
 
@@ -617,6 +637,26 @@ int rz_gradient_per_shot(rz_ctx *c)
 {
     if (!c) return RZ_ERR_INVALID;
     return run(c, true, 1);
+}
+
+int rz_set_residuals(rz_ctx *c, int model, const float *dT, int on_device)
+{
+    TRY(check_model(c, model));
+    ModelHost &h = c->m[model];
+    const size_t n = (size_t)h.d.nshots * h.d.ntr;
+    if (n == 0) { h.have_res = true; return RZ_OK; }
+    if (!dT) return fail(c, RZ_ERR_INVALID, "null residuals");
+    HIPCHK(c, hipMemcpyAsync(c->dres_d + h.rec_off, dT, n * sizeof(float),
+                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    h.have_res = true;
+    return RZ_OK;
+}
+
+int rz_adjoint(rz_ctx *c)
+{
+    if (!c) return RZ_ERR_INVALID;
+    return run(c, true, c->nslots, true);
 }
 
 int rz_get_tmod(rz_ctx *c, int model, float *out, int on_device)
@@ -717,6 +757,7 @@ int rz_eikonal_shot(rz_ctx *c, int model, int src_ix, int src_iy, float *T, floa
     HIPCHK(c, hipMemcpyAsync(&it, c->s_it, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->last_b0 = c->last_b1 = -1;
+    c->fields_epoch = -1;
     if (iters) *iters = it;
     if (it < 0) return fail(c, RZ_ERR_NOT_CONVERGED, "forward did not converge within max_iter");
     return RZ_OK;
@@ -760,6 +801,7 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     HIPCHK(c, hipMemcpyAsync(&it, c->s_it + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->last_b0 = c->last_b1 = -1;
+    c->fields_epoch = -1;
     if (iters) *iters = it;
     if (it < 0) return fail(c, RZ_ERR_NOT_CONVERGED, "adjoint did not converge within max_iter");
     return RZ_OK;
@@ -780,6 +822,7 @@ int rz_ass_grad_shot(rz_ctx *c, int model, const float *lam, float *grad_shot, f
     HIPCHK(c, hipMemcpyAsync(gs.data(), c->s_row2, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->last_b0 = c->last_b1 = -1;
+    c->fields_epoch = -1;
     if (grad_shot) std::memcpy(grad_shot, gs.data(), n * sizeof(float));
     if (grad)
         for (size_t i = 0; i < n; ++i) grad[i] += gs[i];   // ass_grad.c:40-46

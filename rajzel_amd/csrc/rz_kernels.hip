@@ -725,19 +725,21 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const Mode
 // ---------------------------------------------------------------------------
 __global__ void k_residual(const ModelDev *__restrict__ mds, int model, int ntr, int s0, int s1,
                            long long rec_off, const float *__restrict__ tmod, const float *__restrict__ tnb,
-                           const float *__restrict__ tobs, float *carry, float *tres, double *l2_shot,
-                           double *l2_sum)
+                           const float *__restrict__ tobs, const float *__restrict__ direct, float *carry,
+                           float *tres, double *l2_shot, double *l2_sum)
 {
+    // direct != nullptr: the residual Tmod - Tobs is given (an adjoint source
+    // dL/dTmod); no pick is muted and the slope scaling is applied as usual.
     const ModelDev md = mds[model];
     for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
         float cr = carry[i];
         const int k = md.rec_k[i];
         for (int s = s0; s < s1; ++s) {
             const long long x = rec_off + (long long)s * ntr + i;
-            const float to = tobs[x];
+            const float to = direct ? 0.f : tobs[x];
             if (to >= 0.f) {
                 const float tm = tmod[x];
-                float r = tm - to;
+                float r = direct ? direct[x] : tm - to;
                 float g = 0.f;
                 bool have = true;
                 if (k <= md.ny - 2) g = (tnb[x] - tm) / md.dh;
@@ -755,9 +757,9 @@ __global__ void k_residual(const ModelDev *__restrict__ mds, int model, int ntr,
         double l2 = 0.0;
         for (int i = 0; i < ntr; ++i) {
             const long long x = rec_off + (long long)s * ntr + i;
-            const float to = tobs[x];
+            const float to = direct ? 0.f : tobs[x];
             if (to >= 0.f) {
-                const float r = tmod[x] - to;
+                const float r = direct ? direct[x] : tmod[x] - to;
                 l2 += (double)(r * r);
             }
         }
@@ -916,11 +918,11 @@ hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *
 }
 
 hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s1, long long rec_off,
-                           const float *tmod, const float *tnb, const float *tobs, float *carry, float *tres,
-                           double *l2_shot, double *l2_sum, hipStream_t st)
+                           const float *tmod, const float *tnb, const float *tobs, const float *direct,
+                           float *carry, float *tres, double *l2_shot, double *l2_sum, hipStream_t st)
 {
     hipLaunchKernelGGL(k_residual, dim3(1), dim3(256), 0, st, md, model, ntr, s0, s1, rec_off, tmod, tnb, tobs,
-                       carry, tres, l2_shot, l2_sum);
+                       direct, carry, tres, l2_shot, l2_sum);
     return hipGetLastError();
 }
 

@@ -23,7 +23,7 @@ EXPORTS = (
     "rz_get_tmod", "rz_get_tres", "rz_get_grad", "rz_get_misfit", "rz_get_iterations",
     "rz_get_tres_carry", "rz_get_field", "rz_eikonal_shot", "rz_eikonal_adj_shot", "rz_ass_grad_shot",
     "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
-    "rz_get_profile",
+    "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream",
 )
 
 
@@ -47,14 +47,19 @@ class ModelDesc(C.Structure):
 
 
 _lib = None
+_loaded_after_torch = False
 _lock = threading.Lock()
 
 
 def lib():
     """Load the library (raises if it has not been built)."""
-    global _lib
+    global _lib, _loaded_after_torch
     with _lock:
         if _lib is None:
+            # torch bundles its own libamdhip64.so.7; whichever is loaded first
+            # serves the whole process (same soname), so record the order.
+            import sys
+            _loaded_after_torch = "torch" in sys.modules
             if not os.path.exists(LIB_PATH):
                 raise RzError(-1, f"{LIB_PATH} not built: run __graft_entry__.build() or "
                                   "python rajzel_amd/build.py")
@@ -88,6 +93,9 @@ def lib():
                 "rz_batch_shots": (i, [vp]),
                 "rz_set_profiling": (i, [vp, i]),
                 "rz_get_profile": (i, [vp, vp, vp]),
+                "rz_set_residuals": (i, [vp, i, vp, i]),
+                "rz_adjoint": (i, [vp]),
+                "rz_get_stream": (vp, [vp]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -167,7 +175,11 @@ class Context:
 
     # -- inputs --------------------------------------------------------------
     def set_stream(self, stream_handle):
+        """Run on the hipStream_t `stream_handle` (int); 0 / None = HIP's null stream."""
         self._chk(lib().rz_set_stream(self.h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    def get_stream(self):
+        return lib().rz_get_stream(self.h) or 0
 
     def set_slowness(self, model, S):
         """S: numpy (ny, nx) float32, or a device pointer (int)."""
@@ -187,6 +199,18 @@ class Context:
         tobs = np.ascontiguousarray(tobs, np.float32).reshape(ns, ntr)
         self._chk(lib().rz_set_picks(self.h, model, _ptr(tobs), 0))
 
+    def set_residuals(self, model, dT):
+        """Adjoint source dL/dTmod, numpy (nshots, ntr) or a device pointer."""
+        if isinstance(dT, int):
+            self._chk(lib().rz_set_residuals(self.h, model, C.c_void_p(dT), 1))
+            return
+        ns, ntr = self.shapes[model][2:]
+        dT = np.ascontiguousarray(dT, np.float32).reshape(ns, ntr)
+        self._chk(lib().rz_set_residuals(self.h, model, _ptr(dT), 0))
+
+    def adjoint(self):
+        self._chk(lib().rz_adjoint(self.h))
+
     def set_tres_carry(self, model, tres0):
         t = None if tres0 is None else np.ascontiguousarray(tres0, np.float32)
         self._chk(lib().rz_set_tres_carry(self.h, model, _ptr(t)))
@@ -202,8 +226,11 @@ class Context:
         self._chk(lib().rz_sync(self.h))
 
     # -- outputs -------------------------------------------------------------
-    def tmod(self, model=0):
+    def tmod(self, model=0, device_ptr=None):
         ny, nx, ns, ntr = self.shapes[model]
+        if device_ptr is not None:
+            self._chk(lib().rz_get_tmod(self.h, model, C.c_void_p(device_ptr), 1))
+            return None
         out = np.empty((ns, ntr), np.float32)
         self._chk(lib().rz_get_tmod(self.h, model, _ptr(out), 0))
         return out

@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# torch ships its own HIP runtime (libamdhip64.so.7); loading it before
+# librajzel_amd.so makes every test in the session share that one runtime.
+try:
+    import torch  # noqa: F401
+except Exception:  # torch is optional for everything but the autograd tests
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

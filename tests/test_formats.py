@@ -1,0 +1,63 @@
+"""The reference's file formats (rajzel_amd/formats.py)."""
+import os
+
+import numpy as np
+import pytest
+
+from rajzel_amd import formats as F
+
+REFPAR = "/root/reference/par"
+
+
+@pytest.mark.skipif(not os.path.exists(REFPAR), reason="reference example files not present")
+def test_parameter_file_of_the_reference_example():
+    P = F.read_par(os.path.join(REFPAR, "RAJZEL_marm_OBC.inp"))
+    expect = dict(INVMAT=0, NX=500, NY=174, DH=20.0, TIME=6.0, READMOD=1, MFILE="start/marmousi_II_marine",
+                  SEISMO=1, TTNORM=np.float32(1e-10), ITERMAX=600, EPS_ADJ=np.float32(1e-9), GRADT1=22,
+                  GRADT4=500, SWS_TAPER_GRAD_HOR=1, SWS_TAPER_CIRCULAR_PER_SHOT=0, SRTRADIUS=80.0, NLBFGS=20,
+                  LINESEARCH=2, C1=np.float32(1e-4), C2=np.float32(0.9), STEPMAX=10, SCALEFAC=2.0, MIN_ITER=0,
+                  VP0_1=1500.0, VP0_2=2501.0, DVP0=1.0, GRAD0_2=np.float32(1.6),
+                  GRIDSEARCH_FILE="./gridsearch/1D_gridsearch.dat", REFREC1=0.0, REFREC2=0.0)
+    for k, v in expect.items():
+        assert P[k] == v, (k, P[k], v)
+    W = F.read_workflow(os.path.join(REFPAR, "FATT_workflow_marmousi.inp"))
+    assert len(W) == 1 and W[0]["SPATFILTER"] == 4 and W[0]["FILT_SIZE_GRAD"] == 80
+    src = F.read_sources(os.path.join(REFPAR, "source", "source_OBC_VSP.dat"))
+    assert len(src["x"]) == 100 and src["x"][1] == 880.0 and src["depth"][0] == 460.0
+    x, y = F.read_receivers(os.path.join(REFPAR, "receiver", "receiver_OBC.dat"))
+    assert len(x) == 400 and x[1] == 820.0 and np.all(y == 460.0)
+
+
+def test_parameter_file_positional_parsing(tmp_path):
+    lines = ["# header", "modus = 1", "# comment", "nx_whatever = 0x10", "NY = 12", "DH = 2.5", "TIME = 3"]
+    p = tmp_path / "p.inp"
+    p.write_text("\n".join(lines) + "\n")
+    P = F.read_par(str(p))
+    assert P["INVMAT"] == 1 and P["NX"] == 16 and P["NY"] == 12 and P["DH"] == np.float32(2.5) and P["TIME"] == 3
+
+
+def test_picks_roundtrip_c_semantics(tmp_path):
+    t = np.array([0.0, 1e-3, 0.123456789, 7.25, -1.0, 3.4028234e38], np.float32)
+    p = str(tmp_path / "picks_1.dat")
+    F.write_picks(p, t)
+    text = open(p).read().splitlines()
+    assert text[1] == "1.000000e-03 " and text[4] == "-1.000000e+00 "
+    back = F.read_picks(p, len(t))
+    assert np.array_equal(back, F.roundtrip_picks(t))
+    assert back[2] == np.float32(0.1234568)
+    with pytest.raises(ValueError):
+        F.read_picks(p, len(t) + 1)
+
+
+def test_model_layout_x_slow_y_fast(tmp_path):
+    vp = np.arange(12, dtype=np.float32).reshape(3, 4)   # (ny, nx)
+    p = str(tmp_path / "m.vp")
+    F.write_model(p, vp)
+    raw = np.fromfile(p, np.float32)
+    assert np.array_equal(raw[:3], vp[:, 0])               # first column (x=1), all y
+    assert np.array_equal(F.read_model(p, 4, 3), vp)
+
+
+def test_receiver_indices_drop_nonpositive_x():
+    ix, iy = F.receiver_indices(np.array([0.0, 4.0, 20.0], np.float32), np.array([10.0, 10.0, 10.0], np.float32), 10.0)
+    assert list(ix) == [1] and list(iy) == [0]
