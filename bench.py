@@ -61,6 +61,39 @@ def algorithmic_bytes(nx, ny, it_f, it_a):
     return fwd, adj, grd
 
 
+def copy_probe(mib):
+    """One device-to-device hipMemcpy of ``mib`` MiB: the plain copy whose
+    FETCH_SIZE / WRITE_SIZE readings calibrate the HBM-traffic counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")            # the runtime librajzel_amd.so already loaded
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), n) != 0:
+            raise RuntimeError("hipMalloc failed in the copy probe")
+    try:
+        if hip.hipMemset(a, 1, n) != 0 or hip.hipMemcpy(b, a, n, 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("copy probe failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def measured_traffic(path, kernel, workload):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), when it was
+    taken on this same workload; None otherwise."""
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    if k is None or rec.get("workload") != workload:
+        return None
+    return float(k["hbm_bytes_per_launch"])
+
+
 def cpu_baseline(sv, S, tobs, nsample, gpu_tmod, gpu_tres, gpu_itf, gpu_ita):
     """Reference C path (oracle/_ref) on the first nsample shots, one thread."""
     from oracle.bindings import Reference, Restatement
@@ -109,12 +142,17 @@ def main():
     ap.add_argument("--rows-per-lane", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="finish with one device-to-device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_fatt_gradient_1001x3001_200shots.json"),
+                    help="counter summary giving the HBM bytes per launch of the dominant kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the gradient path shards by shot, and a multi-GPU run "
               f"(--gpus {args.gpus}) is not part of this build", file=sys.stderr)
         sys.exit(2)
 
+    from rajzel_amd import formats as F
     from rajzel_amd import native as N
     from rajzel_amd import workloads as W
 
@@ -130,7 +168,8 @@ def main():
     ctx.set_slowness(0, S_true)
     ctx.forward()
     rng = np.random.default_rng(sv.meta["seed"] + 1000)
-    sv.tobs = (ctx.tmod(0) + rng.normal(0.0, sv.noise_s, (sv.nshots, sv.ntr))).astype(np.float32)
+    # as written to and read back from the pick files (printf %e, strtof)
+    sv.tobs = F.roundtrip_picks(ctx.tmod(0) + rng.normal(0.0, sv.noise_s, (sv.nshots, sv.ntr)))
     ctx.set_slowness(0, S)
     ctx.set_picks(0, sv.tobs)
     log(f"[bench] setup done, R={ctx.rows_per_lane}, {ctx.device_bytes / 2**30:.2f} GiB on device")
@@ -160,16 +199,18 @@ def main():
     dom = max(("forward", "adjoint"), key=lambda k: phase_ms[k])
     achieved = phase_bytes[dom] / (phase_ms[dom] * 1e-3) / 1e9
     eval_gbs = (fwd_b + adj_b + grd_b) / (ms * 1e-3) / 1e9
+    workload = f"FATT gradient evaluation, {sv.ny}x{sv.nx} grid, DH={sv.dh} m, " \
+               f"{sv.nshots} shots, {sv.ntr} receivers, 12-layer model"
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": f"k_{dom}",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(args.traffic, f"k_{dom}", workload), "kernel": f"k_{dom}",
                 "bytes_per_launch": phase_bytes[dom], "avg_launch_ms": phase_ms[dom]}
 
     out = {
         "metric": METRIC, "value": value, "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"FATT gradient evaluation, {sv.ny}x{sv.nx} grid, DH={sv.dh} m, "
-                               f"{sv.nshots} shots, {sv.ntr} receivers, 12-layer model",
+        "config": {"workload": workload,
                    "nx": sv.nx, "ny": sv.ny, "shots": sv.nshots, "receivers": sv.ntr,
                    "rows_per_lane": ctx.rows_per_lane},
         "roofline": roofline,
@@ -204,6 +245,8 @@ def main():
         out["parity_sample"] = parity
         log(f"[bench] cpu baseline: {cb['value']:.3f} shot-gradients/s; parity {parity}")
     ctx.close()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     print(json.dumps(out), flush=True)
 
 

@@ -21,7 +21,11 @@
  *     cause.  Nothing here aborts the process (the reference's err() does;
  *     the reference-surface shim maps failures onto it).
  *   - Results are bitwise those of the reference's single-rank float path
- *     (one MPI rank), see DESIGN.md for the exceptions.
+ *     (one MPI rank).  One decision is taken differently: the stopping
+ *     test sums |T - Told| in double over the grid (norm1.c:21-29 sums in
+ *     float, column by column), so the iteration count can differ from the
+ *     reference's only when the mean change lies within float rounding of
+ *     TTNORM.  A converged fixed point (no cell changes) is decided alike.
  *
  * A context is single-threaded: calls on one context must not overlap.
  */

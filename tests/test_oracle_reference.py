@@ -1,0 +1,103 @@
+"""The CPU restatement against the reference's own sources, compiled where
+they lie (oracle/_ref/librzref.so, built by oracle/Makefile), on fresh
+seeded inputs: every routine of the path, bit for bit.  The committed
+golden vectors (test_oracle_golden.py) carry the same comparison to
+machines without the reference build; these cases widen it where it is
+present.
+"""
+import numpy as np
+import pytest
+
+from rajzel_amd import workloads as W
+from tests.helpers import bits_equal, ndiff
+
+SHAPES = [(5, 5), (7, 13), (31, 17), (40, 64)]
+DIRS = [(1, 1), (-1, 1), (-1, -1), (1, -1)]
+
+
+def _S(rng, ny, nx):
+    return (1.0 / rng.uniform(1500, 4500, (ny, nx))).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_sweep_all_directions(ora, ref, shape):
+    ny, nx = shape
+    rng = np.random.default_rng(1000 + ny * nx)
+    dh = 7.5
+    ref.setup(nx, ny, dh)
+    S = _S(rng, ny, nx)
+    T0 = rng.uniform(0.0, 0.2, (ny, nx)).astype(np.float32)
+    for sx, sy in DIRS:
+        a = ora.sweep(S, T0, sx, sy, dh)
+        b = ref.sweep(S, T0, sx, sy)
+        assert bits_equal(a, b), f"({sx},{sy}): {ndiff(a, b)} cells differ"
+
+
+@pytest.mark.parametrize("shape", SHAPES[1:])
+@pytest.mark.parametrize("ttnorm", [1e-10, 1e-4])
+def test_eikonal_sources_everywhere(ora, ref, shape, ttnorm):
+    """Sources at corners, edges and inside; a loose TTNORM stops before the
+    fixed point, so the float L1 test itself is compared."""
+    ny, nx = shape
+    rng = np.random.default_rng(2000 + ny)
+    dh = 10.0
+    ref.setup(nx, ny, dh, 6.0, ttnorm)
+    S = _S(rng, ny, nx)
+    for ix, iy in [(0, 0), (nx - 1, ny - 1), (nx // 2, 0), (0, ny // 2), (nx // 3, ny // 2)]:
+        Ta, ita = ora.eikonal(S, ix, iy, dh, ttnorm=ttnorm)
+        Tb, _, itb = ref.eikonal(S, (ix + 1) * dh, (iy + 1) * dh)
+        assert ita == itb
+        assert bits_equal(Ta, Tb), f"source ({ix},{iy}): {ndiff(Ta, Tb)} cells differ"
+
+
+@pytest.mark.parametrize("shape", SHAPES[1:])
+def test_sweep_adj_all_directions(ora, ref, shape):
+    ny, nx = shape
+    rng = np.random.default_rng(3000 + nx)
+    dh = 5.0
+    ref.setup(nx, ny, dh, 6.0, 1e-10, 1e-9)
+    S = _S(rng, ny, nx)
+    T, _ = ora.eikonal(S, nx // 2, 1, dh)
+    T = T.copy()
+    T[ny // 2, nx // 2] = np.float32(1.5e7)          # a cell at or above INF
+    rx = rng.integers(0, nx, 6).astype(np.int32)
+    ry = rng.integers(0, ny, 6).astype(np.int32)
+    fa = ora.init_recflag(nx, ny, rx, ry)
+    fb = ref.init_recflag(rx, ry)
+    assert np.array_equal(fa, fb)
+    lam0 = rng.uniform(-0.02, 0.02, (ny, nx)).astype(np.float32)
+    for sx, sy in DIRS:
+        a = ora.sweep_adj(lam0, T, fa, sx, sy, dh)
+        b = ref.sweep_adj(lam0, T, fb, sx, sy)
+        assert bits_equal(a, b), f"({sx},{sy}): {ndiff(a, b)} cells differ"
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_eikonal_adj_converged(ora, ref, seed):
+    sv = W.small_case(nx=44, ny=30, nshots=1, seed=seed, rec_row=(2, 17, 29))
+    ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    T, _ = ora.eikonal(sv.S, sv.src_ix[0], sv.src_iy[0], sv.dh)
+    tres = np.random.default_rng(seed).normal(0, 2e-3, sv.ntr).astype(np.float32)
+    la, ita = ora.eikonal_adj(T, sv.rec_ix, sv.rec_iy, tres, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    lb, itb = ref.eikonal_adj(T, sv.rec_ix, sv.rec_iy, tres)
+    assert ita == itb
+    assert bits_equal(la, lb), f"{ndiff(la, lb)} cells differ"
+
+
+def test_grad_obj_with_muted_picks_and_carry(ora, ref):
+    sv = W.small_case(nx=48, ny=26, nshots=4, seed=77, rec_row=(2, 25))
+    ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    St = W.slowness(sv.vp_true)
+    tm = np.stack([ref.eikonal(St, float(sv.src_x_m[s]), float(sv.src_z_m[s]), sv.rec_ix, sv.rec_iy)[1]
+                   for s in range(sv.nshots)])
+    rng = np.random.default_rng(5)
+    tobs = (tm + rng.normal(0, 0.002, tm.shape)).astype(np.float32)
+    tobs[0, 3] = tobs[1, 3] = tobs[3, 0] = -1.0
+    tres0 = np.linspace(-2e-3, 2e-3, sv.ntr).astype(np.float32)
+    b = ref.grad_obj(sv.S, sv.src_x_m, sv.src_z_m, sv.rec_ix, sv.rec_iy, tobs, tres0=tres0, workdir="/tmp")
+    a = ora.grad_obj(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, b["tobs"], sv.dh, sv.time, sv.ttnorm,
+                     sv.eps_adj, tres0=tres0)
+    for k in ("grad", "T", "lam", "tmod", "tres"):
+        assert bits_equal(a[k], b[k]), f"{k}: {ndiff(a[k], b[k])} values differ"
+    assert np.array_equal(a["it_f"], b["it_f"]) and np.array_equal(a["it_a"], b["it_a"])
+    assert a["L2"] == b["L2"]
