@@ -43,7 +43,37 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    build_fd_surface(verbose)
     return LIB
+
+
+def _mpicc():
+    for c in (os.environ.get("MPICC"), "/opt/conda/bin/mpicc", "/usr/bin/mpicc"):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fd_surface(verbose: bool = False):
+    """Compile the reference-surface shim csrc/rz_fd_surface.c (C, needs
+    mpi.h) to _build/rz_fd_surface.o when an MPI C compiler is present.  The
+    object is linked into the reference program together with its own
+    sources; it is not part of librajzel_amd.so.  Returns the object path,
+    or None without an MPI compiler."""
+    mpicc = _mpicc()
+    if mpicc is None:
+        return None
+    obj = os.path.join(HERE, "_build", "rz_fd_surface.o")
+    env = dict(os.environ, MPICH_CC=os.environ.get("MPICH_CC", "gcc"))
+    cmd = [mpicc, "-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wextra", "-Werror",
+           "-I" + os.path.join(os.path.dirname(HERE), "include"),
+           "-c", os.path.join(CSRC, "rz_fd_surface.c"), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stderr}")
+    return obj
 
 
 def _run(cmd, verbose):

@@ -1,0 +1,288 @@
+/*
+ * rz_fd_surface.c -- the reference's C entry points of the FATT gradient
+ * path, on top of the MI355X C ABI (rajzel_amd.h).
+ *
+ * Linked into the reference program in place of eikonal.c, sweep.c,
+ * eikonal_adj.c, sweep_adj.c, ass_grad.c, grad_obj.c and forward.c, it
+ * keeps their fd.h signatures, their globals and their side effects, so
+ * that fatt.c, LBFGS.c, PCG.c, wolfels.c, parabolicls.c, step_length_est.c,
+ * grid_search.c and rajzel.c call it unchanged:
+ *
+ *   grad_obj    (fd.h:72, grad_obj.c:10-79)   every shot of this rank in one
+ *               batched evaluation on the GPU; picks read from the
+ *               <DATA_DIR>_<shot>.dat files as calc_FA_res.c:28-37 does;
+ *               MPI sum, frame copy and precond() as grad_obj.c:57-73
+ *   forward     (fd.h:68, forward.c:10-26)    all shots of this rank in one
+ *               launch; SNAP and SEISMO outputs per shot as eikonal.c:57-72
+ *   eikonal     (fd.h:60, eikonal.c:10-76)    one shot
+ *   eikonal_adj (fd.h:62, eikonal_adj.c:10-59) one shot
+ *   ass_grad    (fd.h:36, ass_grad.c:11-49)   one shot
+ *
+ * sweep() and sweep_adj() (fd.h:145,147) have no callers outside
+ * eikonal.c and eikonal_adj.c; their work happens inside the kernels.
+ *
+ * Arrays are the reference's NR matrices: &A[1][1] is a dense row-major
+ * NY x NX block (util.c:177-204), vectors are 1-based.  A GPU failure ends
+ * the run through the reference's err() (util.c:7-17), as every other
+ * error of the program does.  The device is RAJZEL_AMD_DEVICE (default 0).
+ */
+#include <mpi.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "rajzel_amd.h"
+
+/* globals of the reference (globvar.h:12-29,38,69,96) */
+extern float DH, TIME, TTNORM, EPS_ADJ;
+extern int NX, NY, SEISMO, SNAP, MYID, INFO, NSHOT1, NSHOT2, SWS_TAPER_CIRCULAR_PER_SHOT;
+extern char SNAP_FILE[], DATA_DIR[];
+
+/* routines of the reference that stay in use (fd.h:52,119,143,166,170,180) */
+void cp_grad_frame(float **A);
+void precond(float **grad, int nsrc, float **srcpos, int **recpos, int ntr, int iter);
+void sum_grad_MPI(float **grad);
+void write_picks(float *Tmod, int ntr, int ishot);
+void writemod(char modfile[], float **array, int format);
+void err(char err_text[]);
+
+#define RZS_PATH 74                 /* STRING_SIZE of fd.h:25 */
+
+/* One context per distinct geometry (grid, constants, shot range,
+ * receivers); a new geometry replaces it. */
+static struct {
+    rz_ctx *ctx;
+    rz_model_desc d;
+    int *src_ix, *src_iy, *rec_ix, *rec_iy;
+    float *S_host;                  /* slowness last uploaded (NX*NY) */
+    int s_valid;
+} g_rzs;
+
+static char g_msg[512];
+
+static void rzs_fail(const char *what)
+{
+    snprintf(g_msg, sizeof g_msg, " rajzel_amd %s: %s", what, rz_last_error(g_rzs.ctx));
+    err(g_msg);
+}
+
+static void rzs_check(int rc, const char *what)
+{
+    if (rc != RZ_OK) rzs_fail(what);
+}
+
+static void *rzs_alloc(size_t n)
+{
+    void *p = malloc(n ? n : 1);
+    if (!p) err(" rajzel_amd: host allocation failed");
+    return p;
+}
+
+static void rzs_release(void)
+{
+    rz_destroy(g_rzs.ctx);
+    free(g_rzs.src_ix); free(g_rzs.src_iy); free(g_rzs.rec_ix); free(g_rzs.rec_iy); free(g_rzs.S_host);
+    memset(&g_rzs, 0, sizeof g_rzs);
+}
+
+/* Context for shots [s1, s2) (1-based) and the receivers recpos[1..2][1..ntr]. */
+static rz_ctx *rzs_context(float **srcpos, int s1, int s2, int **recpos, int ntr)
+{
+    const int ns = s2 > s1 ? s2 - s1 : 0;
+    int same = g_rzs.ctx && g_rzs.d.nx == NX && g_rzs.d.ny == NY && g_rzs.d.dh == DH && g_rzs.d.time == TIME &&
+               g_rzs.d.ttnorm == TTNORM && g_rzs.d.eps_adj == EPS_ADJ && g_rzs.d.nshots == ns &&
+               g_rzs.d.ntr == ntr;
+    int *sx = rzs_alloc(ns * sizeof(int)), *sy = rzs_alloc(ns * sizeof(int));
+    int *rx = rzs_alloc(ntr * sizeof(int)), *ry = rzs_alloc(ntr * sizeof(int));
+    for (int s = 0; s < ns; ++s) {
+        sx[s] = rz_grid_index(srcpos[1][s1 + s], DH);      /* eikonal.c:31-32 */
+        sy[s] = rz_grid_index(srcpos[2][s1 + s], DH);
+    }
+    for (int i = 0; i < ntr; ++i) {
+        rx[i] = recpos[1][i + 1] - 1;
+        ry[i] = recpos[2][i + 1] - 1;
+    }
+    if (same)
+        same = !memcmp(sx, g_rzs.src_ix, ns * sizeof(int)) && !memcmp(sy, g_rzs.src_iy, ns * sizeof(int)) &&
+               !memcmp(rx, g_rzs.rec_ix, ntr * sizeof(int)) && !memcmp(ry, g_rzs.rec_iy, ntr * sizeof(int));
+    if (same) {
+        free(sx); free(sy); free(rx); free(ry);
+        return g_rzs.ctx;
+    }
+    rzs_release();
+    rz_model_desc d;
+    memset(&d, 0, sizeof d);
+    d.nx = NX; d.ny = NY; d.dh = DH; d.time = TIME; d.ttnorm = TTNORM; d.eps_adj = EPS_ADJ;
+    d.max_iter = 0;
+    d.nshots = ns; d.src_ix = sx; d.src_iy = sy;
+    d.ntr = ntr; d.rec_ix = rx; d.rec_iy = ry;
+    const char *dev = getenv("RAJZEL_AMD_DEVICE");
+    rz_ctx *c = NULL;
+    if (rz_create(&c, dev ? atoi(dev) : 0, 1, &d, 0, 0) != RZ_OK) {
+        snprintf(g_msg, sizeof g_msg, " rajzel_amd context: %s", rz_last_error(NULL));
+        err(g_msg);
+    }
+    g_rzs.ctx = c;
+    g_rzs.d = d;
+    g_rzs.src_ix = sx; g_rzs.src_iy = sy; g_rzs.rec_ix = rx; g_rzs.rec_iy = ry;
+    g_rzs.S_host = rzs_alloc((size_t)NX * NY * sizeof(float));
+    g_rzs.s_valid = 0;
+    return c;
+}
+
+/* Upload S unless it equals the slowness already on the device. */
+static void rzs_slowness(float **S)
+{
+    const size_t n = (size_t)NX * NY;
+    if (g_rzs.s_valid && !memcmp(g_rzs.S_host, &S[1][1], n * sizeof(float))) return;
+    rzs_check(rz_set_slowness(g_rzs.ctx, 0, &S[1][1], 0), "slowness");
+    memcpy(g_rzs.S_host, &S[1][1], n * sizeof(float));
+    g_rzs.s_valid = 1;
+}
+
+/* Single-shot calls use a context of their own grid and receivers; its
+ * shot list is that of this rank. */
+static rz_ctx *rzs_single(float **srcpos, int **recpos, int ntr)
+{
+    if (g_rzs.ctx && g_rzs.d.nx == NX && g_rzs.d.ny == NY && g_rzs.d.ntr == ntr) {
+        int same = 1;
+        for (int i = 0; i < ntr && same; ++i)
+            same = g_rzs.rec_ix[i] == recpos[1][i + 1] - 1 && g_rzs.rec_iy[i] == recpos[2][i + 1] - 1;
+        if (same && g_rzs.d.dh == DH && g_rzs.d.time == TIME && g_rzs.d.ttnorm == TTNORM &&
+            g_rzs.d.eps_adj == EPS_ADJ)
+            return g_rzs.ctx;
+    }
+    return rzs_context(srcpos, NSHOT1, NSHOT1, recpos, ntr);
+}
+
+void eikonal(float **S, float **TT, float *Tmod, float **srcpos, int ishot, int nshots, int **recpos, int ntr)
+{
+    char filename[RZS_PATH];
+    int iters = 0;
+    if (MYID == 0 && INFO == 1)
+        printf("\n *****  Solve Eikonal equation for shot %d of %d (MPI process %d, MI355X) ***** \n", ishot,
+               nshots, MYID);
+    rz_ctx *c = rzs_single(srcpos, recpos, ntr);
+    rzs_slowness(S);
+    rzs_check(rz_eikonal_shot(c, 0, rz_grid_index(srcpos[1][ishot], DH), rz_grid_index(srcpos[2][ishot], DH),
+                              &TT[1][1], ntr > 0 ? &Tmod[1] : NULL, &iters),
+              "eikonal");
+    if (SNAP == 1) {                                        /* eikonal.c:57-60 */
+        snprintf(filename, sizeof filename, "%s_shot_%d.tt", SNAP_FILE, ishot);
+        writemod(filename, TT, 3);
+    }
+    if (SEISMO == 1) write_picks(Tmod, ntr, ishot);         /* eikonal.c:70-72 */
+}
+
+void eikonal_adj(float **TT, float *Tres, float **lam, int **recpos, int ntr, int ishot, int **recflag)
+{
+    int iters = 0;
+    (void)ishot;
+    (void)recflag;   /* init_recflag(recpos) of grad_obj.c:31; the context holds the same map */
+    if (MYID == 0 && INFO == 1) printf("\n *****  Solve adjoint Eikonal equation (MI355X) ***** \n");
+    rz_ctx *c = rzs_single(NULL, recpos, ntr);
+    rzs_check(rz_eikonal_adj_shot(c, 0, &TT[1][1], ntr > 0 ? &Tres[1] : NULL, &lam[1][1], &iters), "eikonal_adj");
+}
+
+void ass_grad(float **grad, float **grad_shot, float **S, float **lam, float **srcpos, int nshots, int **recpos,
+              int ntr, int ishot)
+{
+    (void)nshots;
+    (void)ishot;
+    if (SWS_TAPER_CIRCULAR_PER_SHOT != 0)                   /* ass_grad.c:31-37 */
+        err(" rajzel_amd: the per-shot gradient taper (SWS_TAPER_CIRCULAR_PER_SHOT) is not supported");
+    rzs_single(srcpos, recpos, ntr);
+    rzs_slowness(S);
+    rzs_check(rz_ass_grad_shot(g_rzs.ctx, 0, &lam[1][1], &grad_shot[1][1], &grad[1][1]), "ass_grad");
+}
+
+/* The last shot's fields and receiver data, which the reference leaves in
+ * the caller's arrays. */
+static void rzs_last_shot(int ns, int ntr, float **TT, float **lam, float *Tmod, float *tm_all)
+{
+    const int last = ns - 1;
+    if (TT) rzs_check(rz_get_field(g_rzs.ctx, 0, last, 0, &TT[1][1], 0), "traveltime field");
+    if (lam) rzs_check(rz_get_field(g_rzs.ctx, 0, last, 1, &lam[1][1], 0), "adjoint field");
+    if (Tmod && ntr > 0) memcpy(&Tmod[1], tm_all + (size_t)last * ntr, ntr * sizeof(float));
+}
+
+void forward(float **S, float **TT, float *Tmod, float **srcpos, int nshots, int **recpos, int ntr)
+{
+    char filename[RZS_PATH];
+    const int ns = NSHOT2 > NSHOT1 ? NSHOT2 - NSHOT1 : 0;
+    (void)nshots;
+    if (ns == 0) return;
+    rz_ctx *c = rzs_context(srcpos, NSHOT1, NSHOT2, recpos, ntr);
+    rzs_slowness(S);
+    rzs_check(rz_forward(c), "forward");
+    float *tm = rzs_alloc((size_t)ns * (ntr > 0 ? ntr : 1) * sizeof(float));
+    if (ntr > 0) rzs_check(rz_get_tmod(c, 0, tm, 0), "receiver times");
+    for (int s = 0; s < ns; ++s) {
+        const int ishot = NSHOT1 + s;
+        if (ntr > 0) memcpy(&Tmod[1], tm + (size_t)s * ntr, ntr * sizeof(float));
+        if (SNAP == 1) {
+            rzs_check(rz_get_field(c, 0, s, 0, &TT[1][1], 0), "traveltime field");
+            snprintf(filename, sizeof filename, "%s_shot_%d.tt", SNAP_FILE, ishot);
+            writemod(filename, TT, 3);
+        }
+        if (SEISMO == 1) write_picks(Tmod, ntr, ishot);
+    }
+    rzs_last_shot(ns, ntr, TT, NULL, Tmod, tm);
+    free(tm);
+}
+
+double grad_obj(float **grad, float **S, float **TT, float **lam, float *Tmod, float *Tobs, float *Tres,
+                float **srcpos, int nshots, int **recpos, int ntr, int iter)
+{
+    const int ns = NSHOT2 > NSHOT1 ? NSHOT2 - NSHOT1 : 0;
+    const size_t ncell = (size_t)NX * NY;
+    double L2 = 0.0, L2sum = 0.0;
+    SEISMO = 0;                                             /* grad_obj.c:24 */
+    if (SWS_TAPER_CIRCULAR_PER_SHOT != 0)                   /* ass_grad.c:31-37 */
+        err(" rajzel_amd: the per-shot gradient taper (SWS_TAPER_CIRCULAR_PER_SHOT) is not supported");
+    if (ns > 0) {
+        rz_ctx *c = rzs_context(srcpos, NSHOT1, NSHOT2, recpos, ntr);
+        rzs_slowness(S);
+        if (ntr > 0) {
+            /* calc_FA_res.c:28-37: every evaluation reads every shot's pick
+             * file with fscanf("%e") into Tobs, in shot order */
+            float *tobs = rzs_alloc((size_t)ns * ntr * sizeof(float));
+            char name[RZS_PATH];
+            for (int s = 0; s < ns; ++s) {
+                snprintf(name, sizeof name, "%s_%i.dat", DATA_DIR, NSHOT1 + s);
+                FILE *fp = fopen(name, "r");
+                if (fp == NULL) err(" picks_?.dat could not be opened !");
+                for (int i = 1; i <= ntr; ++i)
+                    if (fscanf(fp, "%e", &Tobs[i]) != 1) { /* value of Tobs[i] stays, as in the reference */ }
+                fclose(fp);
+                memcpy(tobs + (size_t)s * ntr, &Tobs[1], ntr * sizeof(float));
+            }
+            rzs_check(rz_set_picks(c, 0, tobs, 0), "picks");
+            free(tobs);
+            rzs_check(rz_set_tres_carry(c, 0, &Tres[1]), "residual carry");
+        }
+        rzs_check(rz_gradient(c), "gradient");
+        rzs_check(rz_get_grad(c, 0, &grad[1][1], 0), "gradient");
+        double half = 0.0;
+        rzs_check(rz_get_misfit(c, 0, &half, NULL), "misfit");
+        L2 = 2.0 * half;                                    /* this rank's sum, grad_obj.c:42 */
+        if (ntr > 0) {
+            float *tm = rzs_alloc((size_t)ns * ntr * sizeof(float));
+            rzs_check(rz_get_tmod(c, 0, tm, 0), "receiver times");
+            rzs_check(rz_get_tres_carry(c, 0, &Tres[1]), "residual carry");
+            rzs_last_shot(ns, ntr, TT, lam, Tmod, tm);
+            free(tm);
+        } else {
+            rzs_last_shot(ns, 0, TT, lam, NULL, NULL);
+        }
+    } else {
+        memset(&grad[1][1], 0, ncell * sizeof(float));     /* init_grad.c */
+    }
+    MPI_Barrier(MPI_COMM_WORLD);                            /* grad_obj.c:58-61 */
+    MPI_Allreduce(&L2, &L2sum, 1, MPI_DOUBLE, MPI_SUM, MPI_COMM_WORLD);
+    L2 = 0.5 * L2sum;
+    sum_grad_MPI(grad);                                     /* grad_obj.c:68-73 */
+    cp_grad_frame(grad);
+    precond(grad, nshots, srcpos, recpos, ntr, iter);
+    return L2;
+}

@@ -216,8 +216,9 @@ class _Scan:
             self.p += 1
         return self.d[s:self.p].decode() if self.p > s else None
 
-    def literal(self, ch: bytes):  # " =" / " ," : optional whitespace then ch
-        self._ws()
+    def literal(self, ch: bytes, skip_ws=True):  # " =" / " ," ; "," alone: no whitespace before
+        if skip_ws:
+            self._ws()
         if self.d[self.p:self.p + 1] == ch:
             self.p += 1
             return True
@@ -267,13 +268,67 @@ def read_par(path: str) -> dict:
                 continue
             names = name if isinstance(name, tuple) else (name,)
             for j, (nm, k) in enumerate(zip(names, kind.split(","))):
-                if j > 0 and not sc.literal(b","):
+                # "%f ,%f" for REFREC (read_par.c:123); "%i, %i" / "%f, %f"
+                # elsewhere, where the comma must follow the number directly
+                if j > 0 and not sc.literal(b",", skip_ws=(lineno == 16)):
                     break
                 v = sc.integer() if k == "i" else sc.real() if k == "f" else sc.token()
                 if v is None:
                     break
                 out[nm] = v
     return out
+
+
+def _par_value(v, kind):
+    if kind == "i":
+        return str(int(v))
+    if kind == "f":
+        return repr(float(np.float32(v)))
+    return str(v)
+
+
+def write_par(path: str, values: dict) -> None:
+    """Write a parameter file that read_par.c:72-268 (and read_par above)
+    reads back as ``values``: a comment line, then one ``KEY = value`` line
+    per slot in slot order.  Every slot name of PAR_SLOTS must be given;
+    lists take one value per name."""
+    lines = ["# RAJZEL parameter file"]
+    for _, name, kind in PAR_SLOTS:
+        names = name if isinstance(name, tuple) else (name,)
+        vals = [_par_value(values[nm], k) for nm, k in zip(names, kind.split(","))]
+        key = "_".join(names)
+        lines.append(f"{key} = {', '.join(vals)}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_workflow(path: str, rows) -> None:
+    """Workflow file of read_par_inv.c:24-27: a header line, then per stage
+    PRO SPATFILTER WD_DAMP WD_DAMP1 FILT_SIZE_GRAD FILT_SIZE_GRAD1."""
+    out = ["PRO\tSPATFILTER\tWD_DAMP\tWD_DAMP1\tFILT_SIZE_GRAD\tFILT_SIZE_GRAD1"]
+    for r in rows:
+        out.append(f"{r['PRO']}\t{int(r['SPATFILTER'])}\t{r['WD_DAMP']}\t{r['WD_DAMP1']}\t"
+                   f"{int(r['FILT_SIZE_GRAD'])}\t{int(r['FILT_SIZE_GRAD1'])}")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def write_sources(path: str, x, depth) -> None:
+    """Source file of sources.c:37-71: the count, then ``x z y tshift fc amp``
+    per line with the depth in the third column (the one kept as srcpos[2])."""
+    x = np.asarray(x, np.float32)
+    depth = np.asarray(depth, np.float32)
+    with open(path, "w") as f:
+        f.write(f"{len(x)}\n")
+        for a, d in zip(x, depth):
+            f.write(f"{repr(float(a))} 0.0 {repr(float(d))} 0.0 10.0 1.0\n")
+
+
+def write_receivers(path: str, x, y) -> None:
+    """Receiver file of receiver.c:34-58: ``x y`` per line, one line each."""
+    with open(path, "w") as f:
+        for a, b in zip(np.asarray(x, np.float32), np.asarray(y, np.float32)):
+            f.write(f"{repr(float(a))} {repr(float(b))}\n")
 
 
 def read_workflow(path: str):

@@ -1,0 +1,103 @@
+"""Workspaces for the reference program (oracle/_ref/librajzel_cpu.so) and
+the same program on the MI355X path (oracle/_ref/librajzel_gpu.so, the
+fd.h entry points of rajzel_amd/csrc/rz_fd_surface.c), run by
+oracle/_ref/prog_launcher as ``prog_launcher <lib> <par file> <workflow>``
+from inside the workspace.  All paths in the parameter file are relative
+and short: the reference keeps them in 74-byte buffers (fd.h:25).
+"""
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+
+from rajzel_amd import formats as F
+from rajzel_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref")
+LAUNCHER = os.path.join(REF, "prog_launcher")
+CPU_LIB = os.path.join(REF, "librajzel_cpu.so")
+GPU_LIB = os.path.join(REF, "librajzel_gpu.so")
+
+
+def available(gpu=False):
+    libs = [LAUNCHER, CPU_LIB] + ([GPU_LIB] if gpu else [])
+    return all(os.path.exists(p) for p in libs)
+
+
+def par_values(nx, ny, dh, invmat, **kw):
+    v = dict(INVMAT=invmat, NX=nx, NY=ny, DH=dh, TIME=6.0, SOURCE_FILE="src.dat", READMOD=1,
+             MFILE="model/start", SNAP=0, IDX=1, IDY=1, SNAP_FORMAT=3, SNAP_FILE="tt/snap", SEISMO=1,
+             REC_FILE="rec.dat", REFREC1=0.0, REFREC2=0.0, N_STREAMER=0, REC_INCR_X=0.0, REC_INCR_Y=0.0,
+             PICKS_FILE="picks/mod", TTNORM=1e-10, LOG_FILE="log/rz.log", LOG=0, ITERMAX=3, EPS_ADJ=1e-9,
+             JACOBIAN="jac/grad", DATA_DIR="picks/obs", TAPER=0, TAPERLENGTH=4, INVTYPE=2, GRADT1=1,
+             GRADT2=2, GRADT3=nx - 1, GRADT4=nx, SWS_TAPER_GRAD_VERT=0, SWS_TAPER_GRAD_HOR=0,
+             EXP_TAPER_GRAD_HOR=0.0, SWS_TAPER_GRAD_SOURCES=0, SWS_TAPER_CIRCULAR_PER_SHOT=0, SRTSHAPE=1,
+             SRTRADIUS=20.0, FILTSIZE=1, SWS_TAPER_FILE=0, INV_MODELFILE="model/inv", nfstart=1, nf=1,
+             nfstart_jac=1000, nf_jac=1, VPUPPERLIM=6000.0, VPLOWERLIM=300.0, GRAD_METHOD=2, NLBFGS=5,
+             MODEL_FILTER=0, FILT_SIZE=5, LINESEARCH=1, EPS_SCALE=0.01, C1=1e-4, C2=0.9, STEPMAX=6,
+             SCALEFAC=2.0, MISFIT_LOG_FILE="misfit.dat", MIN_ITER=0, GRAD_FILTER=0, FILT_SIZE_GRAD=0,
+             VP0_1=1500.0, VP0_2=1600.0, DVP0=50.0, GRAD0_1=0.0, GRAD0_2=1.0, DGRAD0=0.5,
+             GRIDSEARCH_FILE="gs.dat")
+    v.update(kw)
+    return v
+
+
+def make_workspace(sv, vp_model, invmat, picks=None, spatfilter=0, **kw):
+    """A fresh directory with parameter, workflow, acquisition and model
+    files (and pick files <DATA_DIR>_<shot>.dat when ``picks`` is given)."""
+    ws = tempfile.mkdtemp(prefix="rzp", dir="/tmp")
+    for d in ("model", "picks", "jac", "log", "tt"):
+        os.makedirs(os.path.join(ws, d))
+    F.write_par(os.path.join(ws, "rz.inp"), par_values(sv.nx, sv.ny, sv.dh, invmat, **kw))
+    F.write_workflow(os.path.join(ws, "wf.inp"),
+                     [dict(PRO=0.0, SPATFILTER=spatfilter, WD_DAMP=1e-3, WD_DAMP1=1e-3, FILT_SIZE_GRAD=8,
+                           FILT_SIZE_GRAD1=6)])
+    F.write_sources(os.path.join(ws, "src.dat"), sv.src_x_m, sv.src_z_m)
+    F.write_receivers(os.path.join(ws, "rec.dat"), sv.rec_x_m, sv.rec_z_m)
+    F.write_model(os.path.join(ws, "model", "start.vp"), vp_model)
+    if picks is not None:
+        for s in range(picks.shape[0]):
+            F.write_picks(os.path.join(ws, "picks", f"obs_{s + 1}.dat"), picks[s])
+    return ws
+
+
+def run(ws, lib, timeout=300):
+    """Run the program in ``ws``; returns the CompletedProcess."""
+    env = dict(os.environ)
+    return subprocess.run([LAUNCHER, lib, "rz.inp", "wf.inp"], cwd=ws, env=env, capture_output=True,
+                          text=True, timeout=timeout)
+
+
+def outputs(ws):
+    """Every output file of the run (path -> bytes), logs excluded: they
+    carry wall-clock times."""
+    out = {}
+    for base, _, files in os.walk(ws):
+        rel = os.path.relpath(base, ws)
+        if rel.split(os.sep)[0] == "log":
+            continue
+        for f in files:
+            p = os.path.join(base, f)
+            with open(p, "rb") as fh:
+                out[os.path.normpath(os.path.join(rel, f))] = fh.read()
+    return out
+
+
+def remove(ws):
+    if ws and ws.startswith("/tmp/rzp"):
+        shutil.rmtree(ws, ignore_errors=True)
+
+
+def survey(nx=72, ny=40, nshots=5, seed=21):
+    """Small acquisition: sources at 10 m depth, receivers on row 2."""
+    sv = W.small_case(nx=nx, ny=ny, nshots=nshots, seed=seed, rec_row=2)
+    return sv
+
+
+def start_model(sv):
+    """1-D gradient start model 1500 + 0.5 z."""
+    z = (np.arange(sv.ny, dtype=np.float32) + 1) * np.float32(sv.dh)
+    return np.repeat((1500.0 + 0.5 * z)[:, None], sv.nx, axis=1).astype(np.float32)

@@ -1,0 +1,122 @@
+"""The reference program driving the MI355X path through the fd.h entry
+points (rajzel_amd/csrc/rz_fd_surface.c): rajzel.c, fatt.c, LBFGS.c,
+PCG.c, wolfels.c, parabolicls.c and grid_search.c run unchanged, and every
+file they write must equal, byte for byte, what the unchanged reference
+program writes from the same inputs.
+
+Both programs are the reference's sources compiled by oracle/Makefile
+(oracle/_ref/librajzel_cpu.so, oracle/_ref/librajzel_gpu.so) and started
+by oracle/_ref/prog_launcher; tests/fd_program.py builds the workspaces.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from rajzel_amd import formats as F
+from tests import fd_program as P
+from tests.helpers import bits_equal
+
+SURFACE = ("eikonal", "eikonal_adj", "ass_grad", "grad_obj", "forward")
+
+
+def _need(gpu=False):
+    if not P.available(gpu):
+        pytest.skip("reference program build (oracle/_ref) not present")
+
+
+def _defined(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True, check=True)
+    return {line.split()[-1] for line in out.stdout.splitlines() if line.strip()}
+
+
+def test_gpu_program_links_the_fd_surface():
+    _need(gpu=True)
+    cpu, gpu = _defined(P.CPU_LIB), _defined(P.GPU_LIB)
+    for name in SURFACE + ("sweep", "sweep_adj", "main", "fatt", "LBFGS", "wolfels"):
+        assert name in cpu, name
+    for name in SURFACE + ("main", "fatt", "LBFGS", "PCG", "wolfels", "parabolicls", "grid_search"):
+        assert name in gpu, name
+    # the replaced sources are gone: their sweeps live in the kernels
+    assert "sweep" not in gpu and "sweep_adj" not in gpu
+
+
+def test_reference_program_forward_equals_restatement(ora):
+    _need()
+    sv = P.survey()
+    ws = P.make_workspace(sv, sv.vp_true, invmat=0)
+    try:
+        r = P.run(ws, P.CPU_LIB)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        picks = np.stack([F.read_picks(os.path.join(ws, "picks", f"mod_{s + 1}.dat"), sv.ntr)
+                          for s in range(sv.nshots)])
+        S = (1.0 / sv.vp_true.astype(np.float64)).astype(np.float32)
+        tm, _, _ = ora.forward(S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.dh)
+        assert bits_equal(picks, F.roundtrip_picks(tm))
+    finally:
+        P.remove(ws)
+
+
+def _observed(sv):
+    """Picks of the reference forward in the true model, N(0, 1 ms) noise,
+    one muted pick; as read back from pick files."""
+    from oracle.bindings import Restatement
+    S = (1.0 / sv.vp_true.astype(np.float64)).astype(np.float32)
+    tm, _, _ = Restatement().forward(S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.dh)
+    obs = F.roundtrip_picks(tm + np.random.default_rng(3).normal(0, 1e-3, tm.shape))
+    obs[1, 4] = -1.0
+    return obs
+
+
+def _compare(make):
+    """Run the unchanged and the MI355X program on identical workspaces."""
+    _need(gpu=True)
+    ws_c, ws_g = make(), make()
+    try:
+        print(f"[case] reference program in {ws_c}, MI355X program in {ws_g}", flush=True)
+        rc = P.run(ws_c, P.CPU_LIB)
+        assert rc.returncode == 0, rc.stdout[-2000:] + rc.stderr[-2000:]
+        rg = P.run(ws_g, P.GPU_LIB)
+        assert rg.returncode == 0, rg.stdout[-2000:] + rg.stderr[-2000:]
+        oc, og = P.outputs(ws_c), P.outputs(ws_g)
+        assert sorted(oc) == sorted(og)
+        differ = [k for k in oc if oc[k] != og[k]]
+        assert not differ, f"files differ: {differ}"
+        return oc
+    finally:
+        P.remove(ws_c)
+        P.remove(ws_g)
+
+
+@pytest.mark.gpu
+def test_forward_program_with_snapshots():
+    sv = P.survey()
+    out = _compare(lambda: P.make_workspace(sv, sv.vp_true, invmat=0, SNAP=1))
+    assert sum(k.startswith("picks/mod_") for k in out) == sv.nshots
+    assert sum(k.startswith("tt/snap_shot_") for k in out) == sv.nshots
+
+
+@pytest.mark.gpu
+def test_fatt_lbfgs_wolfe():
+    sv = P.survey()
+    obs = _observed(sv)
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3))
+    misfit = out["misfit.dat"].decode().split("\n")
+    assert len([m for m in misfit if m.strip()]) == 3
+
+
+@pytest.mark.gpu
+def test_fatt_pcg_parabolic_gauss_filter():
+    sv = P.survey(nx=64, ny=36, nshots=4, seed=22)
+    obs = _observed(sv)
+    _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=2, GRAD_METHOD=1,
+                                      LINESEARCH=2, spatfilter=4))
+
+
+@pytest.mark.gpu
+def test_grid_search_program():
+    sv = P.survey(nx=60, ny=30, nshots=3, seed=23)
+    obs = _observed(sv)
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=2, picks=obs))
+    assert out["gs.dat"]

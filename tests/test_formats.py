@@ -61,3 +61,26 @@ def test_model_layout_x_slow_y_fast(tmp_path):
 def test_receiver_indices_drop_nonpositive_x():
     ix, iy = F.receiver_indices(np.array([0.0, 4.0, 20.0], np.float32), np.array([10.0, 10.0, 10.0], np.float32), 10.0)
     assert list(ix) == [1] and list(iy) == [0]
+
+
+def test_parameter_file_writer_roundtrip(tmp_path):
+    vals = {}
+    for n, name, kind in F.PAR_SLOTS:
+        for nm, k in zip(name if isinstance(name, tuple) else (name,), kind.split(",")):
+            vals[nm] = n if k == "i" else np.float32(n + 0.25) if k == "f" else f"file{n}"
+    p = str(tmp_path / "w.inp")
+    F.write_par(p, vals)
+    assert F.read_par(p) == vals
+
+
+def test_parameter_lists_follow_the_c_formats(tmp_path):
+    # read_par.c:258 reads "%f, %f, %f": a blank before the comma ends the
+    # list; read_par.c:123 reads "%f ,%f", where it is allowed
+    lines = ["# h"] + [f"slot{i} = 0" for i in range(1, 16)] + ["REFREC = 1.5 , 2.5"]
+    lines += [f"slot{i} = 0" for i in range(17, 61)] + ["VP0 = 1500.0 , 1600.0 , 50.0", "GRAD0 = 0.5, 1.5, 0.25"]
+    p = tmp_path / "c.inp"
+    p.write_text("\n".join(lines) + "\n")
+    P = F.read_par(str(p))
+    assert P["REFREC1"] == np.float32(1.5) and P["REFREC2"] == np.float32(2.5)
+    assert P["VP0_1"] == np.float32(1500.0) and "VP0_2" not in P
+    assert (P["GRAD0_1"], P["GRAD0_2"], P["DGRAD0"]) == (np.float32(0.5), np.float32(1.5), np.float32(0.25))
