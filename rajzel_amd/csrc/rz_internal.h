@@ -57,7 +57,8 @@ struct ShotDev {
 
 struct Slots {
     float *TA, *TB, *LA, *LB;      // base of slot 0; slot s at + s*slot_stride
-    long long slot_stride;         // floats
+    double *CA, *CB;               // adjoint cell codes, same indexing
+    long long slot_stride;         // elements
 };
 
 // Per-shot outputs.  tmod/tnb are (nshots, ntr) blocks packed per model at

@@ -15,7 +15,7 @@
 // an LDS ring (new values) and single-lane global loads (old values).
 //
 // Arithmetic is the reference's, operation by operation (compile with
-// -ffp-contract=off): see godunov(), qdh() and adj_sweep().  Any topological order
+// -ffp-contract=off): see godunov_sel(), qdh() and adj_sweep().  Any topological order
 // of a Gauss-Seidel sweep's dependency DAG gives the same bits, so each
 // sweep here equals the reference's sweep bit for bit.
 #include "rz_internal.h"
@@ -127,36 +127,94 @@ __device__ __forceinline__ double sqrt_nr(double x)
     return __builtin_fma(d, h, g);
 }
 
-// Godunov upwind update of sweep.c:89-95.  |a-b| and S*DH are float; the
-// 2-D root is evaluated in double: 2*S^2*DH^2 is S^2 * (2 DH^2) (both
-// products of exact values, hence the same rounding), (a-b)^2 is exact in
-// double, (a+b) is a float sum, /2.0 is exact, one rounding back to float.
-// The root comes from sqrt_nr(); the float rounding of the half-sum can only
-// differ from the one with the correctly rounded root when the double lies
-// within a few ulp of a float tie (low 29 mantissa bits near 0x10000000).
-// Such cells (~1 in 1.6e7) take the correctly rounded sqrt.
-__device__ __forceinline__ float godunov(float a, float b, float s, float dh, double two_dh2)
+// Godunov upwind update of sweep.c:89-95 (godunov_sel below).  |a-b| and
+// S*DH are float; the 2-D root is evaluated in double: 2*S^2*DH^2 is
+// S^2 * (2 DH^2) (both products of exact values, hence the same rounding),
+// (a-b)^2 is exact in double, (a+b) is a float sum, /2.0 is exact, one
+// rounding back to float.  The root comes from sqrt_nr(); the float rounding
+// of the half-sum can only differ from the one with the correctly rounded
+// root when the double lies within a few ulp of a float tie (low 29
+// mantissa bits near 0x10000000).  Such cells (~1 in 1.6e7) take the
+// correctly rounded sqrt.
+
+// Minimum of two forward traveltimes.  They are never NaN and never -0
+// (every value is a sum of non-negative terms starting from +0), so
+// v_min_f32 returns exactly x < y ? x : y, the reference's fminf; the
+// inline form keeps the compiler from canonicalising the operands first.
+__device__ __forceinline__ float mn_t(float x, float y)
+{
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+}
+
+// Godunov update of sweep.c:89-95 without branches: both the 1-D and the
+// 2-D value are formed and one is selected (lanes of a wave mostly
+// disagree, so a branch would run both anyway).  The 2-D root comes from
+// sqrt_nr(); `need` flags a 2-D cell whose half-sum lies near a float
+// rounding tie, where only the correctly rounded root (EXACT) is safe.
+template <bool EXACT>
+__device__ __forceinline__ float godunov_sel(float a, float b, float s, float dh, double two_dh2, bool &need)
 {
     const float sdh = s * dh;
     const float d = a - b;
-    if (fabsf(d) >= sdh) return mn(a, b) + sdh;
+    const bool one = fabsf(d) >= sdh;
+    const float t1 = mn_t(a, b) + sdh;
     const double sd = (double)s;
     const double dd = (double)d;
-    // (a-b)^2 is exact in double, so fma(-dd, dd, .) rounds once, like the
-    // reference's subtraction of pow((a-b),2.0).
-    const double disc = __builtin_fma(-dd, dd, (sd * sd) * two_dh2);
+    double disc = __builtin_fma(-dd, dd, (sd * sd) * two_dh2);
     const double ab = (double)(a + b);
-    double half = (ab + sqrt_nr(disc)) * 0.5;
-    constexpr unsigned kGuard = 16;
-    const unsigned lo = (unsigned)__double_as_longlong(half) & 0x1FFFFFFFu;
-    if (lo - (0x10000000u - kGuard) <= 2 * kGuard) {
-        // The empty asm has a side effect, so the compiler cannot hoist the
-        // correctly rounded sqrt out of this rarely taken branch.
-        double x = disc;
-        asm volatile("" : "+v"(x));
-        half = (ab + __builtin_sqrt(x)) * 0.5;
+    double root;
+    if constexpr (EXACT) {
+        asm volatile("" : "+v"(disc));   // keeps the exact root inside the rare branch
+        root = __builtin_sqrt(disc);
+    } else {
+        root = sqrt_nr(disc);
     }
-    return (float)half;
+    const double half = (ab + root) * 0.5;
+    if constexpr (!EXACT) {
+        constexpr unsigned kGuard = 16;
+        const unsigned lo = (unsigned)__double_as_longlong(half) & 0x1FFFFFFFu;
+        need |= !one && (lo - (0x10000000u - kGuard) <= 2 * kGuard);
+    }
+    return one ? t1 : (float)half;
+}
+
+// One step of a forward sweep for the R cells of a lane (rows in sweep
+// order).  cur/nxt: old values at this and the next key (nxt[r] is the
+// downwind x-neighbour), prev: this lane's new values of the previous
+// step (upwind x-neighbour), up_new/dn_old: rows across the lane boundary.
+template <int R, int SY, bool EXACT>
+__device__ __forceinline__ void fwd_cells(const float (&cur)[R], const float (&nxt)[R], const float (&sl)[R],
+                                          const float (&prev)[R], float up_new, float dn_old, bool first,
+                                          bool last, const bool (&top)[R], const bool (&bot)[R],
+                                          const bool (&rvalid)[R], bool cvalid, float dh, double two_dh2,
+                                          float (&nv)[R], float &fsum, bool &need)
+{
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = SY > 0 ? i : R - 1 - i;
+        const float self = cur[r];
+        const float xu = first ? self : prev[r];
+        const float xd = last ? self : nxt[r];
+        float yu, yd;
+        if (SY > 0) {
+            yu = (r == 0) ? up_new : nv[r > 0 ? r - 1 : 0];
+            yd = (r == R - 1) ? dn_old : cur[r < R - 1 ? r + 1 : r];
+            if (top[r]) yu = self;
+            if (bot[r]) yd = self;
+        } else {
+            yu = (r == R - 1) ? up_new : nv[r < R - 1 ? r + 1 : r];
+            yd = (r == 0) ? dn_old : cur[r > 0 ? r - 1 : 0];
+            if (bot[r]) yu = self;
+            if (top[r]) yd = self;
+        }
+        const float a = mn_t(xu, xd);
+        const float b = mn_t(yu, yd);
+        const float t = godunov_sel<EXACT>(a, b, sl[r], dh, two_dh2, need);
+        nv[r] = mn_t(self, t);
+        if (cvalid && rvalid[r]) fsum += self - nv[r];
+    }
 }
 
 // Block-wide double sum in a fixed order (identical in every wave).
@@ -313,30 +371,18 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
                     if (l == 63) up_new = ringv;
                     if (l == 0) dn_old = qe[u];
                 }
+                // The R cells of a lane form a chain along y; all of them are
+                // first updated with the fast root.  If any cell of the wave
+                // lies near a float rounding tie (rare), the whole step is
+                // redone with the correctly rounded root.
                 float fsum = 0.f;
-#pragma unroll
-                for (int i = 0; i < R; ++i) {
-                    const int r = SY > 0 ? i : R - 1 - i;
-                    const float self = qt[u][r];
-                    const float xu = first ? self : pnew[r];
-                    const float xd = last ? self : qt[un][r];
-                    float yu, yd;
-                    if (SY > 0) {
-                        yu = (r == 0) ? up_new : nv[r > 0 ? r - 1 : 0];
-                        yd = (r == R - 1) ? dn_old : qt[u][r < R - 1 ? r + 1 : r];
-                        if (top[r]) yu = self;
-                        if (bot[r]) yd = self;
-                    } else {
-                        yu = (r == R - 1) ? up_new : nv[r < R - 1 ? r + 1 : r];
-                        yd = (r == 0) ? dn_old : qt[u][r > 0 ? r - 1 : 0];
-                        if (bot[r]) yu = self;
-                        if (top[r]) yd = self;
-                    }
-                    const float a = mn(xu, xd);
-                    const float b = mn(yu, yd);
-                    const float t = godunov(a, b, qs[u][r], dh, two_dh2);
-                    nv[r] = mn(self, t);
-                    if (cvalid && rvalid[r]) fsum += self - nv[r];
+                bool need = false;
+                fwd_cells<R, SY, false>(qt[u], qt[un], qs[u], pnew, up_new, dn_old, first, last, top, bot,
+                                        rvalid, cvalid, dh, two_dh2, nv, fsum, need);
+                if (__builtin_amdgcn_ballot_w64(need)) {
+                    fsum = 0.f;
+                    fwd_cells<R, SY, true>(qt[u], qt[un], qs[u], pnew, up_new, dn_old, first, last, top, bot,
+                                           rvalid, cvalid, dh, two_dh2, nv, fsum, need);
                 }
                 acc += (double)fsum;
 #pragma unroll
@@ -380,15 +426,67 @@ __device__ __forceinline__ float pos_part(float x) { return (x + fabsf(x)) * 0.5
 __device__ __forceinline__ float neg_part(float x) { return (x - fabsf(x)) * 0.5f; }
 
 // ---------------------------------------------------------------------------
+// Per-cell code of the adjoint update, a double in the field layouts:
+//   NaN       cell keeps its value: frame, receiver (sweep_adj.c:26-30)
+//   -1        a TT >= INF in the stencil: lambda = 0 (sweep_adj.c:68)
+//   +0        lhs <= EPS_ADJ: lamt = 0 (sweep_adj.c:60-64)
+//   1/lhs     otherwise, RN64(1/(double)lhs): lamt = rhs/lhs is then the
+//             float rounding of (double)rhs * code (the quotient of two
+//             floats is never closer than 2^-49 to a rounding tie unless
+//             it is one, which the product then also hits exactly).
+// lhs depends on T only, so the first sweep of a solve writes the code in
+// both layouts and the later sweeps read it.
+// ---------------------------------------------------------------------------
+enum AdjMode { kAdjProduce, kAdjConsume };
+
+template <int R>
+__device__ __forceinline__ void ldd(const double *p, double (&o)[R])
+{
+    typedef __attribute__((address_space(1))) const double gdouble;
+    typedef double vd2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) const vd2 gvd2;
+    if constexpr (R == 1) {
+        o[0] = *(gdouble *)p;
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; i += 2) {
+            const vd2 t = *(gvd2 *)(p + i);
+            o[i] = t.x; o[i + 1] = t.y;
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void std_(double *p, const double (&v)[R])
+{
+    typedef __attribute__((address_space(1))) double gdouble;
+    typedef double vd2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) vd2 gvd2;
+    if constexpr (R == 1) {
+        *(gdouble *)p = v[0];
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; i += 2) {
+            vd2 t;
+            t.x = v[i]; t.y = v[i + 1];
+            *(gvd2 *)(p + i) = t;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint sweep (sweep_adj.c:10-77) in direction (SX, SY).
 //   Tf    converged traveltime in this sweep's family layout; the sign bit
 //         marks receiver cells (skipped, sweep_adj.c:30)
 //   Lin   lambda in the same layout; outputs as in fwd_sweep, with the same
 //         fixed per-step set of memory operations.
+//   MODE  kAdjProduce: evaluates lhs and the skip/INF conditions from T and
+//         writes the cell code to Csame (this layout) and Cother (the other
+//         one); kAdjConsume: reads the code from Csame instead.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool WOTHER, bool WSAME, bool FD, bool CHKINF>
+template <int R, int SX, int SY, bool WOTHER, bool WSAME, bool FD, bool CHKINF, int MODE>
 __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
-                            const float *Lin, float *Lother, float *Lsame)
+                            const float *Lin, float *Lother, float *Lsame, double *Csame, double *Cother)
 {
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
@@ -406,6 +504,7 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     const gfloat *tband = gp(Tf) + boff;
     const gfloat *lband = gp(Lin) + boff;
     gfloat *gLsame = gpw(Lsame) + boff, *gLother = gpw(Lother) + boff;
+    double *cband = Csame + boff, *coband = Cother + boff;
     const int k0 = beta * BR + l * R;
 
     bool rvalid[R], frow[R];
@@ -428,6 +527,7 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     float *ring_dummy = w.ring + kMaxBands * kRingQ + l;
 
     float qt[kPF][R], ql[kPF][R], qeu[kPF], qed[kPF], qel[kPF];
+    double qc[MODE == kAdjConsume ? kPF : 1][R];
 
 #define RZ_ADJ_LOAD(SLOT, TAU)                                                              \
     do {                                                                                    \
@@ -435,6 +535,7 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
         const int key_ = clampi(SX > 0 ? tau_ : nx + 62 - tau_, 0, nkeys - 1);              \
         ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                        \
         ldv<R>(lband + key_ * BR + l * R, ql[SLOT]);                                        \
+        if constexpr (MODE == kAdjConsume) ldd<R>(cband + key_ * BR + l * R, qc[SLOT]);     \
         const int c0_ = clampi(SX > 0 ? tau_ - lag0 : nx - 1 - tau_ + lag0, 0, nx - 1);     \
         const int c63_ = clampi(SX > 0 ? tau_ - lag63 : nx - 1 - tau_ + lag63, 0, nx - 1);  \
         const long long eu_ = up_base + (long long)(FA ? c0_ + 63 : c0_) * BR;              \
@@ -467,10 +568,14 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                 ring_next = w.ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
             }
             float nv[R];
+            double cv[R];   // cell codes (kAdjProduce)
 #pragma unroll
-            for (int r = 0; r < R; ++r) nv[r] = ql[u][r];
+            for (int r = 0; r < R; ++r) {
+                nv[r] = ql[u][r];
+                cv[r] = __longlong_as_double(0x7FF8000000000000ll);
+            }
             if (active) {
-                const bool fcol = (c <= 0) || (c >= nx - 1);
+                [[maybe_unused]] const bool fcol = (c <= 0) || (c >= nx - 1);
                 // T of the rows across the lane boundary, at column c
                 float t_above, t_below;
                 if (SY > 0) {   // lane l-1 is one step ahead, lane l+1 one step behind
@@ -501,7 +606,6 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const float traw = qt[u][r];
-                    const bool rec = __float_as_uint(traw) >> 31;
                     const float tc = fabsf(traw);
                     const float tl = fabsf(SX > 0 ? tprev[r] : qt[un][r]);   // T(k, c-1)
                     const float tr = fabsf(SX > 0 ? qt[un][r] : tprev[r]);   // T(k, c+1)
@@ -522,15 +626,20 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     ybp[r] = bp;
                     // bm(k) = bp(k-1): from the row above in this lane
                     const float bm = r > 0 ? ybp[r > 0 ? r - 1 : 0] : qdh<FD>(-(tc - tu), dh, rdh);
-                    lhs[r] = qdh<FD>(pos_part(ap) - neg_part(am), dh, rdh)
-                           + qdh<FD>(pos_part(bp) - neg_part(bm), dh, rdh);
                     amp[r] = pos_part(am);
                     apm[r] = neg_part(ap);
                     bmp[r] = pos_part(bm);
                     bpm[r] = neg_part(bp);
-                    keep[r] = !cvalid || fcol || frow[r] || rec;
-                    if constexpr (CHKINF) infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
-                    else infv[r] = false;
+                    if constexpr (MODE == kAdjProduce) {
+                        const bool rec = __float_as_uint(traw) >> 31;
+                        lhs[r] = qdh<FD>(pos_part(ap) - neg_part(am), dh, rdh)
+                               + qdh<FD>(pos_part(bp) - neg_part(bm), dh, rdh);
+                        keep[r] = !cvalid || fcol || frow[r] || rec;
+                        if constexpr (CHKINF) infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
+                        else infv[r] = false;
+                        cv[r] = keep[r] ? __longlong_as_double(0x7FF8000000000000ll)
+                              : infv[r] ? -1.0 : (lhs[r] > eps ? 1.0 / (double)lhs[r] : 0.0);
+                    }
                 }
                 float fsum = 0.f;
 #pragma unroll
@@ -549,10 +658,19 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     }
                     const float rhs = qdh<FD>(amp[r] * lxm - apm[r] * lxp, dh, rdh)
                                     + qdh<FD>(bmp[r] * lym - bpm[r] * lyp, dh, rdh);
-                    const float lt = (lhs[r] > eps) ? rhs / lhs[r] : 0.f;          // sweep_adj.c:60-64
-                    float v = mn(lc, lt);                                          // sweep_adj.c:66
-                    if (infv[r]) v = 0.f;                                          // sweep_adj.c:68
-                    nv[r] = keep[r] ? lc : v;
+                    if constexpr (MODE == kAdjProduce) {
+                        const float lt = (lhs[r] > eps) ? rhs / lhs[r] : 0.f;      // sweep_adj.c:60-64
+                        float v = mn(lc, lt);                                      // sweep_adj.c:66
+                        if (infv[r]) v = 0.f;                                      // sweep_adj.c:68
+                        nv[r] = keep[r] ? lc : v;
+                    } else {
+                        const double code = qc[u][r];
+                        float lt = (float)((double)rhs * code);                    // rhs/lhs
+                        lt = code > 0.0 ? lt : 0.f;                                // lhs <= EPS_ADJ
+                        float v = mn(lc, lt);
+                        if constexpr (CHKINF) v = code < 0.0 ? 0.f : v;            // TT >= INF
+                        nv[r] = code == code ? v : lc;                             // skipped cell
+                    }
                     if (cvalid && rvalid[r]) fsum += lc - nv[r];
                 }
                 acc += (double)fsum;
@@ -570,6 +688,12 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
             if constexpr (WOTHER) {
                 const int ko = cvalid ? (FA ? (c - l + 63) : (c + l)) : nkeys;
                 stv<R>(gLother + ko * BR + l * R, nv);
+            }
+            if constexpr (MODE == kAdjProduce) {
+                const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
+                const int ko = cvalid ? (FA ? (c - l + 63) : (c + l)) : nkeys;
+                std_<R>(cband + ks * BR + l * R, cv);
+                std_<R>(coband + ko * BR + l * R, cv);
             }
             float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
@@ -685,6 +809,7 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const Mode
     const long long so = (long long)sh.slot * slots.slot_stride;
     const float *TA = slots.TA + so, *TB = slots.TB + so;
     float *LA = slots.LA + so, *LB = slots.LB + so;
+    double *CA = slots.CA + so, *CB = slots.CB + so;
     // init_travel_adj.c:22-58: 0 on the frame, 10*TIME inside, tres at receivers
     const float big = (float)(10.0 * (double)md.time);
     const int bs = (int)md.bstride;
@@ -706,10 +831,20 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const Mode
     int it = 0;
     while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
-        double acc = adj_sweep<R, +1, +1, true, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB, nullptr);
-        acc += adj_sweep<R, +1, -1, false, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, nullptr, LB);
-        acc += adj_sweep<R, -1, +1, true, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA, nullptr);
-        acc += adj_sweep<R, -1, -1, false, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, nullptr, LA);
+        // the first sweep of the solve writes the cell codes, later ones read them
+        double acc;
+        if (it == 0)
+            acc = adj_sweep<R, +1, +1, true, false, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TA, LA,
+                                                                             LB, nullptr, CA, CB);
+        else
+            acc = adj_sweep<R, +1, +1, true, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LA,
+                                                                             LB, nullptr, CA, CB);
+        acc += adj_sweep<R, +1, -1, false, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LB,
+                                                                          nullptr, LB, CB, CA);
+        acc += adj_sweep<R, -1, +1, true, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA,
+                                                                          nullptr, CB, CA);
+        acc += adj_sweep<R, -1, -1, false, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LA,
+                                                                          nullptr, LA, CA, CB);
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
         ++it;

@@ -1,6 +1,6 @@
 """The reference program driving the MI355X path through the fd.h entry
 points (rajzel_amd/csrc/rz_fd_surface.c): rajzel.c, fatt.c, LBFGS.c,
-PCG.c, wolfels.c, parabolicls.c and grid_search.c run unchanged, and every
+PCG.c, wolfels.c, parabolicls.c, gauss_filt.c and grid_search.c run unchanged, and every
 file they write must equal, byte for byte, what the unchanged reference
 program writes from the same inputs.
 
@@ -107,11 +107,21 @@ def test_fatt_lbfgs_wolfe():
 
 
 @pytest.mark.gpu
-def test_fatt_pcg_parabolic_gauss_filter():
-    sv = P.survey(nx=64, ny=36, nshots=4, seed=22)
+def test_fatt_pcg_parabolic_line_search():
+    """PCG.c directions with the parabolic line search (parabolicls.c)."""
+    sv = P.survey()
     obs = _observed(sv)
-    _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=2, GRAD_METHOD=1,
-                                      LINESEARCH=2, spatfilter=4))
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3, GRAD_METHOD=1,
+                                            LINESEARCH=2))
+    assert len([m for m in out["misfit.dat"].decode().split("\n") if m.strip()]) == 3
+
+
+@pytest.mark.gpu
+def test_fatt_wolfe_gauss_filter():
+    """The Gaussian gradient filter of precond.c / fatt.c (SPATFILTER=4)."""
+    sv = P.survey()
+    obs = _observed(sv)
+    _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3, spatfilter=4))
 
 
 @pytest.mark.gpu
