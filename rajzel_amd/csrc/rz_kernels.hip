@@ -256,22 +256,26 @@ __device__ __forceinline__ int sweep_steps(int nx, int nb)
 
 // ---------------------------------------------------------------------------
 // Forward sweep (sweep.c:10-104) in direction (SX, SY) over one shot.
-//   Tin    field in this sweep's family layout (A if SX==SY, else B)
-//   Sf     slowness in the same layout
-//   Tother written in the other family's layout   (if WOTHER)
-//   Tsame  written in the same layout, may be Tin (if WSAME)
+//   Tin    field written by the previous sweep, in the other family's layout
+//          (consecutive forward sweeps alternate between families A and B)
+//   Sf     slowness in this sweep's family layout (A if SX==SY, else B)
+//   Tout   written in this sweep's family layout
 //   INIT   read the initial field (2*TIME, 0 at the source) instead of Tin
 // Returns this lane's sum of (old - new) over its valid cells.
+//
+// Reads of Tin gather each lane's R cells from the other layout (one 16-byte
+// piece per lane, different lines); writes are whole 64*R-float blocks.  Lines
+// read piecewise are full lines to DRAM and reused from L2, while lines
+// written piecewise would reach memory as partial writes.
 //
 // Every step issues the same memory operations whatever the lane or step
 // validity (invalid lanes write to the band's dummy key block nx+63 and to a
 // dummy LDS word), so the compiler can wait on each prefetch with a partial
 // vmcnt instead of draining the queue.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool INIT, bool WOTHER, bool WSAME>
+template <int R, int SX, int SY, bool INIT>
 __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const float *Tin,
-                            const float *Sf, float *Tother, float *Tsame, int src_k, int src_c,
-                            float tinit)
+                            const float *Sf, float *Tout, int src_k, int src_c, float tinit)
 {
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
@@ -288,8 +292,10 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
     const long long boff = (long long)beta * w.bstride;
     const gfloat *tband = gp(Tin) + boff;
     const gfloat *sband = gp(Sf) + boff;
-    gfloat *gTsame = gpw(Tsame) + boff, *gTother = gpw(Tother) + boff;
+    gfloat *gTout = gpw(Tout) + boff;
     const int k0 = beta * BR + l * R;
+    // key of this lane's cells in the other family: B key = A key - 2l + 63
+    const int kshift = FA ? 63 - 2 * l : 2 * l - 63;
 
     bool rvalid[R], top[R], bot[R];
 #pragma unroll
@@ -323,8 +329,9 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
                 qt[SLOT][r_] = (k0 + r_ == src_k && c_ == src_c) ? 0.f : tinit;             \
             qe[SLOT] = (erow == src_k && ce_ == src_c) ? 0.f : tinit;                       \
         } else {                                                                            \
-            ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                    \
-            const int ek_ = SY > 0 ? (FA ? ce_ : ce_ + 63) : (FA ? ce_ + 63 : ce_);         \
+            const int kin_ = clampi(key_ + kshift, 0, nkeys);                               \
+            ldv<R>(tband + kin_ * BR + l * R, qt[SLOT]);                                    \
+            const int ek_ = SY > 0 ? (FA ? ce_ + 63 : ce_) : (FA ? ce_ : ce_ + 63);         \
             qe[SLOT] = eband[ek_ * BR + (SY > 0 ? 0 : 63 * R + R - 1)];                     \
         }                                                                                   \
     } while (0)
@@ -388,13 +395,9 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
 #pragma unroll
                 for (int r = 0; r < R; ++r) pnew[r] = nv[r];
             }
-            if constexpr (WSAME) {
+            {
                 const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
-                stv<R>(gTsame + ks * BR + l * R, nv);
-            }
-            if constexpr (WOTHER) {
-                const int ko = cvalid ? (FA ? (c - l + 63) : (c + l)) : nkeys;
-                stv<R>(gTother + ko * BR + l * R, nv);
+                stv<R>(gTout + ks * BR + l * R, nv);
             }
             float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
@@ -478,17 +481,20 @@ __device__ __forceinline__ void std_(double *p, const double (&v)[R])
 // Adjoint sweep (sweep_adj.c:10-77) in direction (SX, SY).
 //   Tf    converged traveltime in this sweep's family layout; the sign bit
 //         marks receiver cells (skipped, sweep_adj.c:30)
-//   Lin   lambda in the same layout; outputs as in fwd_sweep, with the same
-//         fixed per-step set of memory operations.
-//   MODE  kAdjProduce: evaluates lhs and the skip/INF conditions from T and
-//         writes the cell code to Csame (this layout) and Cother (the other
-//         one); kAdjConsume: reads the code from Csame instead.
+//   Lin   lambda from the previous sweep: this sweep's layout (LSAME, the
+//         sweep then updates it in place) or the other family's (gathered
+//         as in fwd_sweep)
+//   Lout  lambda written in this sweep's layout
+//   Cf    cell codes in this sweep's layout: written (kAdjProduce, lhs and
+//         the skip/INF conditions evaluated from T) or read (kAdjConsume)
+// The memory operations per step are fixed, as in fwd_sweep.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool WOTHER, bool WSAME, bool FD, bool CHKINF, int MODE>
+template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF, int MODE>
 __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
-                            const float *Lin, float *Lother, float *Lsame, double *Csame, double *Cother)
+                            const float *Lin, float *Lout, double *Cf)
 {
     constexpr bool FA = (SX == SY);
+    constexpr bool LA_ = LSAME ? FA : !FA;   // family of Lin's layout
     constexpr int BR = kWave * R;
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
     const int nkeys = nx + 63;
@@ -503,9 +509,10 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     const long long boff = (long long)beta * w.bstride;
     const gfloat *tband = gp(Tf) + boff;
     const gfloat *lband = gp(Lin) + boff;
-    gfloat *gLsame = gpw(Lsame) + boff, *gLother = gpw(Lother) + boff;
-    double *cband = Csame + boff, *coband = Cother + boff;
+    gfloat *gLout = gpw(Lout) + boff;
+    double *cband = Cf + boff;
     const int k0 = beta * BR + l * R;
+    const int kshift = LSAME ? 0 : (FA ? 63 - 2 * l : 2 * l - 63);   // key of this lane's cells in Lin
 
     bool rvalid[R], frow[R];
 #pragma unroll
@@ -534,7 +541,7 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
         const int tau_ = (TAU);                                                             \
         const int key_ = clampi(SX > 0 ? tau_ : nx + 62 - tau_, 0, nkeys - 1);              \
         ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                        \
-        ldv<R>(lband + key_ * BR + l * R, ql[SLOT]);                                        \
+        ldv<R>(lband + clampi(key_ + kshift, 0, nkeys) * BR + l * R, ql[SLOT]);             \
         if constexpr (MODE == kAdjConsume) ldd<R>(cband + key_ * BR + l * R, qc[SLOT]);     \
         const int c0_ = clampi(SX > 0 ? tau_ - lag0 : nx - 1 - tau_ + lag0, 0, nx - 1);     \
         const int c63_ = clampi(SX > 0 ? tau_ - lag63 : nx - 1 - tau_ + lag63, 0, nx - 1);  \
@@ -542,7 +549,9 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
         const long long ed_ = dn_base + (long long)(FA ? c63_ : c63_ + 63) * BR;            \
         qeu[SLOT] = tband[eu_];                                                             \
         qed[SLOT] = tband[ed_];                                                             \
-        qel[SLOT] = lband[SY > 0 ? ed_ : eu_];                                              \
+        const long long el_ = SY > 0 ? dn_base + (long long)(LA_ ? c63_ : c63_ + 63) * BR   \
+                                     : up_base + (long long)(LA_ ? c0_ + 63 : c0_) * BR;    \
+        qel[SLOT] = lband[el_];                                                             \
     } while (0)
 
 #pragma unroll
@@ -681,19 +690,10 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     tprev[r] = qt[u][r];
                 }
             }
-            if constexpr (WSAME) {
+            {
                 const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
-                stv<R>(gLsame + ks * BR + l * R, nv);
-            }
-            if constexpr (WOTHER) {
-                const int ko = cvalid ? (FA ? (c - l + 63) : (c + l)) : nkeys;
-                stv<R>(gLother + ko * BR + l * R, nv);
-            }
-            if constexpr (MODE == kAdjProduce) {
-                const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
-                const int ko = cvalid ? (FA ? (c - l + 63) : (c + l)) : nkeys;
-                std_<R>(cband + ks * BR + l * R, cv);
-                std_<R>(coband + ko * BR + l * R, cv);
+                stv<R>(gLout + ks * BR + l * R, nv);
+                if constexpr (MODE == kAdjProduce) std_<R>(cband + ks * BR + l * R, cv);
             }
             float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
@@ -704,6 +704,34 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     __syncthreads();
 #undef RZ_ADJ_LOAD
     return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Copy a field from layout B to layout A (one wave per band, 8 keys per
+// round: 8 block loads, then 8 stores).  Keys of lanes outside the grid go
+// to the dummy block.
+// ---------------------------------------------------------------------------
+template <int R>
+__device__ void copy_b_to_a(const WaveCtx &w, const float *fb, float *fa)
+{
+    constexpr int BR = kWave * R;
+    if (!w.active) return;
+    const int nx = w.nx, l = w.lane;
+    const int nkeys = nx + 63;
+    const long long boff = (long long)w.beta * w.bstride;
+    const gfloat *src = gp(fb) + boff;
+    gfloat *dst = gpw(fa) + boff;
+    for (int g = 0; g < nkeys; g += 8) {
+        float v[8][R];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ldv<R>(src + min(g + u, nkeys - 1) * BR + l * R, v[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int key = g + u, c = key + l - 63;
+            const int ka = (key < nkeys && (unsigned)c < (unsigned)nx) ? c + l : nkeys;
+            stv<R>(dst + ka * BR + l * R, v[u]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -735,25 +763,25 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_forward(const Mode
     int it = 0;
     while (lnorm >= md.ttnorm) {                                  // eikonal.c:44
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
+        // (+,+) A, (-,+) B, (-,-) A, (+,-) B (eikonal.c:50-53); each reads
+        // the previous sweep's layout and writes its own
         double acc;
         if (it == 0)
-            acc = fwd_sweep<R, +1, +1, true, true, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, nullptr,
-                                                          sh.src_k, sh.src_c, tinit);
+            acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit);
         else
-            acc = fwd_sweep<R, +1, +1, false, true, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, nullptr,
-                                                           sh.src_k, sh.src_c, tinit);
-        acc += fwd_sweep<R, -1, +1, false, true, false>(w, md.dh, md.two_dh2, TB, md.SB, TA, nullptr,
-                                                        sh.src_k, sh.src_c, tinit);
-        acc += fwd_sweep<R, -1, -1, false, true, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, nullptr,
-                                                        sh.src_k, sh.src_c, tinit);
-        acc += fwd_sweep<R, +1, -1, false, true, true>(w, md.dh, md.two_dh2, TB, md.SB, TA, TB,
-                                                       sh.src_k, sh.src_c, tinit);
+            acc = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit);
+        acc += fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TA, md.SB, TB, sh.src_k, sh.src_c, tinit);
+        acc += fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit);
+        acc += fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TA, md.SB, TB, sh.src_k, sh.src_c, tinit);
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;                               // norm1.c:27
         ++it;
     }
     const int nfield = md.nb * (int)md.bstride;
-    if (it == 0) {  // loop never entered: the field is the initial one in both layouts
+    if (it != 0) {  // the last sweep wrote layout B; the adjoint also needs A
+        copy_b_to_a<R>(w, TB, TA);
+        __syncthreads();
+    } else {        // loop never entered: the field is the initial one in both layouts
         for (int p = threadIdx.x; p < nfield; p += blockDim.x) {
             const int b = p / (int)md.bstride, q = p - b * (int)md.bstride;
             const int key = q / (kWave * R), wv = q - key * (kWave * R), l = wv / R, r = wv - l * R;
@@ -831,20 +859,19 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const Mode
     int it = 0;
     while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
-        // the first sweep of the solve writes the cell codes, later ones read them
+        // (+,+) A, (+,-) B, (-,+) B, (-,-) A (eikonal_adj.c:43-46): lambda
+        // stays in A, moves to B, stays, moves back.  The first sweep of each
+        // family writes that family's cell codes, later ones read them.
         double acc;
-        if (it == 0)
-            acc = adj_sweep<R, +1, +1, true, false, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TA, LA,
-                                                                             LB, nullptr, CA, CB);
-        else
-            acc = adj_sweep<R, +1, +1, true, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LA,
-                                                                             LB, nullptr, CA, CB);
-        acc += adj_sweep<R, +1, -1, false, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LB,
-                                                                          nullptr, LB, CB, CA);
-        acc += adj_sweep<R, -1, +1, true, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA,
-                                                                          nullptr, CB, CA);
-        acc += adj_sweep<R, -1, -1, false, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LA,
-                                                                          nullptr, LA, CA, CB);
+        if (it == 0) {
+            acc = adj_sweep<R, +1, +1, true, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, CA);
+            acc += adj_sweep<R, +1, -1, false, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB, CB);
+        } else {
+            acc = adj_sweep<R, +1, +1, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, CA);
+            acc += adj_sweep<R, +1, -1, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB, CB);
+        }
+        acc += adj_sweep<R, -1, +1, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, CB);
+        acc += adj_sweep<R, -1, -1, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LB, LA, CA);
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
         ++it;
