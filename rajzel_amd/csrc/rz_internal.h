@@ -23,10 +23,13 @@ constexpr int kPF = 8;                // prefetch depth == steps between barrier
 constexpr int kLag = kPF;             // extra lag between consecutive bands (>= kPF-1)
 constexpr int kBandOff = kWave + kLag;// step offset between consecutive bands
 constexpr int kRingQ = 32;            // LDS ring slots per band (>= kLag+1+kPF, power of 2)
-constexpr int kMaxBands = 16;         // waves per workgroup (<= 1024 threads)
-// Bands a workgroup may hold at R rows per lane: 16 for R <= 2, 8 above
-// (register budget of 128 resp. 256 VGPRs per lane).
-constexpr int max_bands_for(int R) { return R <= 2 ? 16 : 8; }
+// A workgroup runs two sweeps of one shot at a time (two wave groups, one
+// wave per band each): the second trails the first by pair_delay() epochs.
+constexpr int kGroups = 2;
+constexpr int kMaxBands = 8;          // bands (waves) per group
+// Bands a group may hold at R rows per lane: 8 for R <= 2, 4 above (with
+// two groups: 16 resp. 8 waves, a budget of 128 resp. 256 VGPRs per lane).
+constexpr int max_bands_for(int R) { return R <= 2 ? 8 : 4; }
 
 // INF of the reference (fd.h:29-31): an int compared against float fields.
 constexpr float kInf = 9999999.0f;

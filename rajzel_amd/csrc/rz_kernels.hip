@@ -243,8 +243,35 @@ __device__ __forceinline__ void epoch_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Epoch barrier number i of a sweep pair.  Every eighth one is a full
+// workgroup release/acquire: it waits for the wave's outstanding memory
+// operations, so a value one sweep of a pair stores is visible to the other
+// sweep's loads once such a barrier lies between them (pair_delay()).
+__device__ __forceinline__ void epoch_sync(int i)
+{
+    if ((i & 7) == 7) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        epoch_barrier();
+    }
+}
+
+// Epochs by which the trailing sweep of a pair follows the leading one.  Both
+// run along x in the same direction and along y in opposite directions.  The
+// trailing sweep loads cell (band p, lane l, column q) at step
+// q + (63 - lag) + (nb-1-p)*kBandOff + kPF*D - kPF and needs the leading
+// sweep's values up to column q+1, stored at q + 1 + lag + p*kBandOff; the
+// difference is at least kPF*D - kBandOff*nb.  It must exceed one full-fence
+// interval plus one epoch (8*kPF + kPF steps).
+__device__ __forceinline__ int pair_delay(int nb)
+{
+    return (kBandOff * nb + 9 * kPF + kPF - 1) / kPF;
+}
+
 struct WaveCtx {
-    int nx, ny, nb, beta, lane, active;
+    int nx, ny, nb, beta, lane, active, group;
     long long bstride;
     float *ring;
 };
@@ -252,6 +279,13 @@ struct WaveCtx {
 __device__ __forceinline__ int sweep_steps(int nx, int nb)
 {
     return ((nb - 1) * kBandOff + nx + 63 + kPF - 1) / kPF * kPF;
+}
+
+// The barriers of `nep` epochs without a sweep (a group with nothing to do).
+__device__ __forceinline__ void idle_epochs(int nep)
+{
+    for (int i = 0; i < nep; ++i) epoch_sync(i);
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -275,18 +309,19 @@ __device__ __forceinline__ int sweep_steps(int nx, int nb)
 // ---------------------------------------------------------------------------
 template <int R, int SX, int SY, bool INIT>
 __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const float *Tin,
-                            const float *Sf, float *Tout, int src_k, int src_c, float tinit)
+                            const float *Sf, float *Tout, int src_k, int src_c, float tinit, int pre, int post)
 {
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
     const int nkeys = nx + 63;          // data keys 0..nx+62; key nkeys is the dummy block
     const int gend = sweep_steps(nx, nb);
+    const int nep = gend / kPF;
     if (!w.active) {
-        for (int g0 = 0; g0 < gend; g0 += kPF) epoch_barrier();
-        __syncthreads();
+        idle_epochs(pre + nep + post);
         return 0.0;
     }
+    for (int i = 0; i < pre; ++i) epoch_sync(i);
     const int lag = SY > 0 ? l : 63 - l;
     const int off = (SY > 0 ? beta : nb - 1 - beta) * kBandOff;
     const long long boff = (long long)beta * w.bstride;
@@ -403,8 +438,9 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
             *rw = SY > 0 ? nv[R - 1] : nv[0];
             RZ_FWD_LOAD(u, tau + kPF);
         }
-        epoch_barrier();
+        epoch_sync(pre + g0 / kPF);
     }
+    for (int i = 0; i < post; ++i) epoch_sync(pre + nep + i);
     __syncthreads();
 #undef RZ_FWD_LOAD
     return acc;
@@ -491,7 +527,7 @@ __device__ __forceinline__ void std_(double *p, const double (&v)[R])
 // ---------------------------------------------------------------------------
 template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF, int MODE>
 __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
-                            const float *Lin, float *Lout, double *Cf)
+                            const float *Lin, float *Lout, double *Cf, int pre, int post)
 {
     constexpr bool FA = (SX == SY);
     constexpr bool LA_ = LSAME ? FA : !FA;   // family of Lin's layout
@@ -499,11 +535,12 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
     const int nkeys = nx + 63;
     const int gend = sweep_steps(nx, nb);
+    const int nep = gend / kPF;
     if (!w.active) {
-        for (int g0 = 0; g0 < gend; g0 += kPF) epoch_barrier();
-        __syncthreads();
+        idle_epochs(pre + nep + post);
         return 0.0;
     }
+    for (int i = 0; i < pre; ++i) epoch_sync(i);
     const int lag = SY > 0 ? l : 63 - l;
     const int off = (SY > 0 ? beta : nb - 1 - beta) * kBandOff;
     const long long boff = (long long)beta * w.bstride;
@@ -699,8 +736,9 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
             *rw = SY > 0 ? nv[R - 1] : nv[0];
             RZ_ADJ_LOAD(u, tau + kPF);
         }
-        epoch_barrier();
+        epoch_sync(pre + g0 / kPF);
     }
+    for (int i = 0; i < post; ++i) epoch_sync(pre + nep + i);
     __syncthreads();
 #undef RZ_ADJ_LOAD
     return acc;
@@ -737,49 +775,79 @@ __device__ void copy_b_to_a(const WaveCtx &w, const float *fb, float *fa)
 // ---------------------------------------------------------------------------
 // Forward kernel: one workgroup per shot (eikonal.c:10-76 + calc_FA.c).
 // ---------------------------------------------------------------------------
-template <int R>
-__global__ __launch_bounds__(max_bands_for(R) * kWave) void k_forward(const ModelDev *__restrict__ mds,
-                                                         const ShotDev *__restrict__ shots, Slots slots,
-                                                         const long long *__restrict__ rec_off,
-                                                         const int *__restrict__ shot_base, ShotOut out)
+// Wave context of a two-group workgroup: waves [0, nbg) form group 0 (the
+// leading sweep of a pair), waves [nbg, 2 nbg) group 1 (the trailing one);
+// within a group, wave b handles band b.  Each group has its own LDS ring.
+__device__ __forceinline__ WaveCtx make_wave(const ModelDev &md, float *ring_all)
 {
-    __shared__ float ring[kMaxBands * kRingQ + kWave];
-    __shared__ double red[kMaxBands];
-    const ShotDev sh = shots[blockIdx.x];
-    const ModelDev md = mds[sh.model];
-    const int nwaves = blockDim.x >> 6;
     WaveCtx w;
+    const int nbg = blockDim.x >> 7;                                     // bands per group
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     w.nx = md.nx; w.ny = md.ny; w.nb = md.nb;
-    w.beta = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform band
+    w.group = wave / nbg;
+    w.beta = wave - w.group * nbg;
     w.lane = threadIdx.x & 63;
     w.active = w.beta < md.nb;
     w.bstride = md.bstride;
-    w.ring = ring;
+    w.ring = ring_all + w.group * (kMaxBands * kRingQ + kWave);
+    return w;
+}
+
+template <int R>
+__global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
+    const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, Slots slots,
+    const long long *__restrict__ rec_off, const int *__restrict__ shot_base, ShotOut out)
+{
+    __shared__ float ring[kGroups * (kMaxBands * kRingQ + kWave)];
+    __shared__ double red[kGroups * kMaxBands];
+    const ShotDev sh = shots[blockIdx.x];
+    const ModelDev md = mds[sh.model];
+    const int nwaves = blockDim.x >> 6;
+    const WaveCtx w = make_wave(md, ring);
+    const int nep = sweep_steps(md.nx, md.nb) / kPF;
+    const int D = pair_delay(md.nb);
     const long long so = (long long)sh.slot * slots.slot_stride;
     float *TA = slots.TA + so, *TB = slots.TB + so;
     const float tinit = (float)(2.0 * (double)md.time);          // zero_travel.c:22
     float lnorm = (float)(10.0 * (double)md.time);                // eikonal.c:39
     const float denom = (float)(md.nx * md.ny);
+    // Sweep order (+,+) A, (-,+) B, (-,-) A, (+,-) B (eikonal.c:50-53); each
+    // reads the previous sweep's layout and writes its own.  Sweeps 2+3 and
+    // sweep 4 + the next iteration's sweep 1 run as pairs (same x
+    // direction); that next sweep 1 is run ahead of the stopping test and
+    // simply dropped if the loop ends.
     int it = 0;
+    double acc_next = 0.0;   // group 1: decrease of the sweep 1 run ahead
     while (lnorm >= md.ttnorm) {                                  // eikonal.c:44
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
-        // (+,+) A, (-,+) B, (-,-) A, (+,-) B (eikonal.c:50-53); each reads
-        // the previous sweep's layout and writes its own
-        double acc;
-        if (it == 0)
-            acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit);
+        double acc = 0.0;
+        if (it == 0) {
+            if (w.group == 0)
+                acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit,
+                                                 0, 0);
+            else
+                idle_epochs(nep);
+        } else {
+            acc = acc_next;
+        }
+        if (w.group == 0)
+            acc += fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TA, md.SB, TB, sh.src_k, sh.src_c, tinit, 0, D);
         else
-            acc = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit);
-        acc += fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TA, md.SB, TB, sh.src_k, sh.src_c, tinit);
-        acc += fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit);
-        acc += fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TA, md.SB, TB, sh.src_k, sh.src_c, tinit);
+            acc += fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit, D, 0);
+        if (w.group == 0) {
+            acc += fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TA, md.SB, TB, sh.src_k, sh.src_c, tinit, 0, D);
+            acc_next = 0.0;
+        } else {
+            acc_next = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit,
+                                                   D, 0);
+        }
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;                               // norm1.c:27
         ++it;
     }
     const int nfield = md.nb * (int)md.bstride;
     if (it != 0) {  // the last sweep wrote layout B; the adjoint also needs A
-        copy_b_to_a<R>(w, TB, TA);
+        if (w.group == 0) copy_b_to_a<R>(w, TB, TA);
         __syncthreads();
     } else {        // loop never entered: the field is the initial one in both layouts
         for (int p = threadIdx.x; p < nfield; p += blockDim.x) {
@@ -815,25 +883,19 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_forward(const Mode
 // init_travel_adj.c as an O(ntr) scatter).
 // ---------------------------------------------------------------------------
 template <int R, bool FD, bool CHKINF>
-__global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const ModelDev *__restrict__ mds,
-                                                         const ShotDev *__restrict__ shots, Slots slots,
-                                                         const long long *__restrict__ rec_off,
-                                                         const int *__restrict__ shot_base,
-                                                         const float *__restrict__ tres, ShotOut out)
+__global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
+    const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, Slots slots,
+    const long long *__restrict__ rec_off, const int *__restrict__ shot_base, const float *__restrict__ tres,
+    ShotOut out)
 {
     constexpr int BR = kWave * R;
-    __shared__ float ring[kMaxBands * kRingQ + kWave];
-    __shared__ double red[kMaxBands];
+    __shared__ float ring[kGroups * (kMaxBands * kRingQ + kWave)];
+    __shared__ double red[kGroups * kMaxBands];
     const ShotDev sh = shots[blockIdx.x];
     const ModelDev md = mds[sh.model];
     const int nwaves = blockDim.x >> 6;
-    WaveCtx w;
-    w.nx = md.nx; w.ny = md.ny; w.nb = md.nb;
-    w.beta = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform band
-    w.lane = threadIdx.x & 63;
-    w.active = w.beta < md.nb;
-    w.bstride = md.bstride;
-    w.ring = ring;
+    const WaveCtx w = make_wave(md, ring);
+    const int D = pair_delay(md.nb);
     const long long so = (long long)sh.slot * slots.slot_stride;
     const float *TA = slots.TA + so, *TB = slots.TB + so;
     float *LA = slots.LA + so, *LB = slots.LB + so;
@@ -860,18 +922,31 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave) void k_adjoint(const Mode
     while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
         // (+,+) A, (+,-) B, (-,+) B, (-,-) A (eikonal_adj.c:43-46): lambda
-        // stays in A, moves to B, stays, moves back.  The first sweep of each
-        // family writes that family's cell codes, later ones read them.
+        // stays in A, moves to B, stays, moves back; sweeps 1+2 and 3+4 run
+        // as pairs (same x direction).  The first sweep of each family writes
+        // that family's cell codes, later ones read them.
         double acc;
         if (it == 0) {
-            acc = adj_sweep<R, +1, +1, true, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, CA);
-            acc += adj_sweep<R, +1, -1, false, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB, CB);
+            if (w.group == 0)
+                acc = adj_sweep<R, +1, +1, true, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA,
+                                                                          CA, 0, D);
+            else
+                acc = adj_sweep<R, +1, -1, false, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB,
+                                                                           CB, D, 0);
         } else {
-            acc = adj_sweep<R, +1, +1, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, CA);
-            acc += adj_sweep<R, +1, -1, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB, CB);
+            if (w.group == 0)
+                acc = adj_sweep<R, +1, +1, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA,
+                                                                          CA, 0, D);
+            else
+                acc = adj_sweep<R, +1, -1, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB,
+                                                                           CB, D, 0);
         }
-        acc += adj_sweep<R, -1, +1, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, CB);
-        acc += adj_sweep<R, -1, -1, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LB, LA, CA);
+        if (w.group == 0)
+            acc += adj_sweep<R, -1, +1, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, CB,
+                                                                       0, D);
+        else
+            acc += adj_sweep<R, -1, -1, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LB, LA,
+                                                                        CA, D, 0);
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
         ++it;
@@ -1055,8 +1130,8 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
                           const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st)
 {
     if (nshots <= 0) return hipSuccess;
-    RZ_DISPATCH_R(R, hipLaunchKernelGGL(k_forward<RR>, dim3(nshots), dim3(nb_max * kWave), 0, st, md, shots,
-                                        slots, rec_off, shot_base, out));
+    RZ_DISPATCH_R(R, hipLaunchKernelGGL(k_forward<RR>, dim3(nshots), dim3(kGroups * nb_max * kWave), 0, st, md,
+                                        shots, slots, rec_off, shot_base, out));
     return hipGetLastError();
 }
 
@@ -1065,7 +1140,7 @@ hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *
                           int variant, hipStream_t st)
 {
     if (nshots <= 0) return hipSuccess;
-    const dim3 g(nshots), b(nb_max * kWave);
+    const dim3 g(nshots), b(kGroups * nb_max * kWave);
 #define RZ_ADJ_LAUNCH(FD_, CI_)                                                                    \
     RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, FD_, CI_>), g, b, 0, st, md, shots, slots, \
                                         rec_off, shot_base, tres, out))
