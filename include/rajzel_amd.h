@@ -167,6 +167,16 @@ int rz_ass_grad_shot(rz_ctx *ctx, int model, const float *lam, float *grad_shot,
 int rz_set_profiling(rz_ctx *ctx, int on);
 int rz_get_profile(const rz_ctx *ctx, double *ms, long long *launches);
 
+/* Record rz_forward and rz_gradient, on their first call, as HIP graphs of
+ * the whole evaluation and replay those on later calls (on = 1); on = 0
+ * drops them.  Applies while every shot fits the HBM budget in one batch,
+ * profiling is off and the work stream is not HIP's null stream; otherwise
+ * the calls launch as usual.  Results are identical either way.  No
+ * counterpart in the reference (its evaluation is a host loop,
+ * grad_obj.c:33-52); it serves callers that repeat evaluations of small
+ * problems, where launch overhead matters. */
+int rz_set_graph(rz_ctx *ctx, int on);
+
 /* Block until the context's stream is idle. */
 int rz_sync(rz_ctx *ctx);
 

@@ -30,7 +30,6 @@ struct ModelHost {
     int carry_off = 0;       // offset in the carry buffer
     bool have_S = false, have_picks = false, have_res = false;
     bool fastdiv = false;
-    std::vector<float> tres0;
     // device
     float *S = nullptr, *SA = nullptr, *SB = nullptr;
     int *rec_k_d = nullptr, *rec_c_d = nullptr, *uq_k_d = nullptr, *uq_c_d = nullptr, *uq_r_d = nullptr;
@@ -56,6 +55,10 @@ struct rz_ctx {
     int *shot_base_d = nullptr, *shot_base0_d = nullptr;
     float *tobs_d = nullptr, *tmod_d = nullptr, *tnb_d = nullptr, *tres_d = nullptr, *carry_d = nullptr;
     float *dres_d = nullptr;            // residuals given by rz_set_residuals
+    float *carry0_d = nullptr;          // initial residual carry (rz_set_tres_carry)
+    // whole evaluations recorded as HIP graphs (rz_set_graph): [0] forward, [1] gradient
+    bool graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
     long long s_epoch = 0;              // bumped by every rz_set_slowness
     long long fields_epoch = -1;        // slowness epoch of resident forward fields of all shots
     float *grad_d = nullptr;
@@ -267,6 +270,73 @@ int collect(rz_ctx *c, const std::vector<PhaseMarks> &marks, bool grad)
     return RZ_OK;
 }
 
+// Enqueue the launches of one batch of shots [b0, b1) on the work stream:
+// forward (unless its fields are reused), and with grad the residuals, the
+// adjoint and the gradient.  pm (optional) receives the phase events.
+int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1, PhaseMarks *pm)
+{
+    const Slots sl = slots_of(c);
+    const ShotOut so = outs_of(c);
+    PhaseMarks dummy;
+    PhaseMarks &m = pm ? *pm : dummy;
+    TRY(mark(c, &m.e[0]));
+    if (!reuse)
+        HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+                                 c->shot_base_d, so, c->stream));
+    TRY(mark(c, &m.e[1]));
+    if (!grad) return RZ_OK;
+    for (size_t mi = 0; mi < c->m.size(); ++mi) {
+        const ModelHost &h = c->m[mi];
+        const int s0 = std::max(b0, h.shot_base) - h.shot_base;
+        const int s1 = std::min(b1, h.shot_base + h.d.nshots) - h.shot_base;
+        if (s1 <= s0 || h.d.ntr <= 0) continue;
+        HIPCHK(c, launch_residual(c->md_d, (int)mi, h.d.ntr, s0, s1, h.rec_off, c->tmod_d, c->tnb_d, c->tobs_d,
+                                  direct ? c->dres_d : nullptr, c->carry_d + h.carry_off, c->tres_d,
+                                  c->l2_shot_d + h.shot_base, c->l2_sum_d + mi, c->stream));
+    }
+    TRY(mark(c, &m.e[2]));
+    HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d, c->shot_base_d,
+                             c->tres_d, so, adj_variant(c), c->stream));
+    TRY(mark(c, &m.e[3]));
+    for (size_t mi = 0; mi < c->m.size(); ++mi) {
+        const ModelHost &h = c->m[mi];
+        const int s0 = std::max(b0, h.shot_base) - h.shot_base;
+        const int s1 = std::min(b1, h.shot_base + h.d.nshots) - h.shot_base;
+        if (s1 <= s0) continue;
+        ModelDev mh = make_dev(c, h);
+        HIPCHK(c, launch_gradient(&mh, (int)mi, c->md_d, c->slot_of_d + h.shot_base + s0, s1 - s0, sl,
+                                  c->grad_d + h.grad_off, s0 > 0 ? 1 : 0, c->stream));
+    }
+    TRY(mark(c, &m.e[4]));
+    return RZ_OK;
+}
+
+// Record a whole single-batch evaluation (forward only, or the gradient) as
+// a HIP graph on the work stream.  It holds kernels only: fixed pointers
+// into the context's buffers, no memset or host copy nodes.
+int record_graph(rz_ctx *c, bool grad, hipGraphExec_t *exec)
+{
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = RZ_OK;
+    if (grad) {
+        const hipError_t e = launch_eval_init(c->carry_d, c->carry0_d, c->total_tr, c->l2_sum_d, (int)c->m.size(),
+                                              c->grad_d, c->total_cells, c->stream);
+        if (e != hipSuccess) rc = fail(c, RZ_ERR_HIP, "evaluation init: %s", hipGetErrorString(e));
+    }
+    if (rc == RZ_OK) rc = enqueue_batch(c, grad, false, false, 0, c->total_shots, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != RZ_OK) {
+        if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIPCHK(c, e);
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(c, ei);
+    return RZ_OK;
+}
+
 // Run forward (and, with grad, residual + adjoint + gradient) over the
 // global shot order in batches of at most `cap` shots.
 int run(rz_ctx *c, bool grad, int cap, bool direct = false)
@@ -281,69 +351,47 @@ int run(rz_ctx *c, bool grad, int cap, bool direct = false)
     // adjoint-only run does not recompute them
     const bool reuse = direct && c->fields_epoch == c->s_epoch && c->nslots >= c->total_shots;
     HIPCHK(c, hipSetDevice(c->device));
-    const Slots sl = slots_of(c);
-    const ShotOut so = outs_of(c);
-    if (grad) {
-        std::vector<float> carry0((size_t)std::max(c->total_tr, 1), 0.f);
-        for (auto &h : c->m)
-            if (!h.tres0.empty()) std::copy(h.tres0.begin(), h.tres0.end(), carry0.begin() + h.carry_off);
-        if (c->total_tr > 0) {
-            HIPCHK(c, hipMemcpyAsync(c->carry_d, carry0.data(), c->total_tr * sizeof(float), hipMemcpyHostToDevice,
-                                     c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));   // carry0 is a local
-        }
-        HIPCHK(c, hipMemsetAsync(c->l2_sum_d, 0, c->m.size() * sizeof(double), c->stream));
-        // models without shots still report a zero gradient
-        HIPCHK(c, hipMemsetAsync(c->grad_d, 0, (size_t)c->total_cells * sizeof(float), c->stream));
-    }
     // Shot g lives in slot g % nslots; any batch of at most nslots
     // consecutive shots therefore uses distinct slots.
     const int per = std::max(1, std::min(cap, c->nslots));
-    std::vector<PhaseMarks> marks;
-    for (int b0 = 0; b0 < c->total_shots; b0 += per) {
-        const int b1 = std::min(c->total_shots, b0 + per);
-        PhaseMarks pm;
-        TRY(mark(c, &pm.e[0]));
-        if (!reuse)
-            HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
-                                     c->shot_base_d, so, c->stream));
-        TRY(mark(c, &pm.e[1]));
-        if (grad) {
-            for (size_t mi = 0; mi < c->m.size(); ++mi) {
-                const ModelHost &h = c->m[mi];
-                const int s0 = std::max(b0, h.shot_base) - h.shot_base;
-                const int s1 = std::min(b1, h.shot_base + h.d.nshots) - h.shot_base;
-                if (s1 <= s0 || h.d.ntr <= 0) continue;
-                HIPCHK(c, launch_residual(c->md_d, (int)mi, h.d.ntr, s0, s1, h.rec_off, c->tmod_d, c->tnb_d,
-                                          c->tobs_d, direct ? c->dres_d : nullptr, c->carry_d + h.carry_off,
-                                          c->tres_d, c->l2_shot_d + h.shot_base, c->l2_sum_d + mi, c->stream));
-            }
-            TRY(mark(c, &pm.e[2]));
-            HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
-                                     c->shot_base_d, c->tres_d, so, adj_variant(c), c->stream));
-            TRY(mark(c, &pm.e[3]));
-            for (size_t mi = 0; mi < c->m.size(); ++mi) {
-                const ModelHost &h = c->m[mi];
-                const int s0 = std::max(b0, h.shot_base) - h.shot_base;
-                const int s1 = std::min(b1, h.shot_base + h.d.nshots) - h.shot_base;
-                if (s1 <= s0) continue;
-                ModelDev mh = make_dev(c, h);
-                HIPCHK(c, launch_gradient(&mh, (int)mi, c->md_d, c->slot_of_d + h.shot_base + s0, s1 - s0, sl,
-                                          c->grad_d + h.grad_off, s0 > 0 ? 1 : 0, c->stream));
-            }
-            TRY(mark(c, &pm.e[4]));
+    const bool use_graph = c->graph && !direct && !c->prof && c->stream != nullptr && c->total_shots > 0 &&
+                           per >= c->total_shots;
+    if (use_graph) {
+        hipGraphExec_t &ge = c->gexec[grad ? 1 : 0];
+        if (!ge) TRY(record_graph(c, grad, &ge));
+        HIPCHK(c, hipGraphLaunch(ge, c->stream));
+        c->last_b0 = 0;
+        c->last_b1 = c->total_shots;
+    } else {
+        // models without shots still report a zero gradient
+        if (grad)
+            HIPCHK(c, launch_eval_init(c->carry_d, c->carry0_d, c->total_tr, c->l2_sum_d, (int)c->m.size(),
+                                       c->grad_d, c->total_cells, c->stream));
+        std::vector<PhaseMarks> marks;
+        for (int b0 = 0; b0 < c->total_shots; b0 += per) {
+            const int b1 = std::min(c->total_shots, b0 + per);
+            PhaseMarks pm;
+            TRY(enqueue_batch(c, grad, direct, reuse, b0, b1, &pm));
+            if (c->prof) marks.push_back(pm);
+            c->last_b0 = b0;
+            c->last_b1 = b1;
         }
-        if (c->prof) marks.push_back(pm);
-        c->last_b0 = b0;
-        c->last_b1 = b1;
+        TRY(collect(c, marks, grad));
     }
-    TRY(collect(c, marks, grad));
     c->have_fwd = true;
     c->have_grad = grad;
     // all shots' forward fields stay in their slots only without batching
     if (c->nslots >= c->total_shots && !(grad && cap < c->nslots)) c->fields_epoch = c->s_epoch;
     else if (!reuse) c->fields_epoch = -1;
     return check_convergence(c, grad);
+}
+
+void drop_graphs(rz_ctx *c)
+{
+    for (hipGraphExec_t &g : c->gexec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 int copy_out(rz_ctx *c, void *dst, const void *src, size_t bytes, int on_device)
@@ -374,6 +422,7 @@ void rz_destroy(rz_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);   // also fine for the null stream
     if (c->own && c->own != c->stream) (void)hipStreamSynchronize(c->own);
+    drop_graphs(c);
     for (void *p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
@@ -501,6 +550,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         (rc = dalloc(c, &c->shot_base0_d, nm)) || (rc = dalloc(c, &c->tobs_d, rec_off)) ||
         (rc = dalloc(c, &c->tmod_d, rec_off)) || (rc = dalloc(c, &c->tnb_d, rec_off)) ||
         (rc = dalloc(c, &c->tres_d, rec_off)) || (rc = dalloc(c, &c->carry_d, carry)) ||
+        (rc = dalloc(c, &c->carry0_d, carry)) ||
         (rc = dalloc(c, &c->dres_d, rec_off)) ||
         (rc = dalloc(c, &c->grad_d, cells)) || (rc = dalloc(c, &c->l2_shot_d, shots)) ||
         (rc = dalloc(c, &c->l2_sum_d, nm)) || (rc = dalloc(c, &c->it_f_d, shots)) ||
@@ -563,9 +613,22 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         hipMemcpy(c->shot_base_d, sb.data(), nm * sizeof(int), hipMemcpyHostToDevice) ||
         hipMemcpy(c->shot_base0_d, sb0.data(), nm * sizeof(int), hipMemcpyHostToDevice) ||
         hipMemcpy(c->s_slot0, &zero, sizeof(int), hipMemcpyHostToDevice) ||
-        hipMemcpy(c->s_shot, &s1, sizeof(ShotDev), hipMemcpyHostToDevice))
+        hipMemcpy(c->s_shot, &s1, sizeof(ShotDev), hipMemcpyHostToDevice) ||
+        hipMemset(c->carry0_d, 0, (size_t)std::max(c->total_tr, 1) * sizeof(float)) || hipDeviceSynchronize())
         return bail(fail(c, RZ_ERR_HIP, "table upload failed"));
     *out = c;
+    return RZ_OK;
+}
+
+int rz_set_graph(rz_ctx *c, int on)
+{
+    if (!c) return RZ_ERR_INVALID;
+    c->graph = on != 0;
+    if (!c->graph) {
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        drop_graphs(c);
+    }
     return RZ_OK;
 }
 
@@ -619,8 +682,12 @@ int rz_set_tres_carry(rz_ctx *c, int model, const float *tres0)
 {
     TRY(check_model(c, model));
     ModelHost &h = c->m[model];
-    if (!tres0) { h.tres0.clear(); return RZ_OK; }
-    h.tres0.assign(tres0, tres0 + h.d.ntr);
+    if (h.d.ntr == 0) return RZ_OK;
+    std::vector<float> v(tres0 ? tres0 : nullptr, tres0 ? tres0 + h.d.ntr : nullptr);
+    v.resize(h.d.ntr, 0.f);
+    HIPCHK(c, hipMemcpyAsync(c->carry0_d + h.carry_off, v.data(), h.d.ntr * sizeof(float), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return RZ_OK;
 }
 

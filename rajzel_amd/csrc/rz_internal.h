@@ -83,6 +83,8 @@ hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *
 hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s1, long long rec_off,
                            const float *tmod, const float *tnb, const float *tobs, const float *direct,
                            float *carry, float *tres, double *l2_shot, double *l2_sum, hipStream_t st);
+hipError_t launch_eval_init(float *carry, const float *carry0, int ntr_total, double *l2_sum, int nmodels,
+                            float *grad, long long ncells, hipStream_t st);
 hipError_t launch_gradient(const ModelDev *md_host, int model, const ModelDev *md_dev, const int *slot_of,
                            int nsh, Slots slots, float *grad, int accumulate, hipStream_t st);
 hipError_t launch_field_to_rowmajor(const ModelDev *md_host, const float *field, float *out, int strip,

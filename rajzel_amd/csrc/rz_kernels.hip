@@ -956,57 +956,107 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
 
 // ---------------------------------------------------------------------------
 // Residuals, adjoint sources and misfit (calc_FA_res.c:10-77) for shots
-// [s0, s1) of one model, one workgroup.  carry (ntr) is the reference's
-// Tres array: a muted pick keeps the previous value.  l2_sum accumulates
-// the per-shot sums in shot order (grad_obj.c:42).
+// [s0, s1) of one model.  carry (ntr) is the reference's Tres array: a muted
+// pick keeps the previous value, so each receiver is a chain over the shots
+// (one thread per receiver, loads issued ahead of the chain).
+// direct != nullptr: the residual Tmod - Tobs is given (an adjoint source
+// dL/dTmod); no pick is muted and the slope scaling is applied as usual.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float residual_of(const ModelDev &md, int k, float tm, float tn, float r)
+{
+    // calc_FA_res.c:52-69: divide by the vertical slope at the receiver
+    float g = 0.f;
+    bool have = true;
+    if (k <= md.ny - 2) g = (tn - tm) / md.dh;
+    else if (k == md.ny - 1) g = (tm - tn) / md.dh;
+    else have = false;
+    return have ? ((fabsf(g) > 0.f) ? r / g : 0.f) : r;
+}
+
 __global__ void k_residual(const ModelDev *__restrict__ mds, int model, int ntr, int s0, int s1,
                            long long rec_off, const float *__restrict__ tmod, const float *__restrict__ tnb,
                            const float *__restrict__ tobs, const float *__restrict__ direct, float *carry,
-                           float *tres, double *l2_shot, double *l2_sum)
+                           float *tres)
 {
-    // direct != nullptr: the residual Tmod - Tobs is given (an adjoint source
-    // dL/dTmod); no pick is muted and the slope scaling is applied as usual.
     const ModelDev md = mds[model];
-    for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
-        float cr = carry[i];
-        const int k = md.rec_k[i];
-        for (int s = s0; s < s1; ++s) {
-            const long long x = rec_off + (long long)s * ntr + i;
-            const float to = direct ? 0.f : tobs[x];
-            if (to >= 0.f) {
-                const float tm = tmod[x];
-                float r = direct ? direct[x] : tm - to;
-                float g = 0.f;
-                bool have = true;
-                if (k <= md.ny - 2) g = (tnb[x] - tm) / md.dh;
-                else if (k == md.ny - 1) g = (tm - tnb[x]) / md.dh;
-                else have = false;
-                if (have) r = (fabsf(g) > 0.f) ? r / g : 0.f;
-                cr = r;
-            }
-            tres[x] = cr;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntr) return;
+    const int k = md.rec_k[i];
+    float cr = carry[i];
+    constexpr int U = 8;
+    for (int s = s0; s < s1; s += U) {
+        float to[U], tm[U], tn[U], dd[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long x = rec_off + (long long)min(s + u, s1 - 1) * ntr + i;
+            to[u] = direct ? 0.f : tobs[x];
+            tm[u] = tmod[x];
+            tn[u] = tnb[x];
+            dd[u] = direct ? direct[x] : 0.f;
         }
-        carry[i] = cr;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (s + u >= s1) break;
+            if (to[u] >= 0.f) {                                   // calc_FA_res.c:39
+                const float r = direct ? dd[u] : tm[u] - to[u];   // calc_FA_res.c:42
+                cr = residual_of(md, k, tm[u], tn[u], r);
+            }
+            tres[rec_off + (long long)(s + u) * ntr + i] = cr;
+        }
     }
-    __syncthreads();
-    for (int s = s0 + (int)threadIdx.x; s < s1; s += blockDim.x) {
-        double l2 = 0.0;
-        for (int i = 0; i < ntr; ++i) {
+    carry[i] = cr;
+}
+
+// Misfit of shot s0 + blockIdx.x: sum over receivers, in receiver order, of
+// (double)(r*r) for unmuted picks (calc_FA_res.c:47).  One wave: the lanes
+// form 64 products at a time, lane 0 adds them in order.
+__global__ void k_misfit_shot(int ntr, int s0, long long rec_off, const float *__restrict__ tmod,
+                              const float *__restrict__ tobs, const float *__restrict__ direct, double *l2_shot)
+{
+    const int s = s0 + blockIdx.x, l = threadIdx.x;
+    double l2 = 0.0;
+    for (int i0 = 0; i0 < ntr; i0 += kWave) {
+        const int i = i0 + l;
+        float rr = 0.f;
+        bool use = false;
+        if (i < ntr) {
             const long long x = rec_off + (long long)s * ntr + i;
             const float to = direct ? 0.f : tobs[x];
             if (to >= 0.f) {
                 const float r = direct ? direct[x] : tmod[x] - to;
-                l2 += (double)(r * r);
+                rr = r * r;
+                use = true;
             }
         }
-        l2_shot[s] = l2;
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(use);
+        const int n = min(kWave, ntr - i0);
+        for (int j = 0; j < n; ++j) {
+            const float v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rr), j));
+            if ((m >> j) & 1ull) l2 += (double)v;
+        }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = *l2_sum;
-        for (int s = s0; s < s1; ++s) t += l2_shot[s];
-        *l2_sum = t;
+    if (l == 0) l2_shot[s] = l2;
+}
+
+// grad_obj.c:42: the evaluation's sum of per-shot misfits, in shot order.
+__global__ void k_misfit_sum(int s0, int s1, const double *__restrict__ l2_shot, double *l2_sum)
+{
+    double t = *l2_sum;
+    for (int s = s0; s < s1; ++s) t += l2_shot[s];
+    *l2_sum = t;
+}
+
+// Start of an evaluation: the residual carry from its initial value, the
+// misfit sums and the gradient to zero (kernels rather than memsets, so
+// that an evaluation can be recorded in a HIP graph).
+__global__ void k_eval_init(float *carry, const float *carry0, int ntr_total, double *l2_sum, int nmodels,
+                            float *grad, long long ncells, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (i < ncells) grad[i] = 0.f;
+        if (i < ntr_total) carry[i] = carry0[i];
+        if (i < nmodels) l2_sum[i] = 0.0;
     }
 }
 
@@ -1158,8 +1208,23 @@ hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s
                            const float *tmod, const float *tnb, const float *tobs, const float *direct,
                            float *carry, float *tres, double *l2_shot, double *l2_sum, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_residual, dim3(1), dim3(256), 0, st, md, model, ntr, s0, s1, rec_off, tmod, tnb, tobs,
-                       direct, carry, tres, l2_shot, l2_sum);
+    if (s1 <= s0) return hipSuccess;
+    if (ntr > 0) {
+        hipLaunchKernelGGL(k_residual, dim3((ntr + 63) / 64), dim3(64), 0, st, md, model, ntr, s0, s1, rec_off, tmod,
+                           tnb, tobs, direct, carry, tres);
+        hipLaunchKernelGGL(k_misfit_shot, dim3(s1 - s0), dim3(kWave), 0, st, ntr, s0, rec_off, tmod, tobs, direct,
+                           l2_shot);
+    }
+    hipLaunchKernelGGL(k_misfit_sum, dim3(1), dim3(1), 0, st, s0, s1, l2_shot, l2_sum);
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_init(float *carry, const float *carry0, int ntr_total, double *l2_sum, int nmodels,
+                            float *grad, long long ncells, hipStream_t st)
+{
+    const long long n = std::max<long long>(std::max<long long>(ncells, ntr_total), nmodels);
+    hipLaunchKernelGGL(k_eval_init, dim3(grid_for(n, 256)), dim3(256), 0, st, carry, carry0, ntr_total, l2_sum,
+                       nmodels, grad, ncells, n);
     return hipGetLastError();
 }
 

@@ -23,7 +23,7 @@ EXPORTS = (
     "rz_get_tmod", "rz_get_tres", "rz_get_grad", "rz_get_misfit", "rz_get_iterations",
     "rz_get_tres_carry", "rz_get_field", "rz_eikonal_shot", "rz_eikonal_adj_shot", "rz_ass_grad_shot",
     "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
-    "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream",
+    "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
 )
 
 
@@ -96,6 +96,7 @@ def lib():
                 "rz_set_residuals": (i, [vp, i, vp, i]),
                 "rz_adjoint": (i, [vp]),
                 "rz_get_stream": (vp, [vp]),
+                "rz_set_graph": (i, [vp, i]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -300,6 +301,10 @@ class Context:
         gs = np.empty_like(lam)
         self._chk(lib().rz_ass_grad_shot(self.h, model, _ptr(lam), _ptr(gs), _ptr(g)))
         return gs, g
+
+    def set_graph(self, on=True):
+        """Replay whole evaluations as HIP graphs (rz_set_graph)."""
+        self._chk(lib().rz_set_graph(self.h, 1 if on else 0))
 
     def set_profiling(self, on=True):
         self._chk(lib().rz_set_profiling(self.h, 1 if on else 0))
