@@ -73,13 +73,24 @@ int rz_grid_index(float pos_m, float dh);
 
 /* Create a context on HIP device `device` holding `nmodels` models.
  * On failure *out is NULL and rz_last_error(NULL) gives the reason.
- *   rows_per_lane  1, 2, 4 or 8 rows of a band per lane; 0 = chosen
+ *   rows_per_lane  1, 2, 4 or 8 rows of a band per lane; 0 = chosen by
+ *                  the band count; RZ_ROWS_AUTOTUNE = measured by
+ *                  rz_autotune_rows()
  *   hbm_budget     bytes of device memory for per-shot fields; 0 = enough
  *                  for every shot at once.  Below that, shots run in
  *                  batches that fit and forward fields are recomputed per
  *                  batch (results are unchanged). */
+#define RZ_ROWS_AUTOTUNE (-1)
 int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models, int rows_per_lane,
               size_t hbm_budget);
+
+/* Autotuner of the rows per lane (the tile height of a wave: 64*R rows per
+ * band): times one outer forward + adjoint iteration of these models at
+ * every feasible R on the device and returns the fastest (and its time in
+ * ms, if ms is not NULL).  Creates and destroys temporary contexts.  No
+ * counterpart in the reference. */
+int rz_autotune_rows(int device, int nmodels, const rz_model_desc *models, size_t hbm_budget, int *rows_per_lane,
+                     double *ms);
 void rz_destroy(rz_ctx *ctx);
 const char *rz_last_error(const rz_ctx *ctx);
 

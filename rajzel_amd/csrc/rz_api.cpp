@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <string>
 #include <unordered_map>
+#include <chrono>
 #include <vector>
 
 #include "../../include/rajzel_amd.h"
@@ -455,6 +456,12 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         if ((long long)d.nx * d.ny > (1LL << 28))
             return bail(fail(c, RZ_ERR_INVALID, "model %d: grid too large", i));
         ny_max = std::max(ny_max, d.ny);
+    }
+    if (rows_per_lane == RZ_ROWS_AUTOTUNE) {
+        int best = 0;
+        if (rz_autotune_rows(device, nmodels, models, hbm_budget, &best, nullptr) != RZ_OK)
+            return bail(fail(c, RZ_ERR_INVALID, "autotune: %s", g_create_err.c_str()));
+        rows_per_lane = best;
     }
     c->R = rows_per_lane ? rows_per_lane : choose_R(ny_max);
     if (c->R != 1 && c->R != 2 && c->R != 4 && c->R != 8)
@@ -928,6 +935,51 @@ int rz_get_profile(const rz_ctx *c, double *ms, long long *launches)
 }
 
 int rz_rows_per_lane(const rz_ctx *c) { return c ? c->R : 0; }
+
+int rz_autotune_rows(int device, int nmodels, const rz_model_desc *models, size_t hbm_budget, int *rows_per_lane,
+                     double *ms)
+{
+    if (!rows_per_lane || nmodels <= 0 || !models) return RZ_ERR_INVALID;
+    // One outer iteration of forward and adjoint per candidate, on a smooth
+    // synthetic model (1500 + 0.5 z m/s) with picks of 1 s: the cost per
+    // iteration is what the rows per lane change.
+    std::vector<rz_model_desc> d(models, models + nmodels);
+    for (auto &x : d) x.max_iter = 1;
+    double best_ms = 0.0;
+    int best = 0;
+    for (int R : {1, 2, 4, 8}) {
+        rz_ctx *t = nullptr;
+        if (rz_create(&t, device, nmodels, d.data(), R, hbm_budget) != RZ_OK) continue;
+        bool ok = true;
+        for (int m = 0; m < nmodels && ok; ++m) {
+            std::vector<float> S((size_t)d[m].nx * d[m].ny);
+            for (int k = 0; k < d[m].ny; ++k)
+                for (int i = 0; i < d[m].nx; ++i)
+                    S[(size_t)k * d[m].nx + i] = (float)(1.0 / (1500.0 + 0.5 * (k + 1) * (double)d[m].dh));
+            std::vector<float> picks((size_t)std::max(1, d[m].nshots * d[m].ntr), 1.f);
+            ok = rz_set_slowness(t, m, S.data(), 0) == RZ_OK && rz_set_picks(t, m, picks.data(), 0) == RZ_OK;
+        }
+        double el = 0.0;
+        for (int rep = 0; rep < 2 && ok; ++rep) {   // the first run warms up
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = rz_gradient(t);
+            ok = rc == RZ_OK || rc == RZ_ERR_NOT_CONVERGED;
+            el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        rz_destroy(t);
+        if (ok && (best == 0 || el < best_ms)) {
+            best = R;
+            best_ms = el;
+        }
+    }
+    if (best == 0) {
+        g_create_err = "no rows-per-lane setting can hold these models";
+        return RZ_ERR_INVALID;
+    }
+    *rows_per_lane = best;
+    if (ms) *ms = best_ms;
+    return RZ_OK;
+}
 size_t rz_device_bytes(const rz_ctx *c) { return c ? c->bytes : 0; }
 int rz_batch_shots(const rz_ctx *c) { return c ? c->nslots : 0; }
 

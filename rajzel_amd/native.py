@@ -24,6 +24,7 @@ EXPORTS = (
     "rz_get_tres_carry", "rz_get_field", "rz_eikonal_shot", "rz_eikonal_adj_shot", "rz_ass_grad_shot",
     "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
+    "rz_autotune_rows",
 )
 
 
@@ -97,6 +98,7 @@ def lib():
                 "rz_adjoint": (i, [vp]),
                 "rz_get_stream": (vp, [vp]),
                 "rz_set_graph": (i, [vp, i]),
+                "rz_autotune_rows": (i, [i, i, C.POINTER(ModelDesc), C.c_size_t, C.POINTER(i), C.POINTER(d)]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -114,6 +116,47 @@ def grid_index(pos_m, dh):
     return lib().rz_grid_index(float(pos_m), float(dh))
 
 
+ROWS_AUTOTUNE = -1
+
+
+def _descs(models):
+    """rz_model_desc array for `models`, the arrays it points into (keep
+    them alive for the call) and (ny, nx, nshots, ntr) per model."""
+    keep, shapes = [], []
+    descs = (ModelDesc * len(models))()
+    for k, m in enumerate(models):
+        g = (lambda n, dflt=None: getattr(m, n, m.get(n, dflt) if isinstance(m, dict) else dflt))
+        sx = np.ascontiguousarray(g("src_ix"), np.int32)
+        sy = np.ascontiguousarray(g("src_iy"), np.int32)
+        rx = np.ascontiguousarray(g("rec_ix"), np.int32)
+        ry = np.ascontiguousarray(g("rec_iy"), np.int32)
+        keep += [sx, sy, rx, ry]
+        d = descs[k]
+        d.nx, d.ny, d.dh = int(g("nx")), int(g("ny")), float(g("dh"))
+        d.time = float(g("time", 6.0))
+        d.ttnorm = float(g("ttnorm", 1e-10))
+        d.eps_adj = float(g("eps_adj", 1e-9))
+        d.max_iter = int(g("max_iter", 0) or 0)
+        d.nshots, d.ntr = len(sx), len(rx)
+        ip = C.POINTER(C.c_int)
+        d.src_ix, d.src_iy = sx.ctypes.data_as(ip), sy.ctypes.data_as(ip)
+        d.rec_ix, d.rec_iy = rx.ctypes.data_as(ip), ry.ctypes.data_as(ip)
+        shapes.append((d.ny, d.nx, d.nshots, d.ntr))
+    return descs, keep, shapes
+
+
+def autotune_rows(models, device=0, hbm_budget=0):
+    """(rows per lane, ms of one outer forward + adjoint iteration) of the
+    fastest feasible setting for these models (rz_autotune_rows)."""
+    descs, keep, _ = _descs(models)
+    R, ms = C.c_int(0), C.c_double(0.0)
+    rc = lib().rz_autotune_rows(device, len(models), descs, hbm_budget, C.byref(R), C.byref(ms))
+    del keep
+    if rc != RZ_OK:
+        raise RzError(rc, lib().rz_last_error(None).decode())
+    return R.value, ms.value
+
+
 class Context:
     """One context over a list of models (dicts or Survey-like objects).
 
@@ -122,28 +165,9 @@ class Context:
     """
 
     def __init__(self, models, device=0, rows_per_lane=0, hbm_budget=0):
+        """rows_per_lane: 1, 2, 4, 8; 0 = by band count; ROWS_AUTOTUNE = timed."""
         L = lib()
-        self._keep = []
-        descs = (ModelDesc * len(models))()
-        self.shapes = []
-        for k, m in enumerate(models):
-            g = (lambda n, dflt=None: getattr(m, n, m.get(n, dflt) if isinstance(m, dict) else dflt))
-            sx = np.ascontiguousarray(g("src_ix"), np.int32)
-            sy = np.ascontiguousarray(g("src_iy"), np.int32)
-            rx = np.ascontiguousarray(g("rec_ix"), np.int32)
-            ry = np.ascontiguousarray(g("rec_iy"), np.int32)
-            self._keep += [sx, sy, rx, ry]
-            d = descs[k]
-            d.nx, d.ny, d.dh = int(g("nx")), int(g("ny")), float(g("dh"))
-            d.time = float(g("time", 6.0))
-            d.ttnorm = float(g("ttnorm", 1e-10))
-            d.eps_adj = float(g("eps_adj", 1e-9))
-            d.max_iter = int(g("max_iter", 0) or 0)
-            d.nshots, d.ntr = len(sx), len(rx)
-            ip = C.POINTER(C.c_int)
-            d.src_ix, d.src_iy = sx.ctypes.data_as(ip), sy.ctypes.data_as(ip)
-            d.rec_ix, d.rec_iy = rx.ctypes.data_as(ip), ry.ctypes.data_as(ip)
-            self.shapes.append((d.ny, d.nx, d.nshots, d.ntr))
+        descs, self._keep, self.shapes = _descs(models)
         h = C.c_void_p()
         rc = L.rz_create(C.byref(h), device, len(models), descs, rows_per_lane, hbm_budget)
         if rc != RZ_OK:

@@ -67,6 +67,16 @@ def test_invalid_rows_per_lane():
     assert "rows_per_lane" in str(e.value)
 
 
+def test_autotune_of_infeasible_model_fails_before_device():
+    from rajzel_amd import native as N
+    with pytest.raises(N.RzError) as e:
+        N.autotune_rows([_model(ny=10000)])
+    assert e.value.code == N.RZ_ERR_INVALID and "rows-per-lane" in str(e.value)
+    with pytest.raises(N.RzError) as e:
+        N.Context([_model(ny=10000)], rows_per_lane=N.ROWS_AUTOTUNE)
+    assert "autotune" in str(e.value)
+
+
 def test_null_context_calls_fail_cleanly():
     from rajzel_amd import native as N
     L = N.lib()
