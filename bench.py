@@ -139,7 +139,8 @@ def main():
     ap.add_argument("--plain-shots", type=int, default=10,
                     help="shots timed in the one-shot-per-launch form (rate is per shot)")
     ap.add_argument("--cpu-shots", type=int, default=3, help="shots of the reference CPU sample")
-    ap.add_argument("--rows-per-lane", type=int, default=0)
+    ap.add_argument("--rows-per-lane", type=int, default=-1,
+                    help="rows of a band per lane (1, 2, 4, 8); 0 = by band count, -1 = autotuned")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -222,7 +223,7 @@ def main():
 
     if not args.no_plain:
         n = min(args.plain_shots, sv.nshots)
-        with N.Context([model_dict(subset(sv, n))], rows_per_lane=args.rows_per_lane) as cp:
+        with N.Context([model_dict(subset(sv, n))], rows_per_lane=ctx.rows_per_lane) as cp:
             cp.set_slowness(0, S)
             cp.set_picks(0, sv.tobs[:n])
             cp.gradient(per_shot=True)

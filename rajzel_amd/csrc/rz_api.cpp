@@ -75,7 +75,6 @@ struct rz_ctx {
     int nslots = 0;
     long long slot_floats = 0;
     float *TA = nullptr, *TB = nullptr, *LA = nullptr, *LB = nullptr;
-    double *CA = nullptr, *CB = nullptr;   // adjoint cell codes (layouts A, B)
     // host mirror of the shot table
     std::vector<ShotDev> shots_h;
     int last_b0 = -1, last_b1 = -1;   // global shot range resident in the slots
@@ -198,7 +197,6 @@ Slots slots_of(const rz_ctx *c)
 {
     Slots s;
     s.TA = c->TA; s.TB = c->TB; s.LA = c->LA; s.LB = c->LB;
-    s.CA = c->CA; s.CB = c->CB;
     s.slot_stride = c->slot_floats;
     return s;
 }
@@ -574,9 +572,9 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         (rc = dalloc(c, &c->s_shot, 1)) || (rc = dalloc(c, &c->s_slot0, 1)))
         return bail(rc);
 
-    // field slots (T and lambda in two layouts, the adjoint cell codes in
-    // two layouts): all shots, or as many as the budget allows
-    const size_t per_slot = (size_t)c->slot_floats * (4 * sizeof(float) + 2 * sizeof(double));
+    // field slots (T and lambda in two layouts): all shots, or as many as
+    // the budget allows
+    const size_t per_slot = (size_t)c->slot_floats * 4 * sizeof(float);
     int nslots = std::max(1, shots);
     if (hbm_budget > 0) {
         const size_t fit = hbm_budget / per_slot;
@@ -586,7 +584,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     c->nslots = nslots;
     const size_t sf = (size_t)c->slot_floats * nslots;
     if ((rc = dalloc(c, &c->TA, sf)) || (rc = dalloc(c, &c->TB, sf)) || (rc = dalloc(c, &c->LA, sf)) ||
-        (rc = dalloc(c, &c->LB, sf)) || (rc = dalloc(c, &c->CA, sf)) || (rc = dalloc(c, &c->CB, sf)))
+        (rc = dalloc(c, &c->LB, sf)))
         return bail(rc);
 
     // tables

@@ -60,7 +60,6 @@ struct ShotDev {
 
 struct Slots {
     float *TA, *TB, *LA, *LB;      // base of slot 0; slot s at + s*slot_stride
-    double *CA, *CB;               // adjoint cell codes, same indexing
     long long slot_stride;         // elements
 };
 

@@ -465,55 +465,6 @@ __device__ __forceinline__ float pos_part(float x) { return (x + fabsf(x)) * 0.5
 __device__ __forceinline__ float neg_part(float x) { return (x - fabsf(x)) * 0.5f; }
 
 // ---------------------------------------------------------------------------
-// Per-cell code of the adjoint update, a double in the field layouts:
-//   NaN       cell keeps its value: frame, receiver (sweep_adj.c:26-30)
-//   -1        a TT >= INF in the stencil: lambda = 0 (sweep_adj.c:68)
-//   +0        lhs <= EPS_ADJ: lamt = 0 (sweep_adj.c:60-64)
-//   1/lhs     otherwise, RN64(1/(double)lhs): lamt = rhs/lhs is then the
-//             float rounding of (double)rhs * code (the quotient of two
-//             floats is never closer than 2^-49 to a rounding tie unless
-//             it is one, which the product then also hits exactly).
-// lhs depends on T only, so the first sweep of a solve writes the code in
-// both layouts and the later sweeps read it.
-// ---------------------------------------------------------------------------
-enum AdjMode { kAdjProduce, kAdjConsume };
-
-template <int R>
-__device__ __forceinline__ void ldd(const double *p, double (&o)[R])
-{
-    typedef __attribute__((address_space(1))) const double gdouble;
-    typedef double vd2 __attribute__((ext_vector_type(2)));
-    typedef __attribute__((address_space(1))) const vd2 gvd2;
-    if constexpr (R == 1) {
-        o[0] = *(gdouble *)p;
-    } else {
-#pragma unroll
-        for (int i = 0; i < R; i += 2) {
-            const vd2 t = *(gvd2 *)(p + i);
-            o[i] = t.x; o[i + 1] = t.y;
-        }
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void std_(double *p, const double (&v)[R])
-{
-    typedef __attribute__((address_space(1))) double gdouble;
-    typedef double vd2 __attribute__((ext_vector_type(2)));
-    typedef __attribute__((address_space(1))) vd2 gvd2;
-    if constexpr (R == 1) {
-        *(gdouble *)p = v[0];
-    } else {
-#pragma unroll
-        for (int i = 0; i < R; i += 2) {
-            vd2 t;
-            t.x = v[i]; t.y = v[i + 1];
-            *(gvd2 *)(p + i) = t;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Adjoint sweep (sweep_adj.c:10-77) in direction (SX, SY).
 //   Tf    converged traveltime in this sweep's family layout; the sign bit
 //         marks receiver cells (skipped, sweep_adj.c:30)
@@ -521,13 +472,13 @@ __device__ __forceinline__ void std_(double *p, const double (&v)[R])
 //         sweep then updates it in place) or the other family's (gathered
 //         as in fwd_sweep)
 //   Lout  lambda written in this sweep's layout
-//   Cf    cell codes in this sweep's layout: written (kAdjProduce, lhs and
-//         the skip/INF conditions evaluated from T) or read (kAdjConsume)
-// The memory operations per step are fixed, as in fwd_sweep.
+// The T-only terms (slopes, lhs, skip and INF conditions) are evaluated
+// from T in every sweep: recomputing them costs less than the HBM traffic of
+// storing them.  The memory operations per step are fixed, as in fwd_sweep.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF, int MODE>
+template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF>
 __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
-                            const float *Lin, float *Lout, double *Cf, int pre, int post)
+                            const float *Lin, float *Lout, int pre, int post)
 {
     constexpr bool FA = (SX == SY);
     constexpr bool LA_ = LSAME ? FA : !FA;   // family of Lin's layout
@@ -547,7 +498,6 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     const gfloat *tband = gp(Tf) + boff;
     const gfloat *lband = gp(Lin) + boff;
     gfloat *gLout = gpw(Lout) + boff;
-    double *cband = Cf + boff;
     const int k0 = beta * BR + l * R;
     const int kshift = LSAME ? 0 : (FA ? 63 - 2 * l : 2 * l - 63);   // key of this lane's cells in Lin
 
@@ -571,7 +521,6 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     float *ring_dummy = w.ring + kMaxBands * kRingQ + l;
 
     float qt[kPF][R], ql[kPF][R], qeu[kPF], qed[kPF], qel[kPF];
-    double qc[MODE == kAdjConsume ? kPF : 1][R];
 
 #define RZ_ADJ_LOAD(SLOT, TAU)                                                              \
     do {                                                                                    \
@@ -579,7 +528,6 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
         const int key_ = clampi(SX > 0 ? tau_ : nx + 62 - tau_, 0, nkeys - 1);              \
         ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                        \
         ldv<R>(lband + clampi(key_ + kshift, 0, nkeys) * BR + l * R, ql[SLOT]);             \
-        if constexpr (MODE == kAdjConsume) ldd<R>(cband + key_ * BR + l * R, qc[SLOT]);     \
         const int c0_ = clampi(SX > 0 ? tau_ - lag0 : nx - 1 - tau_ + lag0, 0, nx - 1);     \
         const int c63_ = clampi(SX > 0 ? tau_ - lag63 : nx - 1 - tau_ + lag63, 0, nx - 1);  \
         const long long eu_ = up_base + (long long)(FA ? c0_ + 63 : c0_) * BR;              \
@@ -614,12 +562,8 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                 ring_next = w.ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
             }
             float nv[R];
-            double cv[R];   // cell codes (kAdjProduce)
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                nv[r] = ql[u][r];
-                cv[r] = __longlong_as_double(0x7FF8000000000000ll);
-            }
+            for (int r = 0; r < R; ++r) nv[r] = ql[u][r];
             if (active) {
                 [[maybe_unused]] const bool fcol = (c <= 0) || (c >= nx - 1);
                 // T of the rows across the lane boundary, at column c
@@ -676,16 +620,12 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     apm[r] = neg_part(ap);
                     bmp[r] = pos_part(bm);
                     bpm[r] = neg_part(bp);
-                    if constexpr (MODE == kAdjProduce) {
-                        const bool rec = __float_as_uint(traw) >> 31;
-                        lhs[r] = qdh<FD>(pos_part(ap) - neg_part(am), dh, rdh)
-                               + qdh<FD>(pos_part(bp) - neg_part(bm), dh, rdh);
-                        keep[r] = !cvalid || fcol || frow[r] || rec;
-                        if constexpr (CHKINF) infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
-                        else infv[r] = false;
-                        cv[r] = keep[r] ? __longlong_as_double(0x7FF8000000000000ll)
-                              : infv[r] ? -1.0 : (lhs[r] > eps ? 1.0 / (double)lhs[r] : 0.0);
-                    }
+                    const bool rec = __float_as_uint(traw) >> 31;
+                    lhs[r] = qdh<FD>(pos_part(ap) - neg_part(am), dh, rdh)
+                           + qdh<FD>(pos_part(bp) - neg_part(bm), dh, rdh);
+                    keep[r] = !cvalid || fcol || frow[r] || rec;
+                    if constexpr (CHKINF) infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
+                    else infv[r] = false;
                 }
                 float fsum = 0.f;
 #pragma unroll
@@ -704,19 +644,10 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     }
                     const float rhs = qdh<FD>(amp[r] * lxm - apm[r] * lxp, dh, rdh)
                                     + qdh<FD>(bmp[r] * lym - bpm[r] * lyp, dh, rdh);
-                    if constexpr (MODE == kAdjProduce) {
-                        const float lt = (lhs[r] > eps) ? rhs / lhs[r] : 0.f;      // sweep_adj.c:60-64
-                        float v = mn(lc, lt);                                      // sweep_adj.c:66
-                        if (infv[r]) v = 0.f;                                      // sweep_adj.c:68
-                        nv[r] = keep[r] ? lc : v;
-                    } else {
-                        const double code = qc[u][r];
-                        float lt = (float)((double)rhs * code);                    // rhs/lhs
-                        lt = code > 0.0 ? lt : 0.f;                                // lhs <= EPS_ADJ
-                        float v = mn(lc, lt);
-                        if constexpr (CHKINF) v = code < 0.0 ? 0.f : v;            // TT >= INF
-                        nv[r] = code == code ? v : lc;                             // skipped cell
-                    }
+                    const float lt = (lhs[r] > eps) ? rhs / lhs[r] : 0.f;          // sweep_adj.c:60-64
+                    float v = mn(lc, lt);                                          // sweep_adj.c:66
+                    if (infv[r]) v = 0.f;                                          // sweep_adj.c:68
+                    nv[r] = keep[r] ? lc : v;
                     if (cvalid && rvalid[r]) fsum += lc - nv[r];
                 }
                 acc += (double)fsum;
@@ -730,7 +661,6 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
             {
                 const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
                 stv<R>(gLout + ks * BR + l * R, nv);
-                if constexpr (MODE == kAdjProduce) std_<R>(cband + ks * BR + l * R, cv);
             }
             float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
@@ -899,7 +829,6 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
     const long long so = (long long)sh.slot * slots.slot_stride;
     const float *TA = slots.TA + so, *TB = slots.TB + so;
     float *LA = slots.LA + so, *LB = slots.LB + so;
-    double *CA = slots.CA + so, *CB = slots.CB + so;
     // init_travel_adj.c:22-58: 0 on the frame, 10*TIME inside, tres at receivers
     const float big = (float)(10.0 * (double)md.time);
     const int bs = (int)md.bstride;
@@ -923,30 +852,15 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
         // (+,+) A, (+,-) B, (-,+) B, (-,-) A (eikonal_adj.c:43-46): lambda
         // stays in A, moves to B, stays, moves back; sweeps 1+2 and 3+4 run
-        // as pairs (same x direction).  The first sweep of each family writes
-        // that family's cell codes, later ones read them.
+        // as pairs (same x direction).
         double acc;
-        if (it == 0) {
-            if (w.group == 0)
-                acc = adj_sweep<R, +1, +1, true, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA,
-                                                                          CA, 0, D);
-            else
-                acc = adj_sweep<R, +1, -1, false, FD, CHKINF, kAdjProduce>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB,
-                                                                           CB, D, 0);
+        if (w.group == 0) {
+            acc = adj_sweep<R, +1, +1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, 0, D);
+            acc += adj_sweep<R, -1, +1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, 0, D);
         } else {
-            if (w.group == 0)
-                acc = adj_sweep<R, +1, +1, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA,
-                                                                          CA, 0, D);
-            else
-                acc = adj_sweep<R, +1, -1, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB,
-                                                                           CB, D, 0);
+            acc = adj_sweep<R, +1, -1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB, D, 0);
+            acc += adj_sweep<R, -1, -1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LB, LA, D, 0);
         }
-        if (w.group == 0)
-            acc += adj_sweep<R, -1, +1, true, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, CB,
-                                                                       0, D);
-        else
-            acc += adj_sweep<R, -1, -1, false, FD, CHKINF, kAdjConsume>(w, md.dh, md.rdh, md.eps_adj, TA, LB, LA,
-                                                                        CA, D, 0);
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
         ++it;

@@ -133,7 +133,7 @@ def test_per_shot_and_budget_equal_batched(ora):
     # nb*(nx+64)*64R elements): 3 batches
     R = 1
     nb = -(-sv.ny // (64 * R))
-    slot = -(-(nb * (sv.nx + 64) * 64 * R) // 64) * 64 * (4 * 4 + 2 * 8)
+    slot = -(-(nb * (sv.nx + 64) * 64 * R) // 64) * 64 * (4 * 4)
     with N.Context([model_dict(sv, max_iter=50)], rows_per_lane=R, hbm_budget=2 * slot) as ctx:
         assert ctx.batch_shots == 2
         ctx.set_slowness(0, sv.S)

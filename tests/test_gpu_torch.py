@@ -46,7 +46,7 @@ def test_backward_recomputes_under_hbm_budget(ora):
     vp.grad = None
     R = full.ctx.rows_per_lane
     nb = -(-sv.ny // (64 * R))
-    slot = -(-(nb * (sv.nx + 64) * 64 * R) // 64) * 64 * (4 * 4 + 2 * 8)
+    slot = -(-(nb * (sv.nx + 64) * 64 * R) // 64) * 64 * (4 * 4)
     small = TF.Solver(sv, hbm_budget=2 * slot)
     assert small.ctx.batch_shots == 2
     TF.misfit(vp, small, tobs).backward()
