@@ -90,6 +90,47 @@ __device__ __forceinline__ void stv(gfloat *p, const float (&v)[R])
     }
 }
 
+// R floats of one lane in LDS (a wave's staging line, see Stage).
+typedef __attribute__((address_space(3))) float lfloat;
+typedef __attribute__((address_space(3))) vf2 lvf2;
+typedef __attribute__((address_space(3))) vf4 lvf4;
+
+template <int R>
+__device__ __forceinline__ void ldl(const lfloat *p, float (&o)[R])
+{
+    if constexpr (R == 1) {
+        o[0] = p[0];
+    } else if constexpr (R == 2) {
+        const vf2 t = *reinterpret_cast<const lvf2 *>(p);
+        o[0] = t.x; o[1] = t.y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; i += 4) {
+            const vf4 t = *reinterpret_cast<const lvf4 *>(p + i);
+            o[i] = t.x; o[i + 1] = t.y; o[i + 2] = t.z; o[i + 3] = t.w;
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void stl(lfloat *p, const float (&v)[R])
+{
+    if constexpr (R == 1) {
+        p[0] = v[0];
+    } else if constexpr (R == 2) {
+        vf2 t;
+        t.x = v[0]; t.y = v[1];
+        *reinterpret_cast<lvf2 *>(p) = t;
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; i += 4) {
+            vf4 t;
+            t.x = v[i]; t.y = v[i + 1]; t.z = v[i + 2]; t.w = v[i + 3];
+            *reinterpret_cast<lvf4 *>(p + i) = t;
+        }
+    }
+}
+
 template <int R>
 __device__ __forceinline__ int addr_a(int k, int c, long long bstride)
 {
@@ -264,10 +305,11 @@ __device__ __forceinline__ void epoch_sync(int i)
 // q + (63 - lag) + (nb-1-p)*kBandOff + kPF*D - kPF and needs the leading
 // sweep's values up to column q+1, stored at q + 1 + lag + p*kBandOff; the
 // difference is at least kPF*D - kBandOff*nb.  It must exceed one full-fence
-// interval plus one epoch (8*kPF + kPF steps).
-__device__ __forceinline__ int pair_delay(int nb)
+// interval plus one epoch (8*kPF + kPF steps), plus `hold`, the steps by
+// which a sweep may hold a value before storing it (staged stores).
+__device__ __forceinline__ int pair_delay(int nb, int hold = 0)
 {
-    return (kBandOff * nb + 9 * kPF + kPF - 1) / kPF;
+    return (kBandOff * nb + 9 * kPF + hold + kPF - 1) / kPF;
 }
 
 struct WaveCtx {
@@ -276,10 +318,29 @@ struct WaveCtx {
     float *ring;
 };
 
-__device__ __forceinline__ int sweep_steps(int nx, int nb)
+// Steps of a sweep: the last band's keys plus `hold` steps to drain staged
+// stores, rounded up to whole epochs.
+__device__ __forceinline__ int sweep_steps(int nx, int nb, int hold = 0)
 {
-    return ((nb - 1) * kBandOff + nx + 63 + kPF - 1) / kPF * kPF;
+    return ((nb - 1) * kBandOff + nx + 63 + hold + kPF - 1) / kPF * kPF;
 }
+
+// Staged stores of a forward sweep into the other family's layout.  In that
+// layout the R cells lane l computes at one step land in a different key
+// block for every lane (key + 63 - 2l from A to B, key + 2l - 63 from B to
+// A); stored directly they would be 16-byte pieces of 64 different lines.
+// Instead each lane keeps its pieces in an LDS delay line and stores the
+// piece of step tau - d_l at step tau, where d_l = 2|j_last - j| for lane
+// j = l mod P of its 64-byte segment of P = 16/R lanes: the P lanes of a
+// segment then store the same segment of one block together, and every
+// store instruction writes whole 64-byte segments.
+template <int R>
+struct Stage {
+    static constexpr int P = 16 / R;                 // lanes per 64-byte segment
+    static constexpr int HOLD = 2 * (P - 1);         // longest delay, in steps
+    static constexpr int Q = (HOLD < 4) ? 4 : (HOLD < 8) ? 8 : (HOLD < 16) ? 16 : 32;   // slots (power of 2)
+    static constexpr int WAVE_FLOATS = Q * kWave * R;                                   // 2048 for every R
+};
 
 // The barriers of `nep` epochs without a sweep (a group with nothing to do).
 __device__ __forceinline__ void idle_epochs(int nep)
@@ -290,17 +351,20 @@ __device__ __forceinline__ void idle_epochs(int nep)
 
 // ---------------------------------------------------------------------------
 // Forward sweep (sweep.c:10-104) in direction (SX, SY) over one shot.
-//   Tin    field written by the previous sweep, in the other family's layout
-//          (consecutive forward sweeps alternate between families A and B)
-//   Sf     slowness in this sweep's family layout (A if SX==SY, else B)
-//   Tout   written in this sweep's family layout
+//   Tin    field written by the previous sweep, in this sweep's family
+//          layout (A if SX==SY, else B)
+//   Sf     slowness in this sweep's family layout
+//   Tout   written in the other family's layout, which the next sweep reads
+//          (consecutive forward sweeps alternate between families A and B),
+//          through the staged stores of Stage
+//   stage  this wave's LDS delay line (Stage<R>::WAVE_FLOATS floats)
 //   INIT   read the initial field (2*TIME, 0 at the source) instead of Tin
 // Returns this lane's sum of (old - new) over its valid cells.
 //
-// Reads of Tin gather each lane's R cells from the other layout (one 16-byte
-// piece per lane, different lines); writes are whole 64*R-float blocks.  Lines
-// read piecewise are full lines to DRAM and reused from L2, while lines
-// written piecewise would reach memory as partial writes.
+// All loads are whole 64*R-float blocks of this sweep's layout and all
+// stores whole 64-byte segments of the other layout.  Garbage (values of
+// lanes outside the grid or of steps before the first one) only reaches
+// positions of cells outside the grid, which no valid cell reads.
 //
 // Every step issues the same memory operations whatever the lane or step
 // validity (invalid lanes write to the band's dummy key block nx+63 and to a
@@ -309,13 +373,15 @@ __device__ __forceinline__ void idle_epochs(int nep)
 // ---------------------------------------------------------------------------
 template <int R, int SX, int SY, bool INIT>
 __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const float *Tin,
-                            const float *Sf, float *Tout, int src_k, int src_c, float tinit, int pre, int post)
+                            const float *Sf, float *Tout, float *stage, int src_k, int src_c, float tinit, int pre,
+                            int post)
 {
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
+    constexpr int SP = Stage<R>::P, SQ = Stage<R>::Q;
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
     const int nkeys = nx + 63;          // data keys 0..nx+62; key nkeys is the dummy block
-    const int gend = sweep_steps(nx, nb);
+    const int gend = sweep_steps(nx, nb, Stage<R>::HOLD);
     const int nep = gend / kPF;
     if (!w.active) {
         idle_epochs(pre + nep + post);
@@ -328,9 +394,13 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
     const gfloat *tband = gp(Tin) + boff;
     const gfloat *sband = gp(Sf) + boff;
     gfloat *gTout = gpw(Tout) + boff;
+    lfloat *lstage = (lfloat *)stage + l * R;
     const int k0 = beta * BR + l * R;
     // key of this lane's cells in the other family: B key = A key - 2l + 63
     const int kshift = FA ? 63 - 2 * l : 2 * l - 63;
+    // delay of this lane's staged stores (Stage)
+    const int sj = l % SP, sjl = ((SX > 0) == FA) ? SP - 1 : 0;
+    const int dl = 2 * (sjl > sj ? sjl - sj : sj - sjl);
 
     bool rvalid[R], top[R], bot[R];
 #pragma unroll
@@ -364,9 +434,8 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
                 qt[SLOT][r_] = (k0 + r_ == src_k && c_ == src_c) ? 0.f : tinit;             \
             qe[SLOT] = (erow == src_k && ce_ == src_c) ? 0.f : tinit;                       \
         } else {                                                                            \
-            const int kin_ = clampi(key_ + kshift, 0, nkeys);                               \
-            ldv<R>(tband + kin_ * BR + l * R, qt[SLOT]);                                    \
-            const int ek_ = SY > 0 ? (FA ? ce_ + 63 : ce_) : (FA ? ce_ : ce_ + 63);         \
+            ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                    \
+            const int ek_ = SY > 0 ? (FA ? ce_ : ce_ + 63) : (FA ? ce_ + 63 : ce_);         \
             qe[SLOT] = eband[ek_ * BR + (SY > 0 ? 0 : 63 * R + R - 1)];                     \
         }                                                                                   \
     } while (0)
@@ -430,9 +499,14 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
 #pragma unroll
                 for (int r = 0; r < R; ++r) pnew[r] = nv[r];
             }
-            {
-                const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
-                stv<R>(gTout + ks * BR + l * R, nv);
+            {   // staged store: keep this step's cells, store those of step tau - dl
+                stl<R>(lstage + (tau & (SQ - 1)) * BR, nv);
+                const int tp = tau - dl;
+                float pc[R];
+                ldl<R>(lstage + (tp & (SQ - 1)) * BR, pc);
+                int ko = (SX > 0 ? tp : nx + 62 - tp) + kshift;
+                ko = (unsigned)ko < (unsigned)nkeys ? ko : nkeys;
+                stv<R>(gTout + ko * BR + l * R, pc);
             }
             float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
@@ -675,30 +749,31 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
 }
 
 // ---------------------------------------------------------------------------
-// Copy a field from layout B to layout A (one wave per band, 8 keys per
-// round: 8 block loads, then 8 stores).  Keys of lanes outside the grid go
-// to the dummy block.
+// Copy a field from layout A to layout B (one wave per band, 8 keys per
+// round): each lane gathers its R cells of B key `key` from A key
+// key + 2l - 63 (the A dummy block for cells outside the grid), then whole B
+// blocks are stored.
 // ---------------------------------------------------------------------------
 template <int R>
-__device__ void copy_b_to_a(const WaveCtx &w, const float *fb, float *fa)
+__device__ void copy_a_to_b(const WaveCtx &w, const float *fa, float *fb)
 {
     constexpr int BR = kWave * R;
     if (!w.active) return;
     const int nx = w.nx, l = w.lane;
     const int nkeys = nx + 63;
     const long long boff = (long long)w.beta * w.bstride;
-    const gfloat *src = gp(fb) + boff;
-    gfloat *dst = gpw(fa) + boff;
+    const gfloat *src = gp(fa) + boff;
+    gfloat *dst = gpw(fb) + boff;
     for (int g = 0; g < nkeys; g += 8) {
         float v[8][R];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) ldv<R>(src + min(g + u, nkeys - 1) * BR + l * R, v[u]);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int key = g + u, c = key + l - 63;
             const int ka = (key < nkeys && (unsigned)c < (unsigned)nx) ? c + l : nkeys;
-            stv<R>(dst + ka * BR + l * R, v[u]);
+            ldv<R>(src + ka * BR + l * R, v[u]);
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) stv<R>(dst + min(g + u, nkeys) * BR + l * R, v[u]);
     }
 }
 
@@ -730,19 +805,21 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
 {
     __shared__ float ring[kGroups * (kMaxBands * kRingQ + kWave)];
     __shared__ double red[kGroups * kMaxBands];
+    __shared__ float stage_all[kGroups * max_bands_for(R) * Stage<R>::WAVE_FLOATS];
     const ShotDev sh = shots[blockIdx.x];
     const ModelDev md = mds[sh.model];
     const int nwaves = blockDim.x >> 6;
     const WaveCtx w = make_wave(md, ring);
-    const int nep = sweep_steps(md.nx, md.nb) / kPF;
-    const int D = pair_delay(md.nb);
+    float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
+    const int nep = sweep_steps(md.nx, md.nb, Stage<R>::HOLD) / kPF;
+    const int D = pair_delay(md.nb, Stage<R>::HOLD);
     const long long so = (long long)sh.slot * slots.slot_stride;
     float *TA = slots.TA + so, *TB = slots.TB + so;
     const float tinit = (float)(2.0 * (double)md.time);          // zero_travel.c:22
     float lnorm = (float)(10.0 * (double)md.time);                // eikonal.c:39
     const float denom = (float)(md.nx * md.ny);
     // Sweep order (+,+) A, (-,+) B, (-,-) A, (+,-) B (eikonal.c:50-53); each
-    // reads the previous sweep's layout and writes its own.  Sweeps 2+3 and
+    // reads its own family's layout and writes the next sweep's.  Sweeps 2+3 and
     // sweep 4 + the next iteration's sweep 1 run as pairs (same x
     // direction); that next sweep 1 is run ahead of the stopping test and
     // simply dropped if the loop ends.
@@ -753,31 +830,34 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         double acc = 0.0;
         if (it == 0) {
             if (w.group == 0)
-                acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit,
-                                                 0, 0);
+                acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
+                                                 tinit, 0, 0);
             else
                 idle_epochs(nep);
         } else {
             acc = acc_next;
         }
         if (w.group == 0)
-            acc += fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TA, md.SB, TB, sh.src_k, sh.src_c, tinit, 0, D);
+            acc += fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k, sh.src_c,
+                                               tinit, 0, D);
         else
-            acc += fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit, D, 0);
+            acc += fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
+                                               tinit, D, 0);
         if (w.group == 0) {
-            acc += fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TA, md.SB, TB, sh.src_k, sh.src_c, tinit, 0, D);
+            acc += fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k, sh.src_c,
+                                               tinit, 0, D);
             acc_next = 0.0;
         } else {
-            acc_next = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, TB, md.SA, TA, sh.src_k, sh.src_c, tinit,
-                                                   D, 0);
+            acc_next = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
+                                                   tinit, D, 0);
         }
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;                               // norm1.c:27
         ++it;
     }
     const int nfield = md.nb * (int)md.bstride;
-    if (it != 0) {  // the last sweep wrote layout B; the adjoint also needs A
-        if (w.group == 0) copy_b_to_a<R>(w, TB, TA);
+    if (it != 0) {  // the last sweep wrote layout A; the adjoint also needs B
+        if (w.group == 0) copy_a_to_b<R>(w, TA, TB);
         __syncthreads();
     } else {        // loop never entered: the field is the initial one in both layouts
         for (int p = threadIdx.x; p < nfield; p += blockDim.x) {
