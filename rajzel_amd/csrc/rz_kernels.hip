@@ -542,24 +542,25 @@ __device__ __forceinline__ float neg_part(float x) { return (x - fabsf(x)) * 0.5
 // Adjoint sweep (sweep_adj.c:10-77) in direction (SX, SY).
 //   Tf    converged traveltime in this sweep's family layout; the sign bit
 //         marks receiver cells (skipped, sweep_adj.c:30)
-//   Lin   lambda from the previous sweep: this sweep's layout (LSAME, the
-//         sweep then updates it in place) or the other family's (gathered
-//         as in fwd_sweep)
-//   Lout  lambda written in this sweep's layout
+//   Lin   lambda from the previous sweep, in this sweep's layout
+//   Lout  lambda written: in this sweep's layout (LSAME, Lout == Lin, the
+//         sweep updates it in place) or the other family's, through the
+//         staged stores of fwd_sweep (stage: this wave's LDS delay line)
 // The T-only terms (slopes, lhs, skip and INF conditions) are evaluated
 // from T in every sweep: recomputing them costs less than the HBM traffic of
 // storing them.  The memory operations per step are fixed, as in fwd_sweep.
 // ---------------------------------------------------------------------------
 template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF>
 __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
-                            const float *Lin, float *Lout, int pre, int post)
+                            const float *Lin, float *Lout, float *stage, int pre, int post)
 {
     constexpr bool FA = (SX == SY);
-    constexpr bool LA_ = LSAME ? FA : !FA;   // family of Lin's layout
+    constexpr bool LA_ = FA;   // family of Lin's layout
     constexpr int BR = kWave * R;
+    constexpr int SP = Stage<R>::P, SQ = Stage<R>::Q;
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
     const int nkeys = nx + 63;
-    const int gend = sweep_steps(nx, nb);
+    const int gend = sweep_steps(nx, nb, Stage<R>::HOLD);   // the same for both kinds of output
     const int nep = gend / kPF;
     if (!w.active) {
         idle_epochs(pre + nep + post);
@@ -572,8 +573,13 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     const gfloat *tband = gp(Tf) + boff;
     const gfloat *lband = gp(Lin) + boff;
     gfloat *gLout = gpw(Lout) + boff;
+    lfloat *lstage = (lfloat *)stage + l * R;
     const int k0 = beta * BR + l * R;
-    const int kshift = LSAME ? 0 : (FA ? 63 - 2 * l : 2 * l - 63);   // key of this lane's cells in Lin
+    // key of this lane's cells in the other family, and the delay of its
+    // staged stores (as in fwd_sweep)
+    const int kshift = FA ? 63 - 2 * l : 2 * l - 63;
+    const int sj = l % SP, sjl = ((SX > 0) == FA) ? SP - 1 : 0;
+    const int dl = 2 * (sjl > sj ? sjl - sj : sj - sjl);
 
     bool rvalid[R], frow[R];
 #pragma unroll
@@ -601,7 +607,7 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
         const int tau_ = (TAU);                                                             \
         const int key_ = clampi(SX > 0 ? tau_ : nx + 62 - tau_, 0, nkeys - 1);              \
         ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                        \
-        ldv<R>(lband + clampi(key_ + kshift, 0, nkeys) * BR + l * R, ql[SLOT]);             \
+        ldv<R>(lband + key_ * BR + l * R, ql[SLOT]);                                        \
         const int c0_ = clampi(SX > 0 ? tau_ - lag0 : nx - 1 - tau_ + lag0, 0, nx - 1);     \
         const int c63_ = clampi(SX > 0 ? tau_ - lag63 : nx - 1 - tau_ + lag63, 0, nx - 1);  \
         const long long eu_ = up_base + (long long)(FA ? c0_ + 63 : c0_) * BR;              \
@@ -732,9 +738,17 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     tprev[r] = qt[u][r];
                 }
             }
-            {
+            if constexpr (LSAME) {
                 const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
                 stv<R>(gLout + ks * BR + l * R, nv);
+            } else {   // staged store: keep this step's cells, store those of step tau - dl
+                stl<R>(lstage + (tau & (SQ - 1)) * BR, nv);
+                const int tp = tau - dl;
+                float pc[R];
+                ldl<R>(lstage + (tp & (SQ - 1)) * BR, pc);
+                int ko = (SX > 0 ? tp : nx + 62 - tp) + kshift;
+                ko = (unsigned)ko < (unsigned)nkeys ? ko : nkeys;
+                stv<R>(gLout + ko * BR + l * R, pc);
             }
             float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
@@ -901,11 +915,13 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
     constexpr int BR = kWave * R;
     __shared__ float ring[kGroups * (kMaxBands * kRingQ + kWave)];
     __shared__ double red[kGroups * kMaxBands];
+    __shared__ float stage_all[kGroups * max_bands_for(R) * Stage<R>::WAVE_FLOATS];
     const ShotDev sh = shots[blockIdx.x];
     const ModelDev md = mds[sh.model];
     const int nwaves = blockDim.x >> 6;
     const WaveCtx w = make_wave(md, ring);
-    const int D = pair_delay(md.nb);
+    float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
+    const int D = pair_delay(md.nb, Stage<R>::HOLD);
     const long long so = (long long)sh.slot * slots.slot_stride;
     const float *TA = slots.TA + so, *TB = slots.TB + so;
     float *LA = slots.LA + so, *LB = slots.LB + so;
@@ -930,16 +946,17 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
     int it = 0;
     while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
-        // (+,+) A, (+,-) B, (-,+) B, (-,-) A (eikonal_adj.c:43-46): lambda
-        // stays in A, moves to B, stays, moves back; sweeps 1+2 and 3+4 run
-        // as pairs (same x direction).
+        // (+,+) A, (+,-) B, (-,+) B, (-,-) A (eikonal_adj.c:43-46): sweep 1
+        // reads lambda in A and writes B, sweep 2 updates B in place, sweep 3
+        // reads B and writes A, sweep 4 updates A in place; sweeps 1+2 and
+        // 3+4 run as pairs (same x direction).
         double acc;
         if (w.group == 0) {
-            acc = adj_sweep<R, +1, +1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, 0, D);
-            acc += adj_sweep<R, -1, +1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, 0, D);
+            acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB, stage, 0, D);
+            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA, stage, 0, D);
         } else {
-            acc = adj_sweep<R, +1, -1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LA, LB, D, 0);
-            acc += adj_sweep<R, -1, -1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LB, LA, D, 0);
+            acc = adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, stage, D, 0);
+            acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, stage, D, 0);
         }
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
