@@ -534,9 +534,24 @@ __device__ __forceinline__ float qdh(float x, float dh, double rdh)
     else return x / dh;
 }
 
-// (x + |x|)/2 and (x - |x|)/2 of sweep_adj.c:39-49: exact, signed zeros kept.
-__device__ __forceinline__ float pos_part(float x) { return (x + fabsf(x)) * 0.5f; }
-__device__ __forceinline__ float neg_part(float x) { return (x - fabsf(x)) * 0.5f; }
+// (x + |x|)/2 and (x - |x|)/2 of sweep_adj.c:39-49 for finite |x| < 2^127:
+// max(x, +0) and min(x, +0).  For x != 0 both are exact; for x = -0 the
+// formulas give +0 and -0, and v_max_f32 / v_min_f32 order -0 below +0, so
+// max(-0, +0) = +0 and min(-0, +0) = -0.  Operands are never NaN (slopes
+// of finite traveltimes); the inline form keeps the compiler from
+// canonicalising them first.
+__device__ __forceinline__ float pos_part(float x)
+{
+    float r;
+    asm("v_max_f32 %0, 0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ float neg_part(float x)
+{
+    float r;
+    asm("v_min_f32 %0, 0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 
 // ---------------------------------------------------------------------------
 // Adjoint sweep (sweep_adj.c:10-77) in direction (SX, SY).

@@ -8,8 +8,10 @@ GPU's default mode).
   also reproduces exactly, while the double product errs by < 2^-52.  The
   adjoint uses it for x/DH (sweep_adj.c:33-54) and for rhs/lhs
   (sweep_adj.c:60-64) with RN64(1/lhs) computed once per cell.
-* (x + |x|)/2 and (x - |x|)/2 in float equal the reference's double
-  evaluation rounded to float (sweep_adj.c:39-49), signed zeros included.
+* (x + |x|)/2 and (x - |x|)/2 in float, and max(x, +0) / min(x, +0) with
+  -0 ordered below +0 (v_max_f32 / v_min_f32 on gfx950), equal the
+  reference's double evaluation rounded to float (sweep_adj.c:39-49),
+  signed zeros included.
 """
 import numpy as np
 
@@ -65,6 +67,17 @@ def test_quotient_by_dh_values():
         assert eq, dh
 
 
+def _max0(x):
+    """v_max_f32(x, +0) on gfx950: -0 orders below +0, so max(-0, +0) = +0."""
+    return np.where(x > 0, x, f32(0.0)).astype(f32)
+
+
+def _min0(x):
+    """v_min_f32(x, +0) on gfx950: min(-0, +0) = -0 (and +0 for x = +0)."""
+    neg_zero = (x == 0) & np.signbit(x)
+    return np.where(x < 0, x, np.where(neg_zero, f32(-0.0), f32(0.0))).astype(f32)
+
+
 def test_positive_and_negative_parts_keep_signed_zeros():
     # |x| < 1.7e38 (slopes of traveltimes are far below it)
     rng = np.random.default_rng(4)
@@ -72,7 +85,11 @@ def test_positive_and_negative_parts_keep_signed_zeros():
                            _spread_floats(rng, 100_000, -149, 126)])
     ref_p = ((vals.astype(f64) + np.abs(vals.astype(f64))) / 2.0).astype(f32)
     ref_m = ((vals.astype(f64) - np.abs(vals.astype(f64))) / 2.0).astype(f32)
+    # the float form of the formulas
     p = (vals + np.abs(vals)) * f32(0.5)
     m = (vals - np.abs(vals)) * f32(0.5)
     assert np.array_equal(p.view(np.uint32), ref_p.view(np.uint32))
     assert np.array_equal(m.view(np.uint32), ref_m.view(np.uint32))
+    # the kernels' form: max(x, +0) and min(x, +0) with -0 below +0
+    assert np.array_equal(_max0(vals).view(np.uint32), ref_p.view(np.uint32))
+    assert np.array_equal(_min0(vals).view(np.uint32), ref_m.view(np.uint32))
