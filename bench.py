@@ -61,6 +61,95 @@ def algorithmic_bytes(nx, ny, it_f, it_a):
     return fwd, adj, grd
 
 
+def gauss_filter_timing(nx, ny, fsx=81, fsy=41, reps=5, cpu_rows=84):
+    """The stock Gaussian gradient filter (FILT_SIZE_GRAD x FILT_SIZE_GRAD1 =
+    81 x 41, gauss_filt.c) on an ny x nx field: GPU time per call of
+    rz_gauss_filt on a device buffer (its weight and input copies and their
+    allocation included), and the reference's own C filter on the first
+    cpu_rows rows (1 thread), scaled to the grid.  Rows 0..cpu_rows-21 of
+    that sample depend on the same inputs as in the full grid, so they are
+    compared bit for bit with the GPU result."""
+    import ctypes as C
+    from rajzel_amd import native as N
+    rng = np.random.default_rng(12)
+    A = rng.normal(0.0, 1e-3, (ny, nx)).astype(np.float32)
+    hip = C.CDLL("libamdhip64.so.7")
+    n = C.c_size_t(A.nbytes)
+    d = C.c_void_p()
+    if hip.hipMalloc(C.byref(d), n) != 0:
+        raise RuntimeError("hipMalloc failed")
+    try:
+        times = []
+        for _ in range(reps + 1):
+            if hip.hipMemcpy(d, A.ctypes.data_as(C.c_void_p), n, 1) != 0:
+                raise RuntimeError("hipMemcpy failed")
+            x, y = C.c_int(fsx), C.c_int(fsy)
+            t0 = time.perf_counter()
+            rc = N.lib().rz_gauss_filt(0, None, d, nx, ny, C.byref(x), C.byref(y), 1)
+            times.append(time.perf_counter() - t0)
+            if rc != N.RZ_OK:
+                raise RuntimeError(N.lib().rz_last_error(None).decode())
+        out = np.empty_like(A)
+        if hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), d, n, 2) != 0:
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(d)
+    gpu_ms = float(np.median(times[1:])) * 1e3
+    res = {"grid": f"{ny}x{nx}", "filter": f"{fsx}x{fsy}", "gpu_ms": gpu_ms,
+           "mflop_per_call": 2.0 * nx * ny * fsx * fsy / 1e6}
+    try:
+        from oracle.bindings import Reference
+        ref = Reference()
+        sub = A[:cpu_rows].copy()
+        ref.setup(nx, cpu_rows, 5.0)
+        t0 = time.perf_counter()
+        r, _, _ = ref.gauss_filt(sub, fsx, fsy)
+        dt = time.perf_counter() - t0
+        keep = cpu_rows - fsy // 2
+        res.update(cpu_reference_ms_scaled=dt * 1e3 * ny / cpu_rows, cpu_sample_rows=cpu_rows,
+                   bitwise_vs_reference_rows=bool(np.array_equal(r[:keep].view(np.uint32),
+                                                                 out[:keep].view(np.uint32))))
+    except OSError as e:
+        log(f"reference CPU build unavailable for the filter ({e})")
+    return res
+
+
+def concurrency_scan(sv, rows_per_lane, counts=(1, 8, 64, 128, 200)):
+    """Per-outer-iteration time of the forward and adjoint kernels with n
+    shots in flight (n = 1 ... 200, one workgroup per shot): the difference
+    of runs capped at 3 and at 1 outer iterations, halved.  Flat times mean
+    a per-CU limit (instruction issue, latency); times growing with n mean a
+    shared one (HBM, L2, fabric)."""
+    from rajzel_amd import native as N
+    rows = []
+    for n in counts:
+        res = {}
+        for it in (1, 3):
+            md = dict(model_dict(subset(sv, n)), max_iter=it)
+            with N.Context([md], rows_per_lane=rows_per_lane) as ctx:
+                ctx.set_slowness(0, sv.S)
+                ctx.set_picks(0, np.full((n, sv.ntr), 0.5, np.float32))
+
+                def run():
+                    try:
+                        ctx.gradient()
+                    except N.NotConverged:
+                        pass
+                run()
+                ctx.set_profiling(True)
+                ctx.sync()
+                for _ in range(2):
+                    run()
+                ctx.sync()
+                res[it] = {k: v[0] / max(v[1], 1) for k, v in ctx.profile().items()}
+                R = ctx.rows_per_lane
+        rows.append({"shots": n, "rows_per_lane": R,
+                     "forward_ms_per_iteration": (res[3]["forward"] - res[1]["forward"]) / 2,
+                     "adjoint_ms_per_iteration": (res[3]["adjoint"] - res[1]["adjoint"]) / 2})
+        log(f"[bench] concurrency {rows[-1]}")
+    return rows
+
+
 def copy_probe(mib):
     """One device-to-device hipMemcpy of ``mib`` MiB: the plain copy whose
     FETCH_SIZE / WRITE_SIZE readings calibrate the HBM-traffic counters."""
@@ -143,6 +232,9 @@ def main():
                     help="rows of a band per lane (1, 2, 4, 8); 0 = by band count, -1 = autotuned")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-precond", action="store_true", help="skip the Gaussian gradient filter timing")
+    ap.add_argument("--concurrency-scan", action="store_true",
+                    help="only print per-iteration kernel times versus shots in flight")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="finish with one device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_fatt_gradient_1001x3001_200shots.json"),
@@ -163,6 +255,10 @@ def main():
     S_true = W.slowness(sv.vp_true)
     log(f"[bench] workload {sv.name}: {sv.nshots} shots, {sv.ntr} receivers "
         f"({time.perf_counter() - t_setup:.1f} s to generate)")
+    if args.concurrency_scan:
+        print(json.dumps({"workload": f"{sv.ny}x{sv.nx} grid, {sv.ntr} receivers, picks 0.5 s",
+                          "concurrency": concurrency_scan(sv, args.rows_per_lane)}), flush=True)
+        return
 
     ctx = N.Context([model_dict(sv)], rows_per_lane=args.rows_per_lane)
     # picks: forward in the true model + seeded noise (setup, untimed)
@@ -246,6 +342,9 @@ def main():
         out["parity_sample"] = parity
         log(f"[bench] cpu baseline: {cb['value']:.3f} shot-gradients/s; parity {parity}")
     ctx.close()
+    if not args.no_precond:
+        out["precond_gauss_filt"] = gauss_filter_timing(sv.nx, sv.ny)
+        log(f"[bench] gauss filter: {out['precond_gauss_filt']}")
     if args.copy_probe:
         copy_probe(args.copy_probe)
     print(json.dumps(out), flush=True)

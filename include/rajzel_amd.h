@@ -188,6 +188,19 @@ int rz_get_profile(const rz_ctx *ctx, double *ms, long long *launches);
  * problems, where launch overhead matters. */
 int rz_set_graph(rz_ctx *ctx, int on);
 
+/* Gaussian smoothing of an NY*NX row-major field in place: gauss_filt.c:10-139
+ * (precond.c:35-37 calls it for SPATFILTER = 4 on the gradient, fatt.c:179
+ * on the search direction), bitwise.  The field is extended by edge replication and
+ * convolved with the normalised |fsx| x |fsy| Gaussian, each output summed
+ * in the reference's order.  *fsx, *fsy are FILT_SIZE_GRAD and
+ * FILT_SIZE_GRAD1 and are updated as the reference updates those globals
+ * (an even size is made odd); a size of 0 leaves the field unchanged.
+ * field is a host (on_device = 0) or device pointer on HIP device `device`;
+ * hip_stream (a hipStream_t, NULL = the null stream) orders the work, and
+ * the call returns when it is done.  No context is needed; errors are
+ * reported by rz_last_error(NULL). */
+int rz_gauss_filt(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy, int on_device);
+
 /* Block until the context's stream is idle. */
 int rz_sync(rz_ctx *ctx);
 

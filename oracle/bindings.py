@@ -97,6 +97,9 @@ class Restatement:
         self._fwd.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, fr,
                               C.c_int, _i32, _i32, C.c_int, _i32, _i32, fr, fr, o_i]
         self._fwd.restype = None
+        self._gauss = fn("rzo_gauss_filt")
+        self._gauss.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), fr]
+        self._gauss.restype = C.c_int
 
     def _a(self, x):
         return np.ascontiguousarray(x, dtype=self.dt)
@@ -164,6 +167,14 @@ class Restatement:
         self._frame(nx, ny, A)
         return A
 
+    def gauss_filt(self, A, fsx, fsy):
+        """gauss_filt.c on a copy of A; returns (filtered, fsx, fsy as made odd)."""
+        A = self._a(A).copy()
+        ny, nx = A.shape
+        x, y = C.c_int(fsx), C.c_int(fsy)
+        self._gauss(nx, ny, C.byref(x), C.byref(y), A)
+        return A, x.value, y.value
+
     def grad_obj(self, S, src_ix, src_iy, rec_ix, rec_iy, tobs, dh, time=6.0, ttnorm=1e-10,
                  eps_adj=1e-9, tres0=None, maxit=0):
         """One evaluation; returns dict of L2, grad, last T/lam, per-shot data."""
@@ -229,6 +240,8 @@ class Reference:
         L.rzref_grad_obj.argtypes = [_f32, C.c_int, _f32, _f32, C.c_int, _i32, _i32, _f32, C.c_char_p,
                                      _f32, _f32, _f32, _f32, o_f, o_f, o_f, o_d, o_i, o_i, C.c_int]
         L.rzref_grad_obj.restype = C.c_double
+        L.rzref_gauss_filt.argtypes = [_f32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.rzref_gauss_filt.restype = None
 
     def setup(self, nx, ny, dh, time=6.0, ttnorm=1e-10, eps_adj=1e-9):
         self.nx, self.ny = nx, ny
@@ -275,6 +288,13 @@ class Reference:
         gs = np.empty((self.ny, self.nx), np.float32)
         self.lib.rzref_ass_grad(self._f(S), self._f(lam), gs, g)
         return gs, g
+
+    def gauss_filt(self, A, fsx, fsy):
+        """The reference's gauss_filt() (NX, NY from setup) on a copy of A."""
+        A = self._f(A).copy()
+        x, y = C.c_int(fsx), C.c_int(fsy)
+        self.lib.rzref_gauss_filt(A, C.byref(x), C.byref(y))
+        return A, x.value, y.value
 
     def grad_obj(self, S, src_x_m, src_z_m, rec_ix, rec_iy, tobs, tres0=None, workdir=None):
         ns, ntr = len(src_x_m), len(rec_ix)

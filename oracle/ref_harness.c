@@ -210,3 +210,22 @@ double rzref_grad_obj(const float *S, int nshots, const float *srcx, const float
     free_vector(tmod, 1, nv); free_vector(tob, 1, nv); free_vector(tr, 1, nv);
     return L2;
 }
+
+/* The reference's gauss_filt() on a flat NY x NX field (NX, NY from
+ * rzref_setup).  The sizes go in through the globals it reads and come back
+ * as it leaves them (made odd).  MYID is raised during the call so that its
+ * report of hfsx/hfsy stays off stdout. */
+void rzref_gauss_filt(float *field, int *fsx, int *fsy)
+{
+    float **m = mat_from(field);
+    FILT_SIZE_GRAD = *fsx;
+    FILT_SIZE_GRAD1 = *fsy;
+    const int myid = MYID;
+    MYID = 1;
+    gauss_filt(m);
+    MYID = myid;
+    *fsx = FILT_SIZE_GRAD;
+    *fsy = FILT_SIZE_GRAD1;
+    mat_to(m, field);
+    free_matrix(m, 1, NY, 1, NX);
+}

@@ -26,6 +26,9 @@
  *                             (tests/test_oracle_math.py)
  *   rzo_ass_grad, frame,      elementwise closed forms
  *   rzo_calc_fa_res
+ *   rzo_gauss_filt            kernel = normalised Gaussian (impulse response),
+ *                             constant fields, separable double-sum bound,
+ *                             edge replication (tests/test_oracle_math.py)
  */
 #include <math.h>
 #include <stdlib.h>
@@ -344,4 +347,55 @@ void NAME(rzo_forward)(int nx, int ny, float dh, float time, float ttnorm, int m
         NAME(rzo_calc_fa)(nx, ny, T, ntr, rx, ry, tmod_all + (size_t)s * ntr);
         if (it_f) it_f[s] = itf;
     }
+}
+
+/* ---------------------------------------------------------------------------
+ * Gaussian smoothing of a gradient or model (gauss_filt.c:10-139; precond.c
+ * calls it for SPATFILTER = 4).  *fsx, *fsy are FILT_SIZE_GRAD and
+ * FILT_SIZE_GRAD1: an even size is made odd in place (the reference does so
+ * to its globals, gauss_filt.c:27-41); a size of 0 returns 0 and leaves the
+ * field alone.  hfs = |fs|/2 and sigma = hfs/2 are integer halves; the
+ * weights exp(-ii^2/sx - jj^2/sy) are float, summed in float in the loop
+ * order x outer, y inner, and divided by the sum only over 1..fs (no
+ * normalisation for negative sizes, as the reference's loops).  The field
+ * is extended by edge replication and each output is a float sum over the
+ * window, x outer and y inner.  Returns 1 when the field was filtered.
+ * ------------------------------------------------------------------------- */
+int NAME(rzo_gauss_filt)(int nx, int ny, int *fsx, int *fsy, real *field)
+{
+    if (*fsx == 0) return 0;
+    if (!(*fsx % 2)) *fsx += *fsx > 0 ? 1 : -1;
+    if (*fsy == 0) return 0;
+    if (!(*fsy % 2)) *fsy += *fsy > 0 ? 1 : -1;
+    const int hx = abs(*fsx) / 2, hy = abs(*fsy) / 2;
+    const float sigx = hx / 2, sigy = hy / 2;
+    const float sx = 2.0 * sigx * sigx, sy = 2.0 * sigy * sigy;
+    const int wx = 2 * hx + 1, wy = 2 * hy + 1;
+    float *w = malloc((size_t)wx * wy * sizeof(float));
+    float sum = 0.0;
+    for (int ii = -hx; ii <= hx; ++ii)
+        for (int jj = -hy; jj <= hy; ++jj) {
+            float *k = &w[(size_t)(jj + hy) * wx + (ii + hx)];
+            *k = exp(-((ii * ii) / sx) - ((jj * jj) / sy));
+            sum += *k;
+        }
+    for (int i = 0; i < *fsx; ++i)
+        for (int j = 0; j < *fsy; ++j) w[(size_t)j * wx + i] /= sum;
+    real *src = malloc((size_t)nx * ny * sizeof(real));
+    memcpy(src, field, (size_t)nx * ny * sizeof(real));
+    for (int k = 0; k < ny; ++k)
+        for (int h = 0; h < nx; ++h) {
+            real conv = 0.0;
+            for (int ii = -hx; ii <= hx; ++ii) {
+                const int hh = h + ii < 0 ? 0 : (h + ii > nx - 1 ? nx - 1 : h + ii);
+                for (int jj = -hy; jj <= hy; ++jj) {
+                    const int kk = k + jj < 0 ? 0 : (k + jj > ny - 1 ? ny - 1 : k + jj);
+                    conv += src[IDX(kk, hh)] * (real)w[(size_t)(jj + hy) * wx + (ii + hx)];
+                }
+            }
+            field[IDX(k, h)] = conv;
+        }
+    free(src);
+    free(w);
+    return 1;
 }

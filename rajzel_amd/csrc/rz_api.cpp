@@ -415,6 +415,13 @@ static thread_local std::string g_create_err;
 
 const char *rz_last_error(const rz_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
+}  // extern "C"
+
+// Errors of calls without a context (rz_last_error(NULL)).
+void rz::set_global_error(const char *msg) { g_create_err = msg; }
+
+extern "C" {
+
 void rz_destroy(rz_ctx *c)
 {
     if (!c) return;

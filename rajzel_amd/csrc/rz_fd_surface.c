@@ -17,6 +17,10 @@
  *   eikonal     (fd.h:60, eikonal.c:10-76)    one shot
  *   eikonal_adj (fd.h:62, eikonal_adj.c:10-59) one shot
  *   ass_grad    (fd.h:36, ass_grad.c:11-49)   one shot
+ *   gauss_filt  (fd.h:70, gauss_filt.c:10-139)   the Gaussian smoothing of
+ *               gradient (precond.c:35-37, SPATFILTER = 4) and of the
+ *               search direction (fatt.c:179), on the GPU; linked in
+ *               place of gauss_filt.c
  *
  * sweep() and sweep_adj() (fd.h:145,147) have no callers outside
  * eikonal.c and eikonal_adj.c; their work happens inside the kernels.
@@ -36,6 +40,7 @@
 /* globals of the reference (globvar.h:12-29,38,69,96) */
 extern float DH, TIME, TTNORM, EPS_ADJ;
 extern int NX, NY, SEISMO, SNAP, MYID, INFO, NSHOT1, NSHOT2, SWS_TAPER_CIRCULAR_PER_SHOT;
+extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1;
 extern char SNAP_FILE[], DATA_DIR[];
 
 /* routines of the reference that stay in use (fd.h:52,119,143,166,170,180) */
@@ -285,4 +290,25 @@ double grad_obj(float **grad, float **S, float **TT, float **lam, float *Tmod, f
     cp_grad_frame(grad);
     precond(grad, nshots, srcpos, recpos, ntr, iter);
     return L2;
+}
+
+/* gauss_filt.c:10-139 on the GPU.  The sizes are adjusted in the globals as
+ * the reference adjusts them (an even size is made odd, 0 returns), and the
+ * reference's report of the half sizes is kept. */
+void gauss_filt(float **waveconv)
+{
+    if (FILT_SIZE_GRAD == 0) return;
+    if (!(FILT_SIZE_GRAD % 2)) FILT_SIZE_GRAD += FILT_SIZE_GRAD > 0 ? 1 : -1;
+    if (FILT_SIZE_GRAD1 == 0) return;
+    if (!(FILT_SIZE_GRAD1 % 2)) FILT_SIZE_GRAD1 += FILT_SIZE_GRAD1 > 0 ? 1 : -1;
+    if (MYID == 0) {
+        printf("\n hfsx: %d \n", abs(FILT_SIZE_GRAD) / 2);
+        printf("\n hfsy: %d \n", abs(FILT_SIZE_GRAD1) / 2);
+    }
+    const char *dev = getenv("RAJZEL_AMD_DEVICE");
+    if (rz_gauss_filt(dev ? atoi(dev) : 0, NULL, &waveconv[1][1], NX, NY, &FILT_SIZE_GRAD, &FILT_SIZE_GRAD1, 0) !=
+        RZ_OK) {
+        snprintf(g_msg, sizeof g_msg, " rajzel_amd gauss_filt: %s", rz_last_error(NULL));
+        err(g_msg);
+    }
 }

@@ -91,4 +91,7 @@ hipError_t launch_field_to_rowmajor(const ModelDev *md_host, const float *field,
 hipError_t launch_rowmajor_to_field(const ModelDev *md_host, const float *in, float *field, int family,
                                     hipStream_t st);
 
+// Message of the last failed call that has no context (rz_last_error(NULL)).
+void set_global_error(const char *msg);
+
 }  // namespace rz

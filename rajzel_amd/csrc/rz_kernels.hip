@@ -739,7 +739,10 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     }
                     const float rhs = qdh<FD>(amp[r] * lxm - apm[r] * lxp, dh, rdh)
                                     + qdh<FD>(bmp[r] * lym - bpm[r] * lyp, dh, rdh);
-                    const float lt = (lhs[r] > eps) ? rhs / lhs[r] : 0.f;          // sweep_adj.c:60-64
+                    // sweep_adj.c:60-64 without a branch: every row divides (by 1 where
+                    // lhs <= EPS_ADJ) and the rows' chains can overlap
+                    const bool pos = lhs[r] > eps;
+                    const float lt = pos ? rhs / (pos ? lhs[r] : 1.0f) : 0.f;
                     float v = mn(lc, lt);                                          // sweep_adj.c:66
                     if (infv[r]) v = 0.f;                                          // sweep_adj.c:68
                     nv[r] = keep[r] ? lc : v;

@@ -24,7 +24,7 @@ EXPORTS = (
     "rz_get_tres_carry", "rz_get_field", "rz_eikonal_shot", "rz_eikonal_adj_shot", "rz_ass_grad_shot",
     "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
-    "rz_autotune_rows",
+    "rz_autotune_rows", "rz_gauss_filt",
 )
 
 
@@ -99,6 +99,7 @@ def lib():
                 "rz_get_stream": (vp, [vp]),
                 "rz_set_graph": (i, [vp, i]),
                 "rz_autotune_rows": (i, [i, i, C.POINTER(ModelDesc), C.c_size_t, C.POINTER(i), C.POINTER(d)]),
+                "rz_gauss_filt": (i, [i, vp, vp, i, i, C.POINTER(i), C.POINTER(i), i]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -155,6 +156,24 @@ def autotune_rows(models, device=0, hbm_budget=0):
     if rc != RZ_OK:
         raise RzError(rc, lib().rz_last_error(None).decode())
     return R.value, ms.value
+
+
+def gauss_filt(field, fsx, fsy, device=0, stream=None):
+    """gauss_filt.c on a numpy (NY, NX) float32 array (in place on a copy)
+    or a torch CUDA tensor (in place).  Returns (field, fsx, fsy) with the
+    sizes made odd as the reference does."""
+    x, y = C.c_int(int(fsx)), C.c_int(int(fsy))
+    if hasattr(field, "data_ptr"):
+        ny, nx = field.shape
+        rc = lib().rz_gauss_filt(device, stream, C.c_void_p(field.data_ptr()), nx, ny, C.byref(x), C.byref(y), 1)
+        out = field
+    else:
+        out = np.ascontiguousarray(field, np.float32).copy()
+        ny, nx = out.shape
+        rc = lib().rz_gauss_filt(device, stream, out.ctypes.data_as(C.c_void_p), nx, ny, C.byref(x), C.byref(y), 0)
+    if rc != RZ_OK:
+        raise RzError(rc, lib().rz_last_error(None).decode())
+    return out, x.value, y.value
 
 
 class Context:

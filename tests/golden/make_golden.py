@@ -21,6 +21,9 @@ Cases
                   example acquisition
   c1_forward.npz  eikonal() of the 200x600 forward configuration: receiver
                   times, iterations and the full traveltime grid
+  gauss_filt.npz  gauss_filt() of a random 37x53 field and of a gradient-like
+                  48x130 field, with odd, even (made odd) and negative sizes
+                  (no normalisation), and a size of 0
 """
 import os
 import sys
@@ -110,6 +113,23 @@ def c1_forward(ref):
                         T=T, tmod=tm, iters=np.int32(it))
 
 
+GAUSS_CASES = [("a", 37, 53, 9, 5), ("b", 37, 53, 10, 6), ("c", 48, 130, 81, 41), ("d", 37, 53, -7, 9),
+               ("e", 37, 53, 0, 5)]
+
+
+def gauss_filt(ref):
+    rng = np.random.default_rng(404)
+    out = {}
+    for name, ny, nx, fx, fy in GAUSS_CASES:
+        A = rng.normal(0.0, 1e-3, (ny, nx)).astype(np.float32)
+        ref.setup(nx, ny, 10.0)
+        B, ox, oy = ref.gauss_filt(A, fx, fy)
+        out[f"{name}_in"] = A
+        out[f"{name}_out"] = B
+        out[f"{name}_sizes"] = np.array([fx, fy, ox, oy], np.int32)
+    np.savez_compressed(os.path.join(OUT, "gauss_filt.npz"), **out)
+
+
 def main():
     ref = Reference()
     sweeps(ref)
@@ -119,6 +139,7 @@ def main():
     evaluation(ref, "eval_bands", 70, 150, 3, 41, [(0, 0), (2, 69)], (2, 120, 149))
     evaluation(ref, "eval_surface", 53, 37, 3, 51, [(1, 7)], 2)
     c1_forward(ref)
+    gauss_filt(ref)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

@@ -2,6 +2,18 @@
 import numpy as np
 
 
+def bits_equal_nan(a, b):
+    """bits_equal, except that a NaN matches any NaN: NaN payloads are not
+    carried across machines (x86's default NaN has the sign bit set, the
+    GPU's does not)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return bits_equal(np.where(na, 0, a).astype(a.dtype), np.where(nb, 0, b).astype(b.dtype))
+
+
 def bits_equal(a, b):
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b)

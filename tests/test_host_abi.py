@@ -77,6 +77,23 @@ def test_autotune_of_infeasible_model_fails_before_device():
     assert "autotune" in str(e.value)
 
 
+def test_gauss_filt_argument_checks_before_device():
+    import ctypes as C
+    from rajzel_amd import native as N
+    L = N.lib()
+    x, y = C.c_int(9), C.c_int(5)
+    a = np.zeros((4, 4), np.float32)
+    p = a.ctypes.data_as(C.c_void_p)
+    assert L.rz_gauss_filt(0, None, None, 4, 4, C.byref(x), C.byref(y), 0) == N.RZ_ERR_INVALID
+    assert "null" in L.rz_last_error(None).decode()
+    assert L.rz_gauss_filt(0, None, p, 0, 4, C.byref(x), C.byref(y), 0) == N.RZ_ERR_INVALID
+    # size 0 returns at once (gauss_filt.c:27), without touching a device;
+    # an even size before it is made odd first
+    x, y = C.c_int(8), C.c_int(0)
+    assert L.rz_gauss_filt(0, None, p, 4, 4, C.byref(x), C.byref(y), 0) == N.RZ_OK
+    assert (x.value, y.value) == (9, 0)
+
+
 def test_null_context_calls_fail_cleanly():
     from rajzel_amd import native as N
     L = N.lib()

@@ -68,3 +68,12 @@ def test_muted_picks_keep_previous_residual():
         for i in np.nonzero(tobs[s] < 0)[0]:
             prev = tres[s - 1, i] if s > 0 else tres0[i]
             assert bits_equal(tres[s, i:i + 1], np.array([prev], np.float32))
+
+
+@pytest.mark.parametrize("case", ["a", "b", "c", "d", "e"])
+def test_gauss_filt(ora, case):
+    d = load("gauss_filt.npz")
+    fx, fy, ox, oy = (int(v) for v in d[f"{case}_sizes"])
+    out, x, y = ora.gauss_filt(d[f"{case}_in"], fx, fy)
+    assert (x, y) == (ox, oy)
+    assert bits_equal(out, d[f"{case}_out"]), f"{ndiff(out, d[case + '_out'])} cells differ"

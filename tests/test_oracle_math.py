@@ -212,3 +212,68 @@ def test_fp64_build_close_to_fp32(ora, ora64):
     assert np.max(np.abs(a["tmod"] - b["tmod"])) < 1e-5
     rel = np.linalg.norm(a["grad"] - b["grad"]) / np.linalg.norm(b["grad"])
     assert rel < 1e-2, rel
+
+
+def _gauss_weights(hx, hy):
+    """Normalised Gaussian of gauss_filt.c in double (independent of libm)."""
+    sx = 2.0 * (hx // 2) ** 2
+    sy = 2.0 * (hy // 2) ** 2
+    ii = np.arange(-hx, hx + 1)[None, :]
+    jj = np.arange(-hy, hy + 1)[:, None]
+    w = np.exp(-(ii * ii) / sx - (jj * jj) / sy)
+    return w / w.sum()
+
+
+def test_gauss_filt_impulse_response_is_the_kernel(ora):
+    """A unit impulse far from the edges returns the normalised kernel
+    around it (every other term of each sum is an exact zero), oriented
+    x along columns and y along rows."""
+    ny, nx, hx, hy = 61, 47, 9, 13
+    A = np.zeros((ny, nx), f32)
+    A[30, 23] = 1.0
+    out, fx, fy = ora.gauss_filt(A, 2 * hx + 1, 2 * hy + 1)
+    w = _gauss_weights(hx, hy)
+    win = out[30 - hy:30 + hy + 1, 23 - hx:23 + hx + 1]
+    np.testing.assert_allclose(win, w, rtol=2e-6, atol=1e-12)
+    assert np.count_nonzero(out) == w.size
+    # the check has teeth: swapping the two sigmas (x with y) over the same
+    # window gives different weights
+    ii = np.arange(-hx, hx + 1)[None, :]
+    jj = np.arange(-hy, hy + 1)[:, None]
+    swapped = np.exp(-(ii * ii) / (2.0 * (hy // 2) ** 2) - (jj * jj) / (2.0 * (hx // 2) ** 2))
+    assert not np.allclose(win, swapped / swapped.sum(), rtol=1e-3)
+
+
+def test_gauss_filt_constant_and_edge_replication(ora):
+    """A constant field stays constant (the weights sum to one); a field
+    that varies only along x is filtered like its edge-padded double
+    convolution, which pins the replication at the borders."""
+    ny, nx = 20, 33
+    out, _, _ = ora.gauss_filt(np.full((ny, nx), 2.5, f32), 11, 7)
+    np.testing.assert_allclose(out, 2.5, rtol=1e-6)
+    x = np.linspace(-1.0, 3.0, nx).astype(f32)
+    A = np.broadcast_to(x, (ny, nx)).astype(f32).copy()
+    hx, hy = 5, 3
+    out, _, _ = ora.gauss_filt(A, 2 * hx + 1, 2 * hy + 1)
+    w = _gauss_weights(hx, hy)
+    P = np.pad(A.astype(np.float64), ((hy, hy), (hx, hx)), mode="edge")
+    exp = np.zeros((ny, nx))
+    for j in range(2 * hy + 1):
+        for i in range(2 * hx + 1):
+            exp += w[j, i] * P[j:j + ny, i:i + nx]
+    np.testing.assert_allclose(out, exp, rtol=0, atol=2e-6)
+
+
+def test_gauss_filt_even_and_negative_sizes(ora):
+    """Even sizes grow by one away from zero; negative sizes leave the
+    kernel unnormalised (its sum is the un-normalised Gaussian sum)."""
+    A = np.ones((15, 17), f32)
+    _, fx, fy = ora.gauss_filt(A, 6, -8)
+    assert (fx, fy) == (7, -9)
+    out, _, _ = ora.gauss_filt(A, -7, 9)
+    hx, hy = 3, 4
+    raw = np.exp(-(np.arange(-hx, hx + 1)[None, :] ** 2) / (2.0 * (hx // 2) ** 2)
+                 - (np.arange(-hy, hy + 1)[:, None] ** 2) / (2.0 * (hy // 2) ** 2)).sum()
+    np.testing.assert_allclose(out, raw, rtol=1e-5)
+    same, fx, fy = ora.gauss_filt(A * 3, 0, 5)
+    assert (fx, fy) == (0, 5) and np.array_equal(same, A * 3)

@@ -101,3 +101,21 @@ def test_grad_obj_with_muted_picks_and_carry(ora, ref):
         assert bits_equal(a[k], b[k]), f"{k}: {ndiff(a[k], b[k])} values differ"
     assert np.array_equal(a["it_f"], b["it_f"]) and np.array_equal(a["it_a"], b["it_a"])
     assert a["L2"] == b["L2"]
+
+
+@pytest.mark.parametrize("shape,sizes", [((5, 5), (5, 5)), ((23, 31), (9, 5)), ((23, 31), (8, 12)),
+                                         ((30, 90), (81, 41)), ((19, 29), (-9, 7)), ((19, 29), (7, -9)),
+                                         ((12, 15), (3, 5)), ((40, 50), (0, 5)), ((40, 50), (5, 0)),
+                                         ((1, 40), (7, 7)), ((40, 1), (7, 7))])
+def test_gauss_filt(ora, ref, shape, sizes):
+    """gauss_filt.c: odd, even (made odd), negative (no normalisation), zero,
+    sizes below 5 (sigma 0: NaN weights, as the reference), one-row and
+    one-column grids, and windows wider than the grid."""
+    ny, nx = shape
+    rng = np.random.default_rng(500 + ny + nx + sizes[0])
+    A = rng.normal(0.0, 1.0, (ny, nx)).astype(np.float32)
+    ref.setup(nx, ny, 10.0)
+    r, rx, ry = ref.gauss_filt(A, *sizes)
+    o, ox, oy = ora.gauss_filt(A, *sizes)
+    assert (ox, oy) == (rx, ry)
+    assert bits_equal(o, r), f"{ndiff(o, r)} cells differ"
