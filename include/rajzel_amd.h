@@ -146,6 +146,10 @@ int rz_adjoint(rz_ctx *ctx);
 /* Results of the last rz_forward / rz_gradient / rz_adjoint. */
 int rz_get_tmod(rz_ctx *ctx, int model, float *out, int on_device);      /* nshots*ntr */
 int rz_get_tres(rz_ctx *ctx, int model, float *out, int on_device);      /* nshots*ntr */
+/* T at each receiver's slope neighbour (the row below it, or above it on the
+ * last row; calc_FA_res.c:52-69), nshots*ntr, after a forward.  With
+ * rz_get_tmod it is all of T that calc_FA_res() reads. */
+int rz_get_tnb(rz_ctx *ctx, int model, float *out, int on_device);
 int rz_get_grad(rz_ctx *ctx, int model, float *out, int on_device);      /* NY*NX      */
 /* L2 = 0.5 * sum of per-shot sums (grad_obj.c:61); l2_shot may be NULL. */
 int rz_get_misfit(rz_ctx *ctx, int model, double *L2, double *l2_shot);

@@ -65,7 +65,7 @@ def build_fd_surface(verbose: bool = False):
         return None
     obj = os.path.join(HERE, "_build", "rz_fd_surface.o")
     env = dict(os.environ, MPICH_CC=os.environ.get("MPICH_CC", "gcc"))
-    cmd = [mpicc, "-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wextra", "-Werror",
+    cmd = [mpicc, "-O2", "-fPIC", "-std=gnu99", "-ffp-contract=off", "-Wall", "-Wextra", "-Werror",
            "-I" + os.path.join(os.path.dirname(HERE), "include"),
            "-c", os.path.join(CSRC, "rz_fd_surface.c"), "-o", obj]
     if verbose:

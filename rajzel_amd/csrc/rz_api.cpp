@@ -749,6 +749,14 @@ int rz_get_tmod(rz_ctx *c, int model, float *out, int on_device)
     return copy_out(c, out, c->tmod_d + h.rec_off, (size_t)h.d.nshots * h.d.ntr * sizeof(float), on_device);
 }
 
+int rz_get_tnb(rz_ctx *c, int model, float *out, int on_device)
+{
+    TRY(check_model(c, model));
+    if (!c->have_fwd) return fail(c, RZ_ERR_STATE, "no forward has run");
+    const ModelHost &h = c->m[model];
+    return copy_out(c, out, c->tnb_d + h.rec_off, (size_t)h.d.nshots * h.d.ntr * sizeof(float), on_device);
+}
+
 int rz_get_tres(rz_ctx *c, int model, float *out, int on_device)
 {
     TRY(check_model(c, model));

@@ -17,6 +17,10 @@
  *   eikonal     (fd.h:60, eikonal.c:10-76)    one shot
  *   eikonal_adj (fd.h:62, eikonal_adj.c:10-59) one shot
  *   ass_grad    (fd.h:36, ass_grad.c:11-49)   one shot
+ *   grid_search (fd.h:74, grid_search.c:10-96)  every model of the 1-D
+ *               gradient grid and every shot of this rank in batched
+ *               forward launches; residuals, misfits and the table through
+ *               the reference's calc_FA_res / write_gridsearch
  *   gauss_filt  (fd.h:70, gauss_filt.c:10-139)   the Gaussian smoothing of
  *               gradient (precond.c:35-37, SPATFILTER = 4) and of the
  *               search direction (fatt.c:179), on the GPU; linked in
@@ -30,6 +34,7 @@
  * the run through the reference's err() (util.c:7-17), as every other
  * error of the program does.  The device is RAJZEL_AMD_DEVICE (default 0).
  */
+#include <math.h>
 #include <mpi.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -41,6 +46,7 @@
 extern float DH, TIME, TTNORM, EPS_ADJ;
 extern int NX, NY, SEISMO, SNAP, MYID, INFO, NSHOT1, NSHOT2, SWS_TAPER_CIRCULAR_PER_SHOT;
 extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1;
+extern float VP0_1, VP0_2, DVP0, GRAD0_1, GRAD0_2, DGRAD0;
 extern char SNAP_FILE[], DATA_DIR[];
 
 /* routines of the reference that stay in use (fd.h:52,119,143,166,170,180) */
@@ -50,6 +56,10 @@ void sum_grad_MPI(float **grad);
 void write_picks(float *Tmod, int ntr, int ishot);
 void writemod(char modfile[], float **array, int format);
 void err(char err_text[]);
+double calc_FA_res(float **TT, float *Tmod, float *Tobs, float *Tres, int ntr, int ishot, int **recpos);
+void calc_S(float **Vp, float **S);
+void model_gridsearch(float **Vp, float vp0, float grad0);
+void write_gridsearch(float L2, float vp0, float grad0, int count);
 
 #define RZS_PATH 74                 /* STRING_SIZE of fd.h:25 */
 
@@ -311,4 +321,188 @@ void gauss_filt(float **waveconv)
         snprintf(g_msg, sizeof g_msg, " rajzel_amd gauss_filt: %s", rz_last_error(NULL));
         err(g_msg);
     }
+}
+
+/* grid_search.c:10-96.  The models of the (vp0, grad0) grid are built in the
+ * reference's order with its model_gridsearch() and calc_S(), and their
+ * forwards run as multi-model contexts, every shot of this rank resident
+ * (one launch per chunk of models).  Each model is then finished exactly as
+ * the reference finishes it: calc_FA_res per shot in order (pick files,
+ * Tres carry and the slope scaling need T only at the receiver and at its
+ * slope neighbour, which the library returns), the float L2, the
+ * MPI_Allreduce and write_gridsearch().  With SNAP set the reference writes
+ * a traveltime snapshot per shot; the per-shot path (the shim's eikonal())
+ * is kept for that case. */
+static rz_ctx *rzs_grid_context(int nmod, int ns, float **srcpos, int **recpos, int ntr)
+{
+    int *sx = rzs_alloc(ns * sizeof(int)), *sy = rzs_alloc(ns * sizeof(int));
+    int *rx = rzs_alloc((ntr > 0 ? ntr : 1) * sizeof(int)), *ry = rzs_alloc((ntr > 0 ? ntr : 1) * sizeof(int));
+    for (int s = 0; s < ns; ++s) {
+        sx[s] = rz_grid_index(srcpos[1][NSHOT1 + s], DH);   /* eikonal.c:31-32 */
+        sy[s] = rz_grid_index(srcpos[2][NSHOT1 + s], DH);
+    }
+    for (int i = 0; i < ntr; ++i) {
+        rx[i] = recpos[1][i + 1] - 1;
+        ry[i] = recpos[2][i + 1] - 1;
+    }
+    rz_model_desc *d = rzs_alloc((size_t)nmod * sizeof(rz_model_desc));
+    for (int k = 0; k < nmod; ++k) {
+        memset(&d[k], 0, sizeof d[k]);
+        d[k].nx = NX; d[k].ny = NY; d[k].dh = DH; d[k].time = TIME; d[k].ttnorm = TTNORM;
+        d[k].eps_adj = EPS_ADJ;
+        d[k].nshots = ns; d[k].src_ix = sx; d[k].src_iy = sy;
+        d[k].ntr = ntr; d[k].rec_ix = rx; d[k].rec_iy = ry;
+    }
+    const char *dev = getenv("RAJZEL_AMD_DEVICE");
+    rz_ctx *c = NULL;
+    if (rz_create(&c, dev ? atoi(dev) : 0, nmod, d, 0, 0) != RZ_OK) {
+        snprintf(g_msg, sizeof g_msg, " rajzel_amd grid search context: %s", rz_last_error(NULL));
+        err(g_msg);
+    }
+    free(d); free(sx); free(sy); free(rx); free(ry);
+    return c;
+}
+
+/* calc_FA_res.c:10-77 for one shot from the receiver times and the times
+ * at the receivers' slope neighbours: reads the shot's pick file into
+ * Tobs, keeps Tres of muted picks (the carry), and returns the shot's
+ * l2 = sum of Tres^2 (float products, double sum).  The reference leaves
+ * its l2 uninitialised; like the GPU residual kernel this sum starts at
+ * 0, which is what the reference program computes in grad_obj (the bitwise
+ * program tests compare its misfit logs). */
+static double rzs_calc_fa_res(const float *tm, const float *tn, float *Tobs, float *Tres, int ntr, int ishot,
+                              int **recpos)
+{
+    char name[RZS_PATH];
+    double l2 = 0.0;
+    snprintf(name, sizeof name, "%s_%i.dat", DATA_DIR, ishot);
+    FILE *fp = fopen(name, "r");
+    if (fp == NULL) err(" picks_?.dat could not be opened !");
+    for (int i = 1; i <= ntr; i++) {
+        if (fscanf(fp, "%e", &Tobs[i]) != 1) { /* Tobs[i] keeps its value, as in the reference */ }
+        if (Tobs[i] >= 0.0) {
+            Tres[i] = tm[i - 1] - Tobs[i];
+            l2 += Tres[i] * Tres[i];
+            float gradT;
+            if (recpos[2][i] <= NY - 1) {
+                gradT = (tn[i - 1] - tm[i - 1]) / DH;
+                if (fabs(gradT) > 0.0) Tres[i] = Tres[i] / gradT;
+                else Tres[i] = 0.0;
+            } else if (NY == recpos[2][i]) {
+                gradT = (tm[i - 1] - tn[i - 1]) / DH;
+                if (fabs(gradT) > 0.0) Tres[i] = Tres[i] / gradT;
+                else Tres[i] = 0.0;
+            }
+        }
+    }
+    fclose(fp);
+    return l2;
+}
+
+static void rzs_grid_check(rz_ctx *c, int rc, const char *what)
+{
+    if (rc != RZ_OK) {
+        snprintf(g_msg, sizeof g_msg, " rajzel_amd grid search %s: %s", what, rz_last_error(c));
+        err(g_msg);
+    }
+}
+
+void grid_search(float **Vp, float **S, float **TT, float *Tmod, float *Tobs, float *Tres, float **srcpos,
+                 int nshots, int **recpos, int ntr)
+{
+    if (MYID == 0) {
+        printf("\n\n\n ------------------------------------------------------------------\n");
+        printf("\n\n\n                1D linear gradient grid search                     \n");
+        printf("\n\n\n ------------------------------------------------------------------\n");
+    }
+    SEISMO = 0;
+    const int nvp0 = ceil((VP0_2 - VP0_1) / DVP0);
+    const int ngrad0 = ceil((GRAD0_2 - GRAD0_1) / DGRAD0);
+    const int nobj = nvp0 * ngrad0;
+    if (MYID == 0) printf("nvp0 =  %d, ngrad0 = %d\n", nvp0, ngrad0);
+    const int ns = NSHOT2 > NSHOT1 ? NSHOT2 - NSHOT1 : 0;
+    const size_t ncell = (size_t)NX * NY;
+    const size_t nrec = (size_t)ns * (ntr > 0 ? ntr : 1);
+
+    /* the grid's parameters in the reference's order (float accumulation) */
+    float *vp0s = rzs_alloc((size_t)(nobj > 0 ? nobj : 1) * sizeof(float));
+    float *grad0s = rzs_alloc((size_t)(nobj > 0 ? nobj : 1) * sizeof(float));
+    {
+        int m = 0;
+        float vp0 = VP0_1;
+        for (int i = 1; i <= nvp0; i++) {
+            float grad0 = GRAD0_1;
+            for (int j = 1; j <= ngrad0; j++) {
+                vp0s[m] = vp0; grad0s[m] = grad0; ++m;
+                grad0 += DGRAD0;
+            }
+            vp0 += DVP0;
+        }
+    }
+    const int batched = (SNAP == 0 && ns > 0);
+    /* models per launch: every shot's four fields resident, within ~64 GiB
+     * (a field is at most (NX+64) x (NY+511) floats) */
+    int chunk = 1;
+    if (batched) {
+        const double per_model = (double)ns * 16.0 * (NX + 64.0) * (NY + 511.0);
+        const double fit = (double)(64ull << 30) / per_model;
+        chunk = fit >= 32.0 ? 32 : (fit >= 1.0 ? (int)fit : 1);
+        if (chunk > nobj) chunk = nobj > 0 ? nobj : 1;
+    }
+    float *Sall = batched ? rzs_alloc((size_t)chunk * ncell * sizeof(float)) : NULL;
+    float *tm = batched ? rzs_alloc((size_t)chunk * nrec * sizeof(float)) : NULL;
+    float *tn = batched ? rzs_alloc((size_t)chunk * nrec * sizeof(float)) : NULL;
+    rz_ctx *c = NULL;
+    int c_models = 0;
+    for (int m0 = 0; m0 < nobj; m0 += (batched ? chunk : nobj)) {
+        const int m1 = batched ? (m0 + chunk < nobj ? m0 + chunk : nobj) : nobj;
+        if (batched) {
+            if (c_models != m1 - m0) {
+                rz_destroy(c);
+                c = rzs_grid_context(m1 - m0, ns, srcpos, recpos, ntr);
+                c_models = m1 - m0;
+            }
+            for (int m = m0; m < m1; ++m) {
+                model_gridsearch(Vp, vp0s[m], grad0s[m]);
+                calc_S(Vp, S);
+                memcpy(Sall + (size_t)(m - m0) * ncell, &S[1][1], ncell * sizeof(float));
+            }
+            for (int k = 0; k < m1 - m0; ++k)
+                rzs_grid_check(c, rz_set_slowness(c, k, Sall + (size_t)k * ncell, 0), "slowness");
+            rzs_grid_check(c, rz_forward(c), "forward");
+            if (ntr > 0)
+                for (int k = 0; k < m1 - m0; ++k) {
+                    rzs_grid_check(c, rz_get_tmod(c, k, tm + (size_t)k * nrec, 0), "receiver times");
+                    rzs_grid_check(c, rz_get_tnb(c, k, tn + (size_t)k * nrec, 0), "receiver times");
+                }
+        }
+        for (int m = m0; m < m1; ++m) {
+            if (MYID == 0) printf("Model %d of %d \n", m + 1, nobj);
+            float L2 = 0.0;
+            if (!batched) {
+                model_gridsearch(Vp, vp0s[m], grad0s[m]);
+                calc_S(Vp, S);
+                for (int ishot = NSHOT1; ishot < NSHOT2; ishot++) {
+                    eikonal(S, TT, Tmod, srcpos, ishot, nshots, recpos, ntr);
+                    L2 += calc_FA_res(TT, Tmod, Tobs, Tres, ntr, ishot, recpos);
+                }
+            } else {
+                for (int s = 0; s < ns; ++s) {
+                    const float *a = tm + (size_t)(m - m0) * nrec + (size_t)s * ntr;
+                    const float *b = tn + (size_t)(m - m0) * nrec + (size_t)s * ntr;
+                    for (int i = 1; i <= ntr; ++i) Tmod[i] = a[i - 1];      /* calc_FA.c:15-17 */
+                    L2 += rzs_calc_fa_res(a, b, Tobs, Tres, ntr, NSHOT1 + s, recpos);
+                }
+            }
+            float L2sum = 0.0;
+            MPI_Barrier(MPI_COMM_WORLD);
+            MPI_Allreduce(&L2, &L2sum, 1, MPI_FLOAT, MPI_SUM, MPI_COMM_WORLD);
+            L2 = 0.5 * L2sum;
+            if (MYID == 0) write_gridsearch(L2, vp0s[m], grad0s[m], m + 1);
+        }
+    }
+    /* the reference leaves the last model's last shot in TT */
+    if (batched && nobj > 0) rzs_grid_check(c, rz_get_field(c, c_models - 1, ns - 1, 0, &TT[1][1], 0), "traveltimes");
+    rz_destroy(c);
+    free(Sall); free(tm); free(tn); free(vp0s); free(grad0s);
 }

@@ -24,7 +24,7 @@ EXPORTS = (
     "rz_get_tres_carry", "rz_get_field", "rz_eikonal_shot", "rz_eikonal_adj_shot", "rz_ass_grad_shot",
     "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
-    "rz_autotune_rows", "rz_gauss_filt",
+    "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb",
 )
 
 
@@ -100,6 +100,7 @@ def lib():
                 "rz_set_graph": (i, [vp, i]),
                 "rz_autotune_rows": (i, [i, i, C.POINTER(ModelDesc), C.c_size_t, C.POINTER(i), C.POINTER(d)]),
                 "rz_gauss_filt": (i, [i, vp, vp, i, i, C.POINTER(i), C.POINTER(i), i]),
+                "rz_get_tnb": (i, [vp, i, vp, i]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -270,6 +271,13 @@ class Context:
         self._chk(lib().rz_sync(self.h))
 
     # -- outputs -------------------------------------------------------------
+    def tnb(self, model=0):
+        """T at each receiver's slope neighbour row, (nshots, ntr)."""
+        ny, nx, ns, ntr = self.shapes[model]
+        out = np.empty((ns, ntr), np.float32)
+        self._chk(lib().rz_get_tnb(self.h, model, _ptr(out), 0))
+        return out
+
     def tmod(self, model=0, device_ptr=None):
         ny, nx, ns, ntr = self.shapes[model]
         if device_ptr is not None:

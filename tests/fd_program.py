@@ -101,3 +101,45 @@ def start_model(sv):
     """1-D gradient start model 1500 + 0.5 z."""
     z = (np.arange(sv.ny, dtype=np.float32) + 1) * np.float32(sv.dh)
     return np.repeat((1500.0 + 0.5 * z)[:, None], sv.nx, axis=1).astype(np.float32)
+
+
+def time_grid_search(nx=600, ny=200, dh=10.0, nshots=40, seed=31):
+    """Wall time of the grid-search mode (INVMAT=2, grid_search.c) of the
+    reference program and of the MI355X program on identical workspaces:
+    a 200 x 600 grid, 40 shots, 600 receivers on row 2, a 4 x 4 grid of
+    (vp0, grad0) models.  Returns a dict with both times and whether the
+    grid-search tables are byte-identical."""
+    import time
+    from oracle.bindings import Restatement
+    sv = W.small_case(nx=nx, ny=ny, dh=dh, nshots=nshots, seed=seed, rec_row=2)
+    S = (1.0 / sv.vp_true.astype(np.float64)).astype(np.float32)
+    tm, _, _ = Restatement().forward(S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.dh)
+    obs = F.roundtrip_picks(tm + np.random.default_rng(3).normal(0, 1e-3, tm.shape))
+    grid = dict(VP0_1=1400.0, VP0_2=1600.0, DVP0=50.0, GRAD0_1=0.2, GRAD0_2=1.0, DGRAD0=0.2)
+    res = {"grid": f"{ny}x{nx}", "shots": nshots, "receivers": len(sv.rec_x_m), "models": 16}
+    spaces = []
+    try:
+        for key, lib in (("cpu_reference_s", CPU_LIB), ("gpu_s", GPU_LIB)):
+            ws = make_workspace(sv, start_model(sv), invmat=2, picks=obs, **grid)
+            spaces.append(ws)
+            t0 = time.perf_counter()
+            r = run(ws, lib, timeout=600)
+            res[key] = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        a, b = (outputs(ws)["gs.dat"] for ws in spaces)
+        res["tables_identical"] = a == b
+        res["speedup"] = res["cpu_reference_s"] / res["gpu_s"]
+    finally:
+        for ws in spaces:
+            remove(ws)
+    return res
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    if sys.argv[1:] == ["--time-grid-search"]:
+        print(json.dumps(time_grid_search()), flush=True)
+    else:
+        sys.exit("usage: python -m tests.fd_program --time-grid-search")

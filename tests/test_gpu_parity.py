@@ -34,13 +34,17 @@ def test_forward_small_bitwise(ora, R):
         ctx.set_slowness(0, sv.S)
         ctx.forward()
         tm = ctx.tmod(0)
+        tn = ctx.tnb(0)
         itf, _ = ctx.iterations(0)
+        ry = np.asarray(sv.rec_iy)
+        kn = np.where(ry <= sv.ny - 2, ry + 1, ry - 1)   # calc_FA_res.c:52-69 slope neighbour
         for s in range(sv.nshots):
             T_o, it_o = ora.eikonal(sv.S, sv.src_ix[s], sv.src_iy[s], sv.dh)
             T_g = ctx.field(0, s, "T")
             assert it_o == itf[s]
             assert bits_equal(T_g, T_o), f"shot {s}: {ndiff(T_g, T_o)} cells differ"
             assert bits_equal(tm[s], T_o[sv.rec_iy, sv.rec_ix])
+            assert bits_equal(tn[s], T_o[kn, sv.rec_ix])
 
 
 @pytest.mark.parametrize("R,shape", [(1, (150, 90)), (2, (300, 70)), (1, (64, 40)), (1, (65, 41)),
