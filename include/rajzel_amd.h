@@ -205,6 +205,19 @@ int rz_set_graph(rz_ctx *ctx, int on);
  * reported by rz_last_error(NULL). */
 int rz_gauss_filt(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy, int on_device);
 
+/* Split shots (on by default; RAJZEL_AMD_SPLIT=0 in the environment turns
+ * the default off): a launch with at most one shot per two CUs runs each shot's
+ * two paired sweeps in two workgroups on two CUs of one XCD, coupled
+ * through L2, instead of one workgroup on one CU.  If a launch finds the two
+ * workgroups of a shot on different XCDs, or a wait for the partner times
+ * out, the context repeats the work unsplit and stays unsplit; failure then
+ * reads 1 or 2.  Results are identical either way.  on = 2 also marks the
+ * next split launch as failed before it starts (a test of that fallback).
+ * No counterpart in the reference (its sweeps run one after another on one
+ * core). */
+int rz_set_split(rz_ctx *ctx, int on);
+int rz_get_split(const rz_ctx *ctx, int *active, int *failure);
+
 /* Block until the context's stream is idle. */
 int rz_sync(rz_ctx *ctx);
 

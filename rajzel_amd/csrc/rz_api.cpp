@@ -59,6 +59,12 @@ struct rz_ctx {
     float *carry0_d = nullptr;          // initial residual carry (rz_set_tres_carry)
     // whole evaluations recorded as HIP graphs (rz_set_graph): [0] forward, [1] gradient
     bool graph = false;
+    // split shots (rz_set_split): two workgroups per shot while a launch has
+    // at most `ncu` shots; off for good once a launch reports that it cannot
+    bool split = true;
+    int ncu = 0;
+    int split_fail = 0;                 // 1: workgroups of a shot on different XCDs, 2: a wait timed out
+    SplitSync ss{nullptr, nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     long long s_epoch = 0;              // bumped by every rz_set_slowness
     long long fields_epoch = -1;        // slowness epoch of resident forward fields of all shots
@@ -218,14 +224,26 @@ int ready(rz_ctx *c, bool need_picks)
     return RZ_OK;
 }
 
+// A launch with split shots reported that it could not run split.
+constexpr int kSplitRetry = -100;
+
 int check_convergence(rz_ctx *c, bool adj)
 {
     std::vector<int> itf(c->total_shots), ita(c->total_shots);
     if (c->total_shots == 0) return RZ_OK;
+    int serr = 0;
+    HIPCHK(c, hipMemcpyAsync(&serr, c->ss.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(itf.data(), c->it_f_d, itf.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if (adj)
         HIPCHK(c, hipMemcpyAsync(ita.data(), c->it_a_d, ita.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (serr != 0) {
+        c->split = false;
+        c->split_fail = serr;
+        HIPCHK(c, hipMemsetAsync(c->ss.err, 0, sizeof(int), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return kSplitRetry;
+    }
     for (int i = 0; i < c->total_shots; ++i)
         if (itf[i] < 0 || (adj && ita[i] < 0))
             return fail(c, RZ_ERR_NOT_CONVERGED, "shot %d did not converge within max_iter", i);
@@ -269,6 +287,14 @@ int collect(rz_ctx *c, const std::vector<PhaseMarks> &marks, bool grad)
     return RZ_OK;
 }
 
+void drop_graphs(rz_ctx *c);
+
+// Split shots (two workgroups each) while every workgroup can have a CU of
+// its own: at most one shot per two CUs.  With more shots the workgroups
+// would share CUs (two fit one), which measured slower than one workgroup
+// per shot.
+bool use_split(const rz_ctx *c, int nshots) { return c->split && nshots > 0 && 2 * nshots <= c->ncu; }
+
 // Enqueue the launches of one batch of shots [b0, b1) on the work stream:
 // forward (unless its fields are reused), and with grad the residuals, the
 // adjoint and the gradient.  pm (optional) receives the phase events.
@@ -279,9 +305,15 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
     PhaseMarks dummy;
     PhaseMarks &m = pm ? *pm : dummy;
     TRY(mark(c, &m.e[0]));
-    if (!reuse)
-        HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
-                                 c->shot_base_d, so, c->stream));
+    const bool split = use_split(c, b1 - b0);
+    if (!reuse) {
+        if (split)
+            HIPCHK(c, launch_forward_split(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+                                           c->shot_base_d, so, c->ss, c->stream));
+        else
+            HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+                                     c->shot_base_d, so, c->stream));
+    }
     TRY(mark(c, &m.e[1]));
     if (!grad) return RZ_OK;
     for (size_t mi = 0; mi < c->m.size(); ++mi) {
@@ -294,8 +326,12 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
                                   c->l2_shot_d + h.shot_base, c->l2_sum_d + mi, c->stream));
     }
     TRY(mark(c, &m.e[2]));
-    HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d, c->shot_base_d,
-                             c->tres_d, so, adj_variant(c), c->stream));
+    if (split)
+        HIPCHK(c, launch_adjoint_split(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+                                       c->shot_base_d, c->tres_d, so, adj_variant(c), c->ss, c->stream));
+    else
+        HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+                                 c->shot_base_d, c->tres_d, so, adj_variant(c), c->stream));
     TRY(mark(c, &m.e[3]));
     for (size_t mi = 0; mi < c->m.size(); ++mi) {
         const ModelHost &h = c->m[mi];
@@ -382,7 +418,13 @@ int run(rz_ctx *c, bool grad, int cap, bool direct = false)
     // all shots' forward fields stay in their slots only without batching
     if (c->nslots >= c->total_shots && !(grad && cap < c->nslots)) c->fields_epoch = c->s_epoch;
     else if (!reuse) c->fields_epoch = -1;
-    return check_convergence(c, grad);
+    const int rc = check_convergence(c, grad);
+    if (rc == kSplitRetry) {   // split shots cannot run here: repeat everything unsplit
+        drop_graphs(c);
+        c->fields_epoch = -1;
+        return run(c, grad, cap, direct);
+    }
+    return rc;
 }
 
 void drop_graphs(rz_ctx *c)
@@ -593,6 +635,16 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     if ((rc = dalloc(c, &c->TA, sf)) || (rc = dalloc(c, &c->TB, sf)) || (rc = dalloc(c, &c->LA, sf)) ||
         (rc = dalloc(c, &c->LB, sf)))
         return bail(rc);
+    {
+        const size_t nsh = (size_t)std::max(1, shots);
+        if ((rc = dalloc(c, &c->ss.cnt, nsh * kSyncInts)) || (rc = dalloc(c, &c->ss.part, nsh * 2 * kMaxBands)) ||
+            (rc = dalloc(c, &c->ss.err, 1)))
+            return bail(rc);
+        if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+            c->ncu = 0;
+        const char *env = getenv("RAJZEL_AMD_SPLIT");
+        c->split = !(env && env[0] == '0');
+    }
 
     // tables
     std::vector<ModelDev> mdh;
@@ -626,7 +678,8 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         hipMemcpy(c->shot_base0_d, sb0.data(), nm * sizeof(int), hipMemcpyHostToDevice) ||
         hipMemcpy(c->s_slot0, &zero, sizeof(int), hipMemcpyHostToDevice) ||
         hipMemcpy(c->s_shot, &s1, sizeof(ShotDev), hipMemcpyHostToDevice) ||
-        hipMemset(c->carry0_d, 0, (size_t)std::max(c->total_tr, 1) * sizeof(float)) || hipDeviceSynchronize())
+        hipMemset(c->carry0_d, 0, (size_t)std::max(c->total_tr, 1) * sizeof(float)) ||
+        hipMemset(c->ss.err, 0, sizeof(int)) || hipDeviceSynchronize())
         return bail(fail(c, RZ_ERR_HIP, "table upload failed"));
     *out = c;
     return RZ_OK;
@@ -948,6 +1001,28 @@ int rz_get_profile(const rz_ctx *c, double *ms, long long *launches)
 }
 
 int rz_rows_per_lane(const rz_ctx *c) { return c ? c->R : 0; }
+
+int rz_set_split(rz_ctx *c, int on)
+{
+    if (!c || on < 0 || on > 2) return RZ_ERR_INVALID;
+    if (c->split != (on != 0)) drop_graphs(c);
+    c->split = on != 0;
+    if (on) c->split_fail = 0;
+    if (on == 2) {   // the next split launch finds its failure flag already raised
+        const int two = 2;
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipMemcpy(c->ss.err, &two, sizeof(int), hipMemcpyHostToDevice));
+    }
+    return RZ_OK;
+}
+
+int rz_get_split(const rz_ctx *c, int *active, int *failure)
+{
+    if (!c) return RZ_ERR_INVALID;
+    if (active) *active = c->split ? 1 : 0;
+    if (failure) *failure = c->split_fail;
+    return RZ_OK;
+}
 
 int rz_autotune_rows(int device, int nmodels, const rz_model_desc *models, size_t hbm_budget, int *rows_per_lane,
                      double *ms)

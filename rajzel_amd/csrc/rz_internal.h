@@ -71,6 +71,17 @@ struct ShotOut {
     int *it_f, *it_a;
 };
 
+// Coupling of the two workgroups of split shots (rz_kernels.hip): per shot
+// kSyncInts ints of counters and 2*kMaxBands doubles of partial sums; err
+// becomes nonzero when a launch cannot run split (1: the two workgroups of
+// a shot were on different XCDs, 2: a wait timed out).
+constexpr int kSyncInts = 64;
+struct SplitSync {
+    int *cnt;
+    double *part;
+    int *err;
+};
+
 // Launchers (rz_kernels.hip).  All return hipError_t of the launch.
 hipError_t launch_prepare_slowness(const ModelDev *md_host, int model, const ModelDev *md_dev, float *SA,
                                    float *SB, int *err_flag, hipStream_t st);
@@ -79,6 +90,12 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
 hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
                           const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
                           int variant, hipStream_t st);   // variant bit 0: fast DH division, bit 1: INF guard
+hipError_t launch_forward_split(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots,
+                                Slots slots, const long long *rec_off, const int *shot_base, ShotOut out,
+                                SplitSync ss, hipStream_t st);
+hipError_t launch_adjoint_split(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots,
+                                Slots slots, const long long *rec_off, const int *shot_base, const float *tres,
+                                ShotOut out, int variant, SplitSync ss, hipStream_t st);
 hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s1, long long rec_off,
                            const float *tmod, const float *tnb, const float *tobs, const float *direct,
                            float *carry, float *tres, double *l2_shot, double *l2_sum, hipStream_t st);

@@ -350,6 +350,127 @@ __device__ __forceinline__ void idle_epochs(int nep)
 }
 
 // ---------------------------------------------------------------------------
+// How a sweep keeps pace with the other sweep of its pair.
+//   kPaceLock   both sweeps in one workgroup, in lockstep: `pre` idle epochs
+//               before the sweep and `post` after it, epoch_sync() between
+//               epochs (the trailing sweep has pre = D, the leading post = D)
+//   kPaceLead   split shot, leading sweep in a workgroup of its own: every
+//               eighth epoch it drains its stores and publishes its epoch in
+//               `lead` (L2, same XCD as the partner)
+//   kPaceTrail  split shot, trailing sweep: before each block of eight
+//               epochs it waits until the leading sweep has published the
+//               epoch that the lockstep delay D would have guaranteed
+//               (epoch + D - 9 for every load issued), then drops its L1.
+// `base` makes the published values grow over a whole solve.
+// ---------------------------------------------------------------------------
+enum PaceMode { kPaceLock = 0, kPaceLead = 1, kPaceTrail = 2 };
+constexpr int kPhaseStride = 1 << 20;      // epochs per pair phase are fewer
+constexpr int kPhaseAll = kPhaseStride - 1;
+constexpr unsigned kSpinLimit = 1u << 22;  // polls (about 1 s) before giving up
+
+struct Pace {
+    int pre, post;
+    int *lead;      // leading sweep's published epoch (split modes)
+    int *err;       // set when a wait gives up or the partner is elsewhere
+    int base, D;
+};
+
+__device__ __forceinline__ Pace lockstep(int pre, int post)
+{
+    Pace p;
+    p.pre = pre; p.post = post; p.lead = nullptr; p.err = nullptr; p.base = 0; p.D = 0;
+    return p;
+}
+
+// all of this wave's vector memory operations complete (vmcnt(0))
+__device__ __forceinline__ void drain_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ __forceinline__ void publish(int *p, int v)
+{
+    if (threadIdx.x == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread 0 polls *p >= target (bounded: on timeout, or when another wait of
+// the launch gave up, it sets / sees *err); then the workgroup meets and
+// drops its L1 so that later loads see what the partner stored.  Returns
+// false once the launch is failing (every workgroup then runs to its end
+// without waiting, and the host repeats the work unsplit).
+__device__ bool pace_wait(int *p, int target, int *err)
+{
+    __shared__ int ok_s;
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        for (unsigned n = 0;; ++n) {
+            if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
+            if (n >= kSpinLimit) {
+                __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+        ok_s = ok;
+    }
+    __syncthreads();
+    const bool ok = ok_s;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+    return ok;
+}
+
+template <int MODE>
+__device__ __forceinline__ void pace_begin(const Pace &pc)
+{
+    if constexpr (MODE == kPaceLock) {
+        for (int i = 0; i < pc.pre; ++i) epoch_sync(i);
+    } else if constexpr (MODE == kPaceTrail) {
+        (void)pace_wait(pc.lead, pc.base + pc.D - 2, pc.err);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void pace_epoch(const Pace &pc, int e)
+{
+    if constexpr (MODE == kPaceLock) {
+        epoch_sync(pc.pre + e);
+    } else if constexpr (MODE == kPaceLead) {
+        if ((e & 7) == 7) {
+            drain_vmem();
+            epoch_barrier();
+            publish(pc.lead, pc.base + e);
+        } else {
+            epoch_barrier();
+        }
+    } else {
+        epoch_barrier();
+        if ((e & 7) == 7) (void)pace_wait(pc.lead, pc.base + e + pc.D - 1, pc.err);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void pace_end(const Pace &pc, int nep)
+{
+    if constexpr (MODE == kPaceLock) {
+        for (int i = 0; i < pc.post; ++i) epoch_sync(pc.pre + nep + i);
+    } else if constexpr (MODE == kPaceLead) {
+        drain_vmem();
+        __syncthreads();
+        publish(pc.lead, pc.base + kPhaseAll);
+    }
+    __syncthreads();
+}
+
+// The pacing of a sweep for a wave without a band in it.
+template <int MODE>
+__device__ __forceinline__ void pace_idle(const Pace &pc, int nep)
+{
+    pace_begin<MODE>(pc);
+    for (int e = 0; e < nep; ++e) pace_epoch<MODE>(pc, e);
+    pace_end<MODE>(pc, nep);
+}
+
+// ---------------------------------------------------------------------------
 // Forward sweep (sweep.c:10-104) in direction (SX, SY) over one shot.
 //   Tin    field written by the previous sweep, in this sweep's family
 //          layout (A if SX==SY, else B)
@@ -371,10 +492,10 @@ __device__ __forceinline__ void idle_epochs(int nep)
 // dummy LDS word), so the compiler can wait on each prefetch with a partial
 // vmcnt instead of draining the queue.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool INIT>
+template <int R, int SX, int SY, bool INIT, int PACE = kPaceLock>
 __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const float *Tin,
-                            const float *Sf, float *Tout, float *stage, int src_k, int src_c, float tinit, int pre,
-                            int post)
+                            const float *Sf, float *Tout, float *stage, int src_k, int src_c, float tinit,
+                            const Pace &pc)
 {
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
@@ -384,10 +505,10 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
     const int gend = sweep_steps(nx, nb, Stage<R>::HOLD);
     const int nep = gend / kPF;
     if (!w.active) {
-        idle_epochs(pre + nep + post);
+        pace_idle<PACE>(pc, nep);
         return 0.0;
     }
-    for (int i = 0; i < pre; ++i) epoch_sync(i);
+    pace_begin<PACE>(pc);
     const int lag = SY > 0 ? l : 63 - l;
     const int off = (SY > 0 ? beta : nb - 1 - beta) * kBandOff;
     const long long boff = (long long)beta * w.bstride;
@@ -512,10 +633,9 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
             *rw = SY > 0 ? nv[R - 1] : nv[0];
             RZ_FWD_LOAD(u, tau + kPF);
         }
-        epoch_sync(pre + g0 / kPF);
+        pace_epoch<PACE>(pc, g0 / kPF);
     }
-    for (int i = 0; i < post; ++i) epoch_sync(pre + nep + i);
-    __syncthreads();
+    pace_end<PACE>(pc, nep);
 #undef RZ_FWD_LOAD
     return acc;
 }
@@ -565,9 +685,9 @@ __device__ __forceinline__ float neg_part(float x)
 // from T in every sweep: recomputing them costs less than the HBM traffic of
 // storing them.  The memory operations per step are fixed, as in fwd_sweep.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF>
+template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF, int PACE = kPaceLock>
 __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
-                            const float *Lin, float *Lout, float *stage, int pre, int post)
+                            const float *Lin, float *Lout, float *stage, const Pace &pc)
 {
     constexpr bool FA = (SX == SY);
     constexpr bool LA_ = FA;   // family of Lin's layout
@@ -578,10 +698,10 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     const int gend = sweep_steps(nx, nb, Stage<R>::HOLD);   // the same for both kinds of output
     const int nep = gend / kPF;
     if (!w.active) {
-        idle_epochs(pre + nep + post);
+        pace_idle<PACE>(pc, nep);
         return 0.0;
     }
-    for (int i = 0; i < pre; ++i) epoch_sync(i);
+    pace_begin<PACE>(pc);
     const int lag = SY > 0 ? l : 63 - l;
     const int off = (SY > 0 ? beta : nb - 1 - beta) * kBandOff;
     const long long boff = (long long)beta * w.bstride;
@@ -772,10 +892,9 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
             *rw = SY > 0 ? nv[R - 1] : nv[0];
             RZ_ADJ_LOAD(u, tau + kPF);
         }
-        epoch_sync(pre + g0 / kPF);
+        pace_epoch<PACE>(pc, g0 / kPF);
     }
-    for (int i = 0; i < post; ++i) epoch_sync(pre + nep + i);
-    __syncthreads();
+    pace_end<PACE>(pc, nep);
 #undef RZ_ADJ_LOAD
     return acc;
 }
@@ -863,7 +982,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         if (it == 0) {
             if (w.group == 0)
                 acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
-                                                 tinit, 0, 0);
+                                                 tinit, lockstep(0, 0));
             else
                 idle_epochs(nep);
         } else {
@@ -871,17 +990,17 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         }
         if (w.group == 0)
             acc += fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k, sh.src_c,
-                                               tinit, 0, D);
+                                               tinit, lockstep(0, D));
         else
             acc += fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
-                                               tinit, D, 0);
+                                               tinit, lockstep(D, 0));
         if (w.group == 0) {
             acc += fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k, sh.src_c,
-                                               tinit, 0, D);
+                                               tinit, lockstep(0, D));
             acc_next = 0.0;
         } else {
             acc_next = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
-                                                   tinit, D, 0);
+                                                   tinit, lockstep(D, 0));
         }
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;                               // norm1.c:27
@@ -970,17 +1089,286 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
         // 3+4 run as pairs (same x direction).
         double acc;
         if (w.group == 0) {
-            acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB, stage, 0, D);
-            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA, stage, 0, D);
+            acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB, stage, lockstep(0, D));
+            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA, stage, lockstep(0, D));
         } else {
-            acc = adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, stage, D, 0);
-            acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, stage, D, 0);
+            acc = adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, stage, lockstep(D, 0));
+            acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, stage, lockstep(D, 0));
         }
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
         ++it;
     }
     if (threadIdx.x == 0) out.it_a[shot_base[sh.model] + sh.shot] = it;
+}
+
+// ---------------------------------------------------------------------------
+// Split shots: the two sweeps of a pair run in two workgroups, each on a CU
+// of its own, instead of two wave groups of one workgroup.  A launch has
+// 16 workgroups per 8 shots; workgroup b serves shot (b/16)*8 + b%8 as the
+// leading (role 0, (b/8) even) or trailing (role 1) group.  Workgroups are
+// dealt to the XCDs round-robin by index, so the two of a shot (b and b+8)
+// share an XCD and its L2, through which they exchange fields, progress
+// counters and partial sums; each checks this at its start (XCC_ID), and
+// the launch reports failure if it does not hold (the host then repeats
+// the work with the one-workgroup kernels).  Per shot: kSyncInts ints of
+// counters (one 64-byte line each) and 16 doubles of partial sums.
+// ---------------------------------------------------------------------------
+constexpr int kSyncLead = 0, kSyncDone0 = 16, kSyncDone1 = 32, kSyncXcc = 48;
+
+__device__ __forceinline__ WaveCtx make_wave_split(const ModelDev &md, float *ring, int role)
+{
+    WaveCtx w;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    w.nx = md.nx; w.ny = md.ny; w.nb = md.nb;
+    w.group = role;
+    w.beta = wave;
+    w.lane = threadIdx.x & 63;
+    w.active = w.beta < md.nb;
+    w.bstride = md.bstride;
+    w.ring = ring;
+    return w;
+}
+
+// Both workgroups of a shot have started and run on the same XCD.
+__device__ bool pair_handshake(int *cnt, int role, int *err)
+{
+    if (threadIdx.x == 0) {
+        const int xcc = __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15;   // HW_REG_XCC_ID[3:0]
+        __hip_atomic_store(cnt + kSyncXcc + role, xcc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!pace_wait(cnt + kSyncXcc + (1 - role), 1, err)) return false;
+    __shared__ int same_s;
+    if (threadIdx.x == 0) {
+        const int a = __hip_atomic_load(cnt + kSyncXcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int b = __hip_atomic_load(cnt + kSyncXcc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        same_s = (a == b);
+        if (a != b) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return same_s != 0;
+}
+
+// End of a pair phase: each workgroup drains its stores, says it is done
+// and waits for the other (their next sweeps read what both wrote).
+__device__ bool rendezvous(int *cnt, int role, int phase, int *err)
+{
+    drain_vmem();
+    __syncthreads();
+    publish(cnt + (role ? kSyncDone1 : kSyncDone0), phase + 1);
+    return pace_wait(cnt + (role ? kSyncDone0 : kSyncDone1), phase + 1, err);
+}
+
+// block_sum() over the waves of both workgroups, in the order of the
+// one-workgroup kernel (group 0's waves, then group 1's), with the
+// rendezvous that ends the phase.
+__device__ double pair_sum(double v, double *red, int nwaves, double *part, int *cnt, int role, int phase,
+                           int *err, bool &ok)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nwaves; ++i) part[role * kMaxBands + i] = red[i];
+    ok = rendezvous(cnt, role, phase, err);
+    double t = 0.0;
+    for (int i = 0; i < nwaves; ++i) t += part[i];
+    for (int i = 0; i < nwaves; ++i) t += part[kMaxBands + i];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ bool split_role(int nshots, int &shot, int &role)
+{
+    const int b = blockIdx.x;
+    role = (b >> 3) & 1;
+    shot = (b >> 4) * 8 + (b & 7);
+    return shot < nshots;
+}
+
+template <int R>
+__global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) void k_forward_split(
+    const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, int nshots, Slots slots,
+    const long long *__restrict__ rec_off, const int *__restrict__ shot_base, ShotOut out, SplitSync ss)
+{
+    __shared__ float ring[kMaxBands * kRingQ + kWave];
+    __shared__ double red[kMaxBands];
+    __shared__ float stage_all[max_bands_for(R) * Stage<R>::WAVE_FLOATS];
+    int s, role;
+    if (!split_role(nshots, s, role)) return;
+    const ShotDev sh = shots[s];
+    const ModelDev md = mds[sh.model];
+    const int nwaves = blockDim.x >> 6;
+    int *cnt = ss.cnt + (long long)s * kSyncInts;
+    double *part = ss.part + (long long)s * 2 * kMaxBands;
+    if (!pair_handshake(cnt, role, ss.err)) return;
+    const WaveCtx w = make_wave_split(md, ring, role);
+    float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
+    const int nep = sweep_steps(md.nx, md.nb, Stage<R>::HOLD) / kPF;
+    Pace pc;
+    pc.pre = pc.post = 0; pc.lead = cnt + kSyncLead; pc.err = ss.err; pc.base = 0;
+    pc.D = pair_delay(md.nb, Stage<R>::HOLD);
+    const long long so = (long long)sh.slot * slots.slot_stride;
+    float *TA = slots.TA + so, *TB = slots.TB + so;
+    const float tinit = (float)(2.0 * (double)md.time);          // zero_travel.c:22
+    float lnorm = (float)(10.0 * (double)md.time);                // eikonal.c:39
+    const float denom = (float)(md.nx * md.ny);
+    // The sweeps and pairs of k_forward: role 0 runs sweep 1 of the first
+    // iteration alone, then sweeps 2 and 4; role 1 runs sweeps 3 and the
+    // next iteration's sweep 1.
+    int it = 0, phase = 0;
+    bool ok = true;
+    double acc_next = 0.0;
+    while (lnorm >= md.ttnorm) {                                  // eikonal.c:44
+        if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
+        double acc = 0.0;
+        if (it == 0) {
+            if (role == 0) {
+                pc.base = phase * kPhaseStride;
+                acc = fwd_sweep<R, +1, +1, true, kPaceLead>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k,
+                                                            sh.src_c, tinit, pc);
+            }
+            ok = rendezvous(cnt, role, phase++, ss.err);
+            if (!ok) break;
+        } else {
+            acc = acc_next;
+        }
+        pc.base = phase * kPhaseStride;
+        if (role == 0)
+            acc += fwd_sweep<R, -1, +1, false, kPaceLead>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k,
+                                                          sh.src_c, tinit, pc);
+        else
+            acc += fwd_sweep<R, -1, -1, false, kPaceTrail>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k,
+                                                           sh.src_c, tinit, pc);
+        ok = rendezvous(cnt, role, phase++, ss.err);
+        if (!ok) break;
+        pc.base = phase * kPhaseStride;
+        if (role == 0) {
+            acc += fwd_sweep<R, +1, -1, false, kPaceLead>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k,
+                                                          sh.src_c, tinit, pc);
+            acc_next = 0.0;
+        } else {
+            acc_next = fwd_sweep<R, +1, +1, false, kPaceTrail>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage,
+                                                               sh.src_k, sh.src_c, tinit, pc);
+        }
+        const double tot = pair_sum(acc, red, nwaves, part, cnt, role, phase++, ss.err, ok);
+        if (!ok) break;
+        lnorm = (float)tot / denom;                               // norm1.c:27
+        ++it;
+    }
+    if (role != 0) return;
+    const int nfield = md.nb * (int)md.bstride;
+    if (it != 0) {  // the last sweep wrote layout A; the adjoint also needs B
+        copy_a_to_b<R>(w, TA, TB);
+        __syncthreads();
+    } else {        // loop never entered: the field is the initial one in both layouts
+        for (int p = threadIdx.x; p < nfield; p += blockDim.x) {
+            const int b = p / (int)md.bstride, q = p - b * (int)md.bstride;
+            const int key = q / (kWave * R), wv = q - key * (kWave * R), l = wv / R, r = wv - l * R;
+            const int k = b * kWave * R + l * R + r;
+            TA[p] = (k == sh.src_k && key - l == sh.src_c) ? 0.f : tinit;
+            TB[p] = (k == sh.src_k && key + l - 63 == sh.src_c) ? 0.f : tinit;
+        }
+        __syncthreads();
+    }
+    const long long ob = rec_off[sh.model] + (long long)sh.shot * md.ntr;
+    for (int i = threadIdx.x; i < md.ntr; i += blockDim.x) {
+        const int k = md.rec_k[i], c = md.rec_c[i];
+        const int kn = (k <= md.ny - 2) ? k + 1 : (k == md.ny - 1 ? k - 1 : k);
+        out.tmod[ob + i] = TA[addr_a<R>(k, c, md.bstride)];
+        out.tnb[ob + i] = TA[addr_a<R>(kn, c, md.bstride)];
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < md.nuq; u += blockDim.x) {
+        const int k = md.uq_k[u], c = md.uq_c[u];
+        const int pa = addr_a<R>(k, c, md.bstride), pb = addr_b<R>(k, c, md.bstride);
+        TA[pa] = __uint_as_float(__float_as_uint(TA[pa]) | 0x80000000u);
+        TB[pb] = __uint_as_float(__float_as_uint(TB[pb]) | 0x80000000u);
+    }
+    if (threadIdx.x == 0) out.it_f[shot_base[sh.model] + sh.shot] = it;
+}
+
+template <int R, bool FD, bool CHKINF>
+__global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) void k_adjoint_split(
+    const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, int nshots, Slots slots,
+    const long long *__restrict__ rec_off, const int *__restrict__ shot_base, const float *__restrict__ tres,
+    ShotOut out, SplitSync ss)
+{
+    constexpr int BR = kWave * R;
+    __shared__ float ring[kMaxBands * kRingQ + kWave];
+    __shared__ double red[kMaxBands];
+    __shared__ float stage_all[max_bands_for(R) * Stage<R>::WAVE_FLOATS];
+    int s, role;
+    if (!split_role(nshots, s, role)) return;
+    const ShotDev sh = shots[s];
+    const ModelDev md = mds[sh.model];
+    const int nwaves = blockDim.x >> 6;
+    int *cnt = ss.cnt + (long long)s * kSyncInts;
+    double *part = ss.part + (long long)s * 2 * kMaxBands;
+    if (!pair_handshake(cnt, role, ss.err)) return;
+    const WaveCtx w = make_wave_split(md, ring, role);
+    float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
+    Pace pc;
+    pc.pre = pc.post = 0; pc.lead = cnt + kSyncLead; pc.err = ss.err; pc.base = 0;
+    pc.D = pair_delay(md.nb, Stage<R>::HOLD);
+    const long long so = (long long)sh.slot * slots.slot_stride;
+    const float *TA = slots.TA + so, *TB = slots.TB + so;
+    float *LA = slots.LA + so, *LB = slots.LB + so;
+    const float big = (float)(10.0 * (double)md.time);
+    if (role == 0) {   // init_travel_adj.c:22-58, before the leading sweep's first publication
+        const int bs = (int)md.bstride;
+        for (int b = 0; b < md.nb; ++b)
+            for (int q = threadIdx.x; q < bs; q += blockDim.x) {
+                const int key = q / BR, wv = q - key * BR, l = wv / R, r = wv - l * R;
+                const int k = b * BR + l * R + r, c = key - l;
+                const bool inner = k > 0 && k < md.ny - 1 && c > 0 && c < md.nx - 1;
+                LA[b * bs + q] = inner ? big : 0.f;
+            }
+        __syncthreads();
+        const long long ob = rec_off[sh.model] + (long long)sh.shot * md.ntr;
+        for (int u = threadIdx.x; u < md.nuq; u += blockDim.x)
+            LA[addr_a<R>(md.uq_k[u], md.uq_c[u], md.bstride)] = tres[ob + md.uq_r[u]];
+        __syncthreads();
+    }
+    float lnorm = big;                                            // eikonal_adj.c:34
+    const float denom = (float)(md.nx * md.ny);
+    int it = 0, phase = 0;
+    bool ok = true;
+    while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
+        if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
+        double acc;
+        pc.base = phase * kPhaseStride;
+        if (role == 0)
+            acc = adj_sweep<R, +1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB,
+                                                                      stage, pc);
+        else
+            acc = adj_sweep<R, +1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB,
+                                                                      stage, pc);
+        ok = rendezvous(cnt, role, phase++, ss.err);
+        if (!ok) break;
+        pc.base = phase * kPhaseStride;
+        if (role == 0)
+            acc += adj_sweep<R, -1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA,
+                                                                       stage, pc);
+        else
+            acc += adj_sweep<R, -1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA,
+                                                                       stage, pc);
+        const double tot = pair_sum(acc, red, nwaves, part, cnt, role, phase++, ss.err, ok);
+        if (!ok) break;
+        lnorm = (float)tot / denom;
+        ++it;
+    }
+    if (role == 0 && threadIdx.x == 0) out.it_a[shot_base[sh.model] + sh.shot] = it;
+}
+
+// Zero the split shots' counters before a launch (a kernel, so that graphs
+// can hold it).
+__global__ void k_split_reset(int *cnt, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        cnt[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1211,6 +1599,42 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
     if (nshots <= 0) return hipSuccess;
     RZ_DISPATCH_R(R, hipLaunchKernelGGL(k_forward<RR>, dim3(nshots), dim3(kGroups * nb_max * kWave), 0, st, md,
                                         shots, slots, rec_off, shot_base, out));
+    return hipGetLastError();
+}
+
+hipError_t launch_forward_split(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots,
+                                Slots slots, const long long *rec_off, const int *shot_base, ShotOut out,
+                                SplitSync ss, hipStream_t st)
+{
+    if (nshots <= 0) return hipSuccess;
+    const long long n = (long long)nshots * kSyncInts;
+    hipLaunchKernelGGL(k_split_reset, dim3((unsigned)std::min<long long>((n + 255) / 256, 1024)), dim3(256), 0, st,
+                       ss.cnt, n);
+    const dim3 g(16 * ((nshots + 7) / 8)), b(nb_max * kWave);
+    RZ_DISPATCH_R(R, hipLaunchKernelGGL(k_forward_split<RR>, g, b, 0, st, md, shots, nshots, slots, rec_off,
+                                        shot_base, out, ss));
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_split(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots,
+                                Slots slots, const long long *rec_off, const int *shot_base, const float *tres,
+                                ShotOut out, int variant, SplitSync ss, hipStream_t st)
+{
+    if (nshots <= 0) return hipSuccess;
+    const long long n = (long long)nshots * kSyncInts;
+    hipLaunchKernelGGL(k_split_reset, dim3((unsigned)std::min<long long>((n + 255) / 256, 1024)), dim3(256), 0, st,
+                       ss.cnt, n);
+    const dim3 g(16 * ((nshots + 7) / 8)), b(nb_max * kWave);
+#define RZ_ADJ_LAUNCH(FD_, CI_)                                                                           \
+    RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint_split<RR, FD_, CI_>), g, b, 0, st, md, shots, nshots, \
+                                        slots, rec_off, shot_base, tres, out, ss))
+    switch (variant & 3) {
+    case 0: RZ_ADJ_LAUNCH(false, false); break;
+    case 1: RZ_ADJ_LAUNCH(true, false); break;
+    case 2: RZ_ADJ_LAUNCH(false, true); break;
+    default: RZ_ADJ_LAUNCH(true, true); break;
+    }
+#undef RZ_ADJ_LAUNCH
     return hipGetLastError();
 }
 

@@ -24,7 +24,7 @@ EXPORTS = (
     "rz_get_tres_carry", "rz_get_field", "rz_eikonal_shot", "rz_eikonal_adj_shot", "rz_ass_grad_shot",
     "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
-    "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb",
+    "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb", "rz_set_split", "rz_get_split",
 )
 
 
@@ -101,6 +101,8 @@ def lib():
                 "rz_autotune_rows": (i, [i, i, C.POINTER(ModelDesc), C.c_size_t, C.POINTER(i), C.POINTER(d)]),
                 "rz_gauss_filt": (i, [i, vp, vp, i, i, C.POINTER(i), C.POINTER(i), i]),
                 "rz_get_tnb": (i, [vp, i, vp, i]),
+                "rz_set_split": (i, [vp, i]),
+                "rz_get_split": (i, [vp, C.POINTER(i), C.POINTER(i)]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -366,6 +368,16 @@ class Context:
         n = np.zeros(4, np.int64)
         self._chk(lib().rz_get_profile(self.h, _ptr(ms), _ptr(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("forward", "residual", "adjoint", "gradient"))}
+
+    def set_split(self, on=True):
+        """Two workgroups per shot (rz_set_split; on=2 forces the fallback)."""
+        self._chk(lib().rz_set_split(self.h, int(on)))
+
+    def split_state(self):
+        """(active, failure) of rz_get_split."""
+        a, f = C.c_int(0), C.c_int(0)
+        self._chk(lib().rz_get_split(self.h, C.byref(a), C.byref(f)))
+        return a.value, f.value
 
     @property
     def rows_per_lane(self):

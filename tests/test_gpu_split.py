@@ -1,0 +1,61 @@
+"""Split shots (two workgroups per shot, rz_set_split): the same bits as one
+workgroup per shot and as the CPU restatement, and the fallback when a
+split launch reports failure."""
+import numpy as np
+import pytest
+
+from rajzel_amd import workloads as W
+from tests.helpers import bits_equal, model_dict, ndiff, oracle_picks
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(N, sv, R, split):
+    with N.Context([model_dict(sv, max_iter=200)], rows_per_lane=R) as ctx:
+        ctx.set_split(split)
+        ctx.set_slowness(0, sv.S)
+        ctx.set_picks(0, sv.tobs)
+        ctx.gradient()
+        itf, ita = ctx.iterations(0)
+        out = dict(grad=ctx.grad(0), tmod=ctx.tmod(0), tres=ctx.tres(0), L2=ctx.misfit(0)[0], itf=itf, ita=ita,
+                   state=ctx.split_state())
+        if split == 2:
+            ctx.gradient()          # stays unsplit after the fallback
+            out["again"] = ctx.grad(0)
+            out["state2"] = ctx.split_state()
+    return out
+
+
+@pytest.mark.parametrize("R,shape,ns", [(1, (37, 53), 3), (2, (150, 70), 4), (4, (300, 90), 5), (8, (700, 40), 3),
+                                        (1, (200, 41), 9)])
+def test_split_equals_one_workgroup_and_oracle(ora, R, shape, ns):
+    from rajzel_amd import native as N
+    ny, nx = shape
+    sv = W.small_case(nx=nx, ny=ny, nshots=ns, seed=90 + ny, rec_row=(2, ny - 1))
+    oracle_picks(ora, sv)
+    print(f"[case] split {ny}x{nx} R={R} shots={ns}", flush=True)
+    a = _run(N, sv, R, True)
+    assert a["state"] == (1, 0)
+    b = _run(N, sv, R, False)
+    for k in ("grad", "tmod", "tres"):
+        assert bits_equal(a[k], b[k]), f"{k}: {ndiff(a[k], b[k])} differ"
+    assert a["L2"] == b["L2"]
+    assert np.array_equal(a["itf"], b["itf"]) and np.array_equal(a["ita"], b["ita"])
+    if ny * nx <= 40000:
+        o = ora.grad_obj(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.tobs, sv.dh)
+        assert bits_equal(a["grad"], o["grad"]) and a["L2"] == o["L2"]
+
+
+def test_failed_split_launch_falls_back():
+    """The next split launch starts with its failure flag raised (forced from
+    the host): the context repeats the work unsplit, reports failure 2 and
+    stays unsplit; the results are those of one workgroup per shot."""
+    from rajzel_amd import native as N
+    sv = W.small_case(nx=64, ny=130, nshots=4, seed=5, rec_row=(2, 129))
+    sv.tobs = np.full((sv.nshots, sv.ntr), 0.3, np.float32)
+    print("[case] split fallback 130x64", flush=True)
+    a = _run(N, sv, 2, 2)
+    b = _run(N, sv, 2, False)
+    assert a["state"] == (0, 2) and a["state2"] == (0, 2)
+    assert bits_equal(a["grad"], b["grad"]) and bits_equal(a["again"], b["grad"])
+    assert np.array_equal(a["itf"], b["itf"]) and np.array_equal(a["ita"], b["ita"])
