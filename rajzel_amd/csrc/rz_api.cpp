@@ -65,6 +65,17 @@ struct rz_ctx {
     int ncu = 0;
     int split_fail = 0;                 // 1: workgroups of a shot on different XCDs, 2: a wait timed out
     SplitSync ss{nullptr, nullptr, nullptr};
+    // Mixed launches: with more shots than half the CUs (and fewer than
+    // the CUs), the shots that took the most outer iterations in the
+    // previous evaluation run split on a side stream, the rest beside them
+    // with one workgroup each, so that every workgroup has a CU.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<int> hist_f, hist_a;    // outer iterations per shot in the last run (-1 unknown)
+    ShotDev *plan_d = nullptr;          // [4][total_shots]: forward split / rest, adjoint split / rest
+    int plan_n[4] = {0, 0, 0, 0};
+    std::vector<int> plan_ids;          // shot order of the plan now on the device
+    bool plan_on = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     long long s_epoch = 0;              // bumped by every rz_set_slowness
     long long fields_epoch = -1;        // slowness epoch of resident forward fields of all shots
@@ -244,6 +255,9 @@ int check_convergence(rz_ctx *c, bool adj)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return kSplitRetry;
     }
+    c->hist_f = itf;
+    if (adj) c->hist_a = ita;
+    else if ((int)c->hist_a.size() != c->total_shots) c->hist_a.assign(c->total_shots, -1);
     for (int i = 0; i < c->total_shots; ++i)
         if (itf[i] < 0 || (adj && ita[i] < 0))
             return fail(c, RZ_ERR_NOT_CONVERGED, "shot %d did not converge within max_iter", i);
@@ -289,6 +303,69 @@ int collect(rz_ctx *c, const std::vector<PhaseMarks> &marks, bool grad)
 
 void drop_graphs(rz_ctx *c);
 
+// The side stream follows the work stream from here on / the work stream
+// waits for everything the side stream has so far.
+int fork_side(rz_ctx *c)
+{
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    return RZ_OK;
+}
+
+int join_side(rz_ctx *c)
+{
+    HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return RZ_OK;
+}
+
+// Decide the mixed launches of the next single-batch run from the last
+// run's iteration counts (stable order: more iterations first, then shot
+// order) and put the shot lists on the device when they change.
+int plan_mixed(rz_ctx *c, int per)
+{
+    const int T = c->total_shots;
+    c->plan_on = false;
+    if (!c->split || per < T || T <= 0 || 2 * T <= c->ncu || T >= c->ncu) return RZ_OK;
+    if ((int)c->hist_f.size() != T || (int)c->hist_a.size() != T) return RZ_OK;
+    for (int i = 0; i < T; ++i)
+        if (c->hist_f[i] < 0 || c->hist_a[i] < 0) return RZ_OK;
+    const int ns = c->ncu - T;
+    std::vector<int> ids;
+    ids.reserve(4 * (size_t)T);
+    for (const std::vector<int> *h : {&c->hist_f, &c->hist_a}) {
+        std::vector<int> ord(T);
+        for (int i = 0; i < T; ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return (*h)[a] > (*h)[b]; });
+        std::vector<char> in(T, 0);
+        for (int i = 0; i < ns; ++i) in[ord[i]] = 1;
+        for (int i = 0; i < T; ++i) if (in[i]) ids.push_back(i);       // split, in shot order
+        for (int i = 0; i < T; ++i) if (!in[i]) ids.push_back(i);      // the rest
+    }
+    if (!c->side) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
+    if (!c->plan_d) TRY(dalloc(c, &c->plan_d, 4 * (size_t)T));
+    if (ids != c->plan_ids) {
+        std::vector<ShotDev> lists(4 * (size_t)T);
+        for (int q = 0; q < 2; ++q)
+            for (int i = 0; i < T; ++i) {
+                const int shot = ids[(size_t)q * T + i];
+                // forward lists at [0, T) and [T, 2T), adjoint at [2T, 3T) and [3T, 4T)
+                const size_t at = i < ns ? (size_t)(2 * q) * T + i : (size_t)(2 * q + 1) * T + (i - ns);
+                lists[at] = c->shots_h[shot];
+            }
+        HIPCHK(c, hipMemcpy(c->plan_d, lists.data(), lists.size() * sizeof(ShotDev), hipMemcpyHostToDevice));
+        c->plan_ids = ids;
+        drop_graphs(c);
+    }
+    c->plan_n[0] = ns; c->plan_n[1] = T - ns; c->plan_n[2] = ns; c->plan_n[3] = T - ns;
+    c->plan_on = true;
+    return RZ_OK;
+}
+
 // Split shots (two workgroups each) while every workgroup can have a CU of
 // its own: at most one shot per two CUs.  With more shots the workgroups
 // would share CUs (two fit one), which measured slower than one workgroup
@@ -306,7 +383,16 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
     PhaseMarks &m = pm ? *pm : dummy;
     TRY(mark(c, &m.e[0]));
     const bool split = use_split(c, b1 - b0);
-    if (!reuse) {
+    const bool mixed = !split && c->plan_on && b0 == 0 && b1 == c->total_shots;
+    if (!reuse && mixed) {
+        const long long T = c->total_shots;
+        TRY(fork_side(c));
+        HIPCHK(c, launch_forward_split(c->R, c->nb_max, c->md_d, c->plan_d, c->plan_n[0], sl, c->rec_off_d,
+                                       c->shot_base_d, so, c->ss, c->side));
+        HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->plan_d + T, c->plan_n[1], sl, c->rec_off_d,
+                                 c->shot_base_d, so, c->stream));
+        TRY(join_side(c));
+    } else if (!reuse) {
         if (split)
             HIPCHK(c, launch_forward_split(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
                                            c->shot_base_d, so, c->ss, c->stream));
@@ -326,7 +412,15 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
                                   c->l2_shot_d + h.shot_base, c->l2_sum_d + mi, c->stream));
     }
     TRY(mark(c, &m.e[2]));
-    if (split)
+    if (mixed) {
+        const long long T = c->total_shots;
+        TRY(fork_side(c));
+        HIPCHK(c, launch_adjoint_split(c->R, c->nb_max, c->md_d, c->plan_d + 2 * T, c->plan_n[2], sl, c->rec_off_d,
+                                       c->shot_base_d, c->tres_d, so, adj_variant(c), c->ss, c->side));
+        HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->plan_d + 3 * T, c->plan_n[3], sl, c->rec_off_d,
+                                 c->shot_base_d, c->tres_d, so, adj_variant(c), c->stream));
+        TRY(join_side(c));
+    } else if (split)
         HIPCHK(c, launch_adjoint_split(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
                                        c->shot_base_d, c->tres_d, so, adj_variant(c), c->ss, c->stream));
     else
@@ -389,6 +483,7 @@ int run(rz_ctx *c, bool grad, int cap, bool direct = false)
     // Shot g lives in slot g % nslots; any batch of at most nslots
     // consecutive shots therefore uses distinct slots.
     const int per = std::max(1, std::min(cap, c->nslots));
+    TRY(plan_mixed(c, per));
     const bool use_graph = c->graph && !direct && !c->prof && c->stream != nullptr && c->total_shots > 0 &&
                            per >= c->total_shots;
     if (use_graph) {
@@ -473,6 +568,10 @@ void rz_destroy(rz_ctx *c)
     drop_graphs(c);
     for (void *p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }

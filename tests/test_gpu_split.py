@@ -59,3 +59,32 @@ def test_failed_split_launch_falls_back():
     assert a["state"] == (0, 2) and a["state2"] == (0, 2)
     assert bits_equal(a["grad"], b["grad"]) and bits_equal(a["again"], b["grad"])
     assert np.array_equal(a["itf"], b["itf"]) and np.array_equal(a["ita"], b["ita"])
+
+
+def test_mixed_launch_after_history_equals_one_workgroup():
+    """More shots than half the CUs: the second evaluation splits the shots
+    that needed the most iterations in the first and runs the others
+    beside them; every result equals the one-workgroup run's."""
+    from rajzel_amd import native as N
+    sv = W.small_case(nx=90, ny=150, nshots=160, seed=12, rec_row=(2, 149))
+    rng = np.random.default_rng(3)
+    sv.tobs = (0.2 + rng.uniform(0.0, 0.3, (sv.nshots, sv.ntr))).astype(np.float32)
+    print("[case] mixed launch 150x90, 160 shots", flush=True)
+    res = {}
+    for split in (False, True):
+        with N.Context([model_dict(sv, max_iter=200)], rows_per_lane=2) as ctx:
+            ctx.set_split(split)
+            ctx.set_slowness(0, sv.S)
+            ctx.set_picks(0, sv.tobs)
+            ctx.gradient()
+            first = ctx.grad(0)
+            ctx.gradient()
+            itf, ita = ctx.iterations(0)
+            res[split] = dict(first=first, grad=ctx.grad(0), tmod=ctx.tmod(0), tres=ctx.tres(0),
+                              L2=ctx.misfit(0)[0], itf=itf, ita=ita, state=ctx.split_state())
+    a, b = res[True], res[False]
+    assert a["state"][1] == 0
+    for k in ("first", "grad", "tmod", "tres"):
+        assert bits_equal(a[k], b[k]), f"{k}: {ndiff(a[k], b[k])} differ"
+    assert a["L2"] == b["L2"]
+    assert np.array_equal(a["itf"], b["itf"]) and np.array_equal(a["ita"], b["ita"])
