@@ -168,19 +168,20 @@ def copy_probe(mib):
         hip.hipFree(b)
 
 
-def measured_traffic(path, kernel, workload):
-    """HBM bytes per launch of ``kernel`` from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), when it was
-    taken on this same workload; None otherwise."""
+def measured_traffic(path, phase, workload):
+    """HBM bytes of one evaluation's ``phase`` ("forward" or "adjoint": the
+    one-workgroup and the split kernel of the phase together) from the
+    committed counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    calibration), when it was taken on this same workload; None otherwise."""
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    k = rec.get("kernels", {}).get(kernel)
+    k = rec.get("phases", {}).get(phase)
     if k is None or rec.get("workload") != workload:
         return None
-    return float(k["hbm_bytes_per_launch"])
+    return float(k["hbm_bytes"])
 
 
 def cpu_baseline(sv, S, tobs, nsample, gpu_tmod, gpu_tres, gpu_itf, gpu_ita):
@@ -233,6 +234,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-precond", action="store_true", help="skip the Gaussian gradient filter timing")
+    ap.add_argument("--no-split", action="store_true",
+                    help="one workgroup per shot in every launch (RAJZEL_AMD_SPLIT=0)")
     ap.add_argument("--concurrency-scan", action="store_true",
                     help="only print per-iteration kernel times versus shots in flight")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -245,6 +248,8 @@ def main():
               f"(--gpus {args.gpus}) is not part of this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.no_split:
+        os.environ["RAJZEL_AMD_SPLIT"] = "0"
     from rajzel_amd import formats as F
     from rajzel_amd import native as N
     from rajzel_amd import workloads as W
@@ -300,7 +305,7 @@ def main():
                f"{sv.nshots} shots, {sv.ntr} receivers, 12-layer model"
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(args.traffic, f"k_{dom}", workload), "kernel": f"k_{dom}",
+                "traffic": measured_traffic(args.traffic, dom, workload), "kernel": f"k_{dom}",
                 "bytes_per_launch": phase_bytes[dom], "avg_launch_ms": phase_ms[dom]}
 
     out = {
