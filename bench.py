@@ -274,7 +274,12 @@ def main():
     sv.tobs = F.roundtrip_picks(ctx.tmod(0) + rng.normal(0.0, sv.noise_s, (sv.nshots, sv.ntr)))
     ctx.set_slowness(0, S)
     ctx.set_picks(0, sv.tobs)
-    log(f"[bench] setup done, R={ctx.rows_per_lane}, {ctx.device_bytes / 2**30:.2f} GiB on device")
+    # One evaluation as part of the setup: it gives the context the shots'
+    # outer-iteration counts that schedule the split shots of later ones (in
+    # an inversion every evaluation after the first has them).
+    ctx.gradient()
+    log(f"[bench] setup done, R={ctx.rows_per_lane}, {ctx.device_bytes / 2**30:.2f} GiB on device, "
+        f"split {ctx.split_state()}")
 
     for _ in range(args.warmup):
         ctx.gradient()
@@ -305,7 +310,8 @@ def main():
                f"{sv.nshots} shots, {sv.ntr} receivers, 12-layer model"
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(args.traffic, dom, workload), "kernel": f"k_{dom}",
+                "traffic": measured_traffic(args.traffic, dom, workload),
+                "kernel": f"{dom} phase: k_{dom} (+ k_{dom}_split in mixed launches)",
                 "bytes_per_launch": phase_bytes[dom], "avg_launch_ms": phase_ms[dom]}
 
     out = {
