@@ -88,3 +88,29 @@ def test_mixed_launch_after_history_equals_one_workgroup():
         assert bits_equal(a[k], b[k]), f"{k}: {ndiff(a[k], b[k])} differ"
     assert a["L2"] == b["L2"]
     assert np.array_equal(a["itf"], b["itf"]) and np.array_equal(a["ita"], b["ita"])
+
+
+def test_mixed_launch_replayed_as_graph():
+    """Mixed launches (side stream fork and join) recorded as HIP graphs and
+    replayed: every replay equals the one-workgroup run."""
+    from rajzel_amd import native as N
+    sv = W.small_case(nx=70, ny=120, nshots=150, seed=14, rec_row=(2, 119))
+    rng = np.random.default_rng(4)
+    sv.tobs = (0.2 + rng.uniform(0.0, 0.3, (sv.nshots, sv.ntr))).astype(np.float32)
+    print("[case] mixed launch as graph 120x70, 150 shots", flush=True)
+    with N.Context([model_dict(sv, max_iter=200)], rows_per_lane=2) as ref:
+        ref.set_split(False)
+        ref.set_slowness(0, sv.S)
+        ref.set_picks(0, sv.tobs)
+        ref.gradient()
+        want = ref.grad(0)
+    with N.Context([model_dict(sv, max_iter=200)], rows_per_lane=2) as ctx:
+        ctx.set_slowness(0, sv.S)
+        ctx.set_picks(0, sv.tobs)
+        ctx.gradient()                  # iteration history
+        ctx.set_graph(True)
+        for rep in range(3):
+            ctx.gradient()
+            ctx.sync()
+            assert bits_equal(ctx.grad(0), want), f"replay {rep}: {ndiff(ctx.grad(0), want)} differ"
+        assert ctx.split_state() == (1, 0)
