@@ -318,6 +318,28 @@ struct WaveCtx {
     float *ring;
 };
 
+// A value every lane of the wave holds, as a scalar.  The sweeps are
+// separate functions, whose arguments arrive in vector registers; without
+// this the compiler computes every index derived from them per lane.
+template <typename T>
+__device__ __forceinline__ T uni(T v)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    int u[sizeof(T) / 4];
+    __builtin_memcpy(u, &v, sizeof(T));
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) u[k] = __builtin_amdgcn_readfirstlane(u[k]);
+    __builtin_memcpy(&v, u, sizeof(T));
+    return v;
+}
+
+__device__ __forceinline__ WaveCtx uni_ctx(const WaveCtx &w)
+{
+    WaveCtx u = uni(w);
+    u.lane = w.lane;
+    return u;
+}
+
 // Steps of a sweep: the last band's keys plus `hold` steps to drain staged
 // stores, rounded up to whole epochs.
 __device__ __forceinline__ int sweep_steps(int nx, int nb, int hold = 0)
@@ -385,9 +407,14 @@ __device__ __forceinline__ Pace lockstep(int pre, int post)
 // all of this wave's vector memory operations complete (vmcnt(0))
 __device__ __forceinline__ void drain_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// Counters of split shots, addressed as global memory (the sweeps are
+// separate functions, where a plain pointer would be a generic one: generic
+// accesses make every later wait of the wave a full one).
+typedef __attribute__((address_space(1))) int gint;
+
 __device__ __forceinline__ void publish(int *p, int v)
 {
-    if (threadIdx.x == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_store((gint *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Thread 0 polls *p >= target (bounded: on timeout, or when another wait of
@@ -401,10 +428,10 @@ __device__ bool pace_wait(int *p, int target, int *err)
     if (threadIdx.x == 0) {
         int ok = 1;
         for (unsigned n = 0;; ++n) {
-            if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
+            if (__hip_atomic_load((gint *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            if (__hip_atomic_load((gint *)err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
             if (n >= kSpinLimit) {
-                __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store((gint *)err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
             }
@@ -493,10 +520,18 @@ __device__ __forceinline__ void pace_idle(const Pace &pc, int nep)
 // vmcnt instead of draining the queue.
 // ---------------------------------------------------------------------------
 template <int R, int SX, int SY, bool INIT, int PACE = kPaceLock>
-__device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const float *Tin,
-                            const float *Sf, float *Tout, float *stage, int src_k, int src_c, float tinit,
-                            const Pace &pc)
+__device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh_, double two_dh2_,
+                                                      const float *Tin_, const float *Sf_, float *Tout_,
+                                                      float *stage_, int src_k_, int src_c_, float tinit_,
+                                                      const Pace pc_)
 {
+    const WaveCtx w = uni_ctx(w_);
+    const float dh = uni(dh_), tinit = uni(tinit_);
+    const double two_dh2 = uni(two_dh2_);
+    const float *Tin = uni(Tin_), *Sf = uni(Sf_);
+    float *Tout = uni(Tout_), *stage = uni(stage_);
+    const int src_k = uni(src_k_), src_c = uni(src_c_);
+    const Pace pc = uni(pc_);
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
     constexpr int SP = Stage<R>::P, SQ = Stage<R>::Q;
@@ -539,7 +574,8 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
     const int erow = SY > 0 ? (beta + 1) * BR : beta * BR - 1;
     const int ering = clampi(SY > 0 ? beta - 1 : beta + 1, 0, kMaxBands - 1);
     const bool ring_writer = (SY > 0 ? (l == 63) : (l == 0)) && (SY > 0 ? (beta + 1 < nb) : (beta > 0));
-    float *ring_dummy = w.ring + kMaxBands * kRingQ + l;
+    lfloat *ring = (lfloat *)w.ring;
+    lfloat *ring_dummy = ring + kMaxBands * kRingQ + l;
 
     float qt[kPF][R], qs[kPF][R], qe[kPF];
 
@@ -563,6 +599,12 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
 
 #pragma unroll
     for (int u = 0; u < kPF; ++u) RZ_FWD_LOAD(u, u - off);
+    // The set-up ends in a full wait: the waits the compiler derives at the
+    // loop head then come from the loop's own pattern of prefetches (a slot
+    // is waited for about seven steps after its load), not from a set-up
+    // load just before the loop, which would cost a full memory latency in
+    // every epoch.
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
 
     float pnew[R];
 #pragma unroll
@@ -583,7 +625,7 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
             const float ringv = ring_next;
             {
                 const int cn = SX > 0 ? tau + 1 - lag : nx - 2 - tau + lag;
-                ring_next = w.ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
+                ring_next = ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
             }
             float nv[R];
 #pragma unroll
@@ -629,7 +671,7 @@ __device__ double fwd_sweep(const WaveCtx &w, float dh, double two_dh2, const fl
                 ko = (unsigned)ko < (unsigned)nkeys ? ko : nkeys;
                 stv<R>(gTout + ko * BR + l * R, pc);
             }
-            float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
+            lfloat *rw = (ring_writer && cvalid) ? &ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
             RZ_FWD_LOAD(u, tau + kPF);
         }
@@ -686,9 +728,16 @@ __device__ __forceinline__ float neg_part(float x)
 // storing them.  The memory operations per step are fixed, as in fwd_sweep.
 // ---------------------------------------------------------------------------
 template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF, int PACE = kPaceLock>
-__device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, const float *Tf,
-                            const float *Lin, float *Lout, float *stage, const Pace &pc)
+__device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh_, double rdh_, float eps_,
+                                                      const float *Tf_, const float *Lin_, float *Lout_,
+                                                      float *stage_, const Pace pc_)
 {
+    const WaveCtx w = uni_ctx(w_);
+    const float dh = uni(dh_), eps = uni(eps_);
+    const double rdh = uni(rdh_);
+    const float *Tf = uni(Tf_), *Lin = uni(Lin_);
+    float *Lout = uni(Lout_), *stage = uni(stage_);
+    const Pace pc = uni(pc_);
     constexpr bool FA = (SX == SY);
     constexpr bool LA_ = FA;   // family of Lin's layout
     constexpr int BR = kWave * R;
@@ -733,7 +782,8 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
     const long long dn_base = has_dn ? w.bstride : 0;
     const int ering = clampi(SY > 0 ? beta - 1 : beta + 1, 0, kMaxBands - 1);
     const bool ring_writer = (SY > 0 ? (l == 63) : (l == 0)) && (SY > 0 ? has_dn : has_up);
-    float *ring_dummy = w.ring + kMaxBands * kRingQ + l;
+    lfloat *ring = (lfloat *)w.ring;
+    lfloat *ring_dummy = ring + kMaxBands * kRingQ + l;
 
     float qt[kPF][R], ql[kPF][R], qeu[kPF], qed[kPF], qel[kPF];
 
@@ -754,19 +804,35 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
         qel[SLOT] = lband[el_];                                                             \
     } while (0)
 
+    // Slot u holds the blocks of step u (mod kPF).  A step reads its own
+    // slot, the next one (T and lambda at the next key) and the previous one
+    // (T at the previous key), then reloads the previous slot with the
+    // blocks of kPF - 1 steps ahead: no value outlives its slot, so the
+    // steady-state loop needs no register copies, and its waits are counted
+    // ones (a slot is waited for about five steps after its load).  The
+    // set-up ends in a full wait, so that the loop head's waits do not
+    // depend on set-up loads.  Slot kPF-1 is first loaded by step 0; before
+    // that it stands for T at the key before the first, which only cells
+    // that keep their value read.
 #pragma unroll
-    for (int u = 0; u < kPF; ++u) RZ_ADJ_LOAD(u, u - off);
+    for (int u = 0; u < kPF - 1; ++u) RZ_ADJ_LOAD(u, u - off);
+#pragma unroll
+    for (int r = 0; r < R; ++r) qt[kPF - 1][r] = 0.f;
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
 
-    float tprev[R], pnew[R], xcarry[R], ybp[R];
+    // xcp/xcn: positive and negative parts of the x quotient carried to the
+    // next step (ap(c) is am(c+1)); the parts of +0 are +0
+    float pnew[R], xcp[R], xcn[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) { tprev[r] = 0.f; pnew[r] = 0.f; xcarry[r] = 0.f; ybp[r] = 0.f; }
+    for (int r = 0; r < R; ++r) { pnew[r] = 0.f; xcp[r] = 0.f; xcn[r] = 0.f; }
     double acc = 0.0;
     float ring_next = 0.f;
 
     for (int g0 = 0; g0 < gend; g0 += kPF) {
 #pragma unroll
         for (int u = 0; u < kPF; ++u) {
-            const int un = (u + 1) % kPF;
+            const int un = (u + 1) % kPF, up = (u + kPF - 1) % kPF;
+            const float (&tprev)[R] = qt[up];   // T at the previous key
             const int tau = g0 + u - off;
             const bool active = tau >= 0 && tau < nkeys;   // wave-uniform
             const int c = SX > 0 ? tau - lag : nx - 1 - tau + lag;
@@ -774,7 +840,7 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
             const float ringv = ring_next;   // read one step ahead, as in fwd_sweep
             {
                 const int cn = SX > 0 ? tau + 1 - lag : nx - 2 - tau + lag;
-                ring_next = w.ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
+                ring_next = ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
             }
             float nv[R];
 #pragma unroll
@@ -805,8 +871,11 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     if (l == 0) l_up = qel[u];
                     if (l == 63) l_dn = ringv;
                 }
-                // T-only part of the cell equations (no dependence on lambda)
-                float lhs[R], amp[R], apm[R], bmp[R], bpm[R], xq[R];
+                // T-only part of the cell equations (no dependence on lambda).
+                // Each quotient's positive and negative parts are taken once:
+                // am(c+1) = ap(c) comes from the previous step, bm(k) = bp(k-1)
+                // from the row above in this lane.
+                float lhs[R], amp[R], apm[R], bmp[R], bpm[R], xqp[R], xqn[R], bpp[R];
                 bool keep[R], infv[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -816,29 +885,39 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                     const float tr = fabsf(SX > 0 ? qt[un][r] : tprev[r]);   // T(k, c+1)
                     const float tu = fabsf(r == 0 ? t_above : qt[u][r > 0 ? r - 1 : 0]);
                     const float td = fabsf(r == R - 1 ? t_below : qt[u][r < R - 1 ? r + 1 : r]);
-                    // ap(c) and am(c+1) are the same quotient; carried along x
-                    float ap, am;
+                    float app, amm;   // pos_part(ap), neg_part(am)
                     if (SX > 0) {
-                        am = xcarry[r];
-                        ap = qdh<FD>(-(tr - tc), dh, rdh);
-                        xq[r] = ap;
+                        const float ap = qdh<FD>(-(tr - tc), dh, rdh);
+                        app = pos_part(ap);
+                        apm[r] = neg_part(ap);
+                        amp[r] = xcp[r];
+                        amm = xcn[r];
+                        xqp[r] = app;
+                        xqn[r] = apm[r];
                     } else {
-                        ap = xcarry[r];
-                        am = qdh<FD>(-(tc - tl), dh, rdh);
-                        xq[r] = am;
+                        const float am = qdh<FD>(-(tc - tl), dh, rdh);
+                        amp[r] = pos_part(am);
+                        amm = neg_part(am);
+                        app = xcp[r];
+                        apm[r] = xcn[r];
+                        xqp[r] = amp[r];
+                        xqn[r] = amm;
                     }
                     const float bp = qdh<FD>(-(td - tc), dh, rdh);
-                    ybp[r] = bp;
-                    // bm(k) = bp(k-1): from the row above in this lane
-                    const float bm = r > 0 ? ybp[r > 0 ? r - 1 : 0] : qdh<FD>(-(tc - tu), dh, rdh);
-                    amp[r] = pos_part(am);
-                    apm[r] = neg_part(ap);
-                    bmp[r] = pos_part(bm);
+                    bpp[r] = pos_part(bp);
                     bpm[r] = neg_part(bp);
+                    float bmm;        // neg_part(bm)
+                    if (r > 0) {
+                        bmp[r] = bpp[r > 0 ? r - 1 : 0];
+                        bmm = bpm[r > 0 ? r - 1 : 0];
+                    } else {
+                        const float bm = qdh<FD>(-(tc - tu), dh, rdh);
+                        bmp[r] = pos_part(bm);
+                        bmm = neg_part(bm);
+                    }
                     const bool rec = __float_as_uint(traw) >> 31;
-                    lhs[r] = qdh<FD>(pos_part(ap) - neg_part(am), dh, rdh)
-                           + qdh<FD>(pos_part(bp) - neg_part(bm), dh, rdh);
-                    keep[r] = !cvalid || fcol || frow[r] || rec;
+                    lhs[r] = qdh<FD>(app - amm, dh, rdh) + qdh<FD>(bpp[r] - bmm, dh, rdh);
+                    keep[r] = (!cvalid) | fcol | frow[r] | rec;
                     if constexpr (CHKINF) infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
                     else infv[r] = false;
                 }
@@ -871,9 +950,9 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                 acc += (double)fsum;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    xcarry[r] = xq[r];
+                    xcp[r] = xqp[r];
+                    xcn[r] = xqn[r];
                     pnew[r] = nv[r];
-                    tprev[r] = qt[u][r];
                 }
             }
             if constexpr (LSAME) {
@@ -888,9 +967,9 @@ __device__ double adj_sweep(const WaveCtx &w, float dh, double rdh, float eps, c
                 ko = (unsigned)ko < (unsigned)nkeys ? ko : nkeys;
                 stv<R>(gLout + ko * BR + l * R, pc);
             }
-            float *rw = (ring_writer && cvalid) ? &w.ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
+            lfloat *rw = (ring_writer && cvalid) ? &ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
-            RZ_ADJ_LOAD(u, tau + kPF);
+            RZ_ADJ_LOAD(up, tau + kPF - 1);
         }
         pace_epoch<PACE>(pc, g0 / kPF);
     }
