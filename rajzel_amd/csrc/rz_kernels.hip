@@ -352,14 +352,16 @@ __device__ __forceinline__ int sweep_steps(int nx, int nb, int hold = 0)
 // block for every lane (key + 63 - 2l from A to B, key + 2l - 63 from B to
 // A); stored directly they would be 16-byte pieces of 64 different lines.
 // Instead each lane keeps its pieces in an LDS delay line and stores the
-// piece of step tau - d_l at step tau, where d_l = 2|j_last - j| for lane
+// piece of step tau - d_l at step tau, where d_l = 2|j_last - j| + 1 for lane
 // j = l mod P of its 64-byte segment of P = 16/R lanes: the P lanes of a
 // segment then store the same segment of one block together, and every
-// store instruction writes whole 64-byte segments.
+// store instruction writes whole 64-byte segments.  With the extra step of
+// delay every piece was written at least one step earlier, so a step reads
+// its piece from LDS before its arithmetic and the LDS latency overlaps it.
 template <int R>
 struct Stage {
     static constexpr int P = 16 / R;                 // lanes per 64-byte segment
-    static constexpr int HOLD = 2 * (P - 1);         // longest delay, in steps
+    static constexpr int HOLD = 2 * (P - 1) + 1;     // longest delay, in steps
     static constexpr int Q = (HOLD < 4) ? 4 : (HOLD < 8) ? 8 : (HOLD < 16) ? 16 : 32;   // slots (power of 2)
     static constexpr int WAVE_FLOATS = Q * kWave * R;                                   // 2048 for every R
 };
@@ -556,7 +558,7 @@ __device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh
     const int kshift = FA ? 63 - 2 * l : 2 * l - 63;
     // delay of this lane's staged stores (Stage)
     const int sj = l % SP, sjl = ((SX > 0) == FA) ? SP - 1 : 0;
-    const int dl = 2 * (sjl > sj ? sjl - sj : sj - sjl);
+    const int dl = 2 * (sjl > sj ? sjl - sj : sj - sjl) + 1;
 
     bool rvalid[R], top[R], bot[R];
 #pragma unroll
@@ -630,6 +632,11 @@ __device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh
             float nv[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) nv[r] = qt[u][r];
+            // staged store, part 1: the piece of step tau - dl (written at
+            // least one step ago), read before this step's arithmetic
+            const int tp = tau - dl;
+            float piece[R];
+            ldl<R>(lstage + (tp & (SQ - 1)) * BR, piece);
             if (active) {
                 const bool first = SX > 0 ? (c == 0) : (c == nx - 1);
                 const bool last = SX > 0 ? (c == nx - 1) : (c == 0);
@@ -662,14 +669,11 @@ __device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh
 #pragma unroll
                 for (int r = 0; r < R; ++r) pnew[r] = nv[r];
             }
-            {   // staged store: keep this step's cells, store those of step tau - dl
+            {   // staged store, part 2: keep this step's cells, store the piece
                 stl<R>(lstage + (tau & (SQ - 1)) * BR, nv);
-                const int tp = tau - dl;
-                float pc[R];
-                ldl<R>(lstage + (tp & (SQ - 1)) * BR, pc);
                 int ko = (SX > 0 ? tp : nx + 62 - tp) + kshift;
                 ko = (unsigned)ko < (unsigned)nkeys ? ko : nkeys;
-                stv<R>(gTout + ko * BR + l * R, pc);
+                stv<R>(gTout + ko * BR + l * R, piece);
             }
             lfloat *rw = (ring_writer && cvalid) ? &ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
@@ -739,7 +743,6 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     float *Lout = uni(Lout_), *stage = uni(stage_);
     const Pace pc = uni(pc_);
     constexpr bool FA = (SX == SY);
-    constexpr bool LA_ = FA;   // family of Lin's layout
     constexpr int BR = kWave * R;
     constexpr int SP = Stage<R>::P, SQ = Stage<R>::Q;
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
@@ -763,7 +766,7 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     // staged stores (as in fwd_sweep)
     const int kshift = FA ? 63 - 2 * l : 2 * l - 63;
     const int sj = l % SP, sjl = ((SX > 0) == FA) ? SP - 1 : 0;
-    const int dl = 2 * (sjl > sj ? sjl - sj : sj - sjl);
+    const int dl = 2 * (sjl > sj ? sjl - sj : sj - sjl) + 1;
 
     bool rvalid[R], frow[R];
 #pragma unroll
@@ -799,8 +802,8 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
         const long long ed_ = dn_base + (long long)(FA ? c63_ : c63_ + 63) * BR;            \
         qeu[SLOT] = tband[eu_];                                                             \
         qed[SLOT] = tband[ed_];                                                             \
-        const long long el_ = SY > 0 ? dn_base + (long long)(LA_ ? c63_ : c63_ + 63) * BR   \
-                                     : up_base + (long long)(LA_ ? c0_ + 63 : c0_) * BR;    \
+        const long long el_ = SY > 0 ? dn_base + (long long)(FA ? c63_ : c63_ + 63) * BR    \
+                                     : up_base + (long long)(FA ? c0_ + 63 : c0_) * BR;     \
         qel[SLOT] = lband[el_];                                                             \
     } while (0)
 
@@ -845,7 +848,15 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
             float nv[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) nv[r] = ql[u][r];
-            if (active) {
+
+            // Split shots compute every step: an inactive step only has lanes
+            // that keep their value (cvalid is false), and without the branch
+            // the eight steps of an epoch form one block, in which the
+            // scheduler overlaps a step's T-only terms with the previous
+            // step's lambda chain (one wave per SIMD has no other wave to do
+            // that).  In a workgroup with both sweeps the other wave hides
+            // that latency, and skipping inactive steps saves their work.
+            if (PACE != kPaceLock || active) {
                 [[maybe_unused]] const bool fcol = (c <= 0) || (c >= nx - 1);
                 // T of the rows across the lane boundary, at column c
                 float t_above, t_below;
@@ -958,18 +969,22 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
             if constexpr (LSAME) {
                 const int ks = cvalid ? (SX > 0 ? tau : nx + 62 - tau) : nkeys;
                 stv<R>(gLout + ks * BR + l * R, nv);
-            } else {   // staged store: keep this step's cells, store those of step tau - dl
-                stl<R>(lstage + (tau & (SQ - 1)) * BR, nv);
+            } else {   // staged store, as in fwd_sweep (the piece is read here: reading
+                       // it before the arithmetic measured no faster in this sweep)
                 const int tp = tau - dl;
-                float pc[R];
-                ldl<R>(lstage + (tp & (SQ - 1)) * BR, pc);
+                float piece[R];
+                ldl<R>(lstage + (tp & (SQ - 1)) * BR, piece);
+                stl<R>(lstage + (tau & (SQ - 1)) * BR, nv);
                 int ko = (SX > 0 ? tp : nx + 62 - tp) + kshift;
                 ko = (unsigned)ko < (unsigned)nkeys ? ko : nkeys;
-                stv<R>(gLout + ko * BR + l * R, pc);
+                stv<R>(gLout + ko * BR + l * R, piece);
             }
             lfloat *rw = (ring_writer && cvalid) ? &ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
             RZ_ADJ_LOAD(up, tau + kPF - 1);
+            // keeps the reloads in their step (the scheduler would otherwise sink
+            // them toward their use, shortening the prefetch distance)
+            __builtin_amdgcn_sched_barrier(0);
         }
         pace_epoch<PACE>(pc, g0 / kPF);
     }
