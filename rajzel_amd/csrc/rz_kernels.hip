@@ -353,17 +353,24 @@ __device__ __forceinline__ int sweep_steps(int nx, int nb, int hold = 0)
 // A); stored directly they would be 16-byte pieces of 64 different lines.
 // Instead each lane keeps its pieces in an LDS delay line and stores the
 // piece of step tau - d_l at step tau, where d_l = 2|j_last - j| + 1 for lane
-// j = l mod P of its 64-byte segment of P = 16/R lanes: the P lanes of a
-// segment then store the same segment of one block together, and every
-// store instruction writes whole 64-byte segments.  With the extra step of
-// delay every piece was written at least one step earlier, so a step reads
-// its piece from LDS before its arithmetic and the LDS latency overlaps it.
+// j = l mod P of its segment of P lanes: the P lanes of a segment then store
+// the same segment of one block together, and every store instruction
+// writes whole segments.  Segments are whole 128-byte cache lines (P = 32/R)
+// where a workgroup has 8 waves (R >= 4: 16 KiB of LDS per wave) and
+// 64-byte halves (P = 16/R) where it has 16 (8 KiB per wave).  Scattered
+// half lines cost a sixth of the time per iteration with every CU busy
+// (measured against contiguous stores), whole lines less.  With the extra
+// step of delay every piece was written at least one step earlier, so a
+// step reads its piece from LDS before its arithmetic and the LDS latency
+// overlaps it.
 template <int R>
 struct Stage {
-    static constexpr int P = 16 / R;                 // lanes per 64-byte segment
+    static constexpr int BYTES = R >= 4 ? 128 : 64;  // bytes per segment
+    static constexpr int P = BYTES / (4 * R);        // lanes per segment
     static constexpr int HOLD = 2 * (P - 1) + 1;     // longest delay, in steps
     static constexpr int Q = (HOLD < 4) ? 4 : (HOLD < 8) ? 8 : (HOLD < 16) ? 16 : 32;   // slots (power of 2)
-    static constexpr int WAVE_FLOATS = Q * kWave * R;                                   // 2048 for every R
+    static constexpr int WAVE_FLOATS = Q * kWave * R;                                   // 16 or 8 KiB
+    static_assert(HOLD < Q, "a slot is overwritten only after its piece is stored");
 };
 
 // The barriers of `nep` epochs without a sweep (a group with nothing to do).
@@ -512,7 +519,7 @@ __device__ __forceinline__ void pace_idle(const Pace &pc, int nep)
 // Returns this lane's sum of (old - new) over its valid cells.
 //
 // All loads are whole 64*R-float blocks of this sweep's layout and all
-// stores whole 64-byte segments of the other layout.  Garbage (values of
+// stores whole segments (Stage) of the other layout.  Garbage (values of
 // lanes outside the grid or of steps before the first one) only reaches
 // positions of cells outside the grid, which no valid cell reads.
 //
