@@ -9,6 +9,7 @@
 #include <cstring>
 #include <cstdint>
 #include <string>
+#include <mutex>
 #include <unordered_map>
 #include <chrono>
 #include <vector>
@@ -31,6 +32,10 @@ struct ModelHost {
     int carry_off = 0;       // offset in the carry buffer
     bool have_S = false, have_picks = false, have_res = false;
     bool fastdiv = false;
+    bool ft_dh = false;     // the float FMA quotient is exact for this DH (ft_div)
+    bool ft_s = false;      // ... and the slowness keeps its inputs clear of underflow
+    float rdh_hi = 0.f, rdh_lo = 0.f, s_min_ft = 0.f;
+    double t_min_ft = 0.0;  // smallest nonzero T the float quotient allows
     // device
     float *S = nullptr, *SA = nullptr, *SB = nullptr;
     int *rec_k_d = nullptr, *rec_c_d = nullptr, *uq_k_d = nullptr, *uq_c_d = nullptr, *uq_r_d = nullptr;
@@ -44,6 +49,8 @@ struct rz_ctx {
     int R = 4, nb_max = 1;
     bool fastdiv = true;
     bool chkinf = false;   // INF guard of sweep_adj.c:68 can fire only if 2*TIME >= INF
+    bool ft = false;       // every model allows the float T-only quotients (adjoint variant 2)
+    bool ft_allowed = true;   // RAJZEL_AMD_FLOAT_T=0 turns them off (tests compare both)
     std::vector<ModelHost> m;
     int total_shots = 0;
     long long total_rec = 0, total_cells = 0;
@@ -176,6 +183,71 @@ bool fast_div_exact(float dh)
     return true;
 }
 
+// The adjoint's T-only quotients (slopes and the terms of lhs, sweep_adj.c:
+// 33-54) as fma(x, rh, x * rl) in float, rh = RN(1/DH), rl = RN(1/DH - rh):
+// two instructions instead of three.  Where x/DH keeps away from float
+// rounding ties (a DH with a short significand: 5 = 101b puts every
+// quotient at least 2^-27 relative from a tie) and the sum rh + rl errs by
+// ~2^-48, this is RN(x/DH) -- provided x * rl is a normal float (|x| >=
+// xmin).  Confirmed for every significand of one binade; other binades
+// follow by scaling with powers of 2 while nothing underflows or overflows.
+struct FtDiv {
+    bool ok;
+    float rh, rl;
+    double xmin;   // smallest |x| != 0 for which the identity holds
+};
+
+FtDiv ft_div(float dh)
+{
+    static std::mutex mu;
+    static std::unordered_map<uint32_t, FtDiv> cache;
+    uint32_t key;
+    std::memcpy(&key, &dh, 4);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    FtDiv r{false, 0.f, 0.f, 0.0};
+    const volatile float one = 1.f;
+    r.rh = one / dh;
+    r.rl = (float)(1.0 / (double)dh - (double)r.rh);
+    r.xmin = r.rl != 0.f ? std::ldexp(1.0, -126) / std::fabs((double)r.rl) : 0.0;
+    bool ok = std::isfinite(r.rh) && r.rh > 0.f && std::isfinite(r.rl) && r.xmin < std::ldexp(1.0, 60);
+    for (uint32_t i = 0; ok && i < (1u << 23); ++i) {
+        const uint32_t bits = 0x3F800000u | i;   // x in [1, 2)
+        float x;
+        std::memcpy(&x, &bits, 4);
+        const volatile float q = x / dh;
+        const float t = x * r.rl;
+        const float f = std::fma(x, r.rh, t);
+        ok = std::memcmp((const void *)&q, &f, 4) == 0;
+    }
+    r.ok = ok;
+    cache[key] = r;
+    return r;
+}
+
+// The float T-only quotients of a model: its DH passes ft_div and every
+// input of those quotients is 0 or at least xmin.  The inputs are
+// differences of traveltimes and quotients of such differences by DH; a
+// nonzero difference of two values >= t_min is at least t_min * 2^-24.  The
+// forward's traveltimes are 0 (source), the initial 2*TIME, or at least
+// S*DH/2 (a 1-D update adds S*DH, a 2-D one half of a root above S*DH), so
+// the host requires 2*TIME >= t_min and the slowness >= t_min/(0.48*DH)
+// (checked with every slowness upload); a caller's T (rz_eikonal_adj_shot)
+// is checked directly.
+void set_ft(ModelHost &h)
+{
+    const FtDiv d = ft_div(h.d.dh);
+    h.rdh_hi = d.rh;
+    h.rdh_lo = d.rl;
+    const double dh = h.d.dh;
+    h.t_min_ft = d.xmin * std::ldexp(1.0, 25) * std::max(1.0, dh);
+    const double smin = h.t_min_ft / (0.48 * dh);
+    h.s_min_ft = (float)std::min(smin, 1e30);
+    const double tmax = std::max(2.0 * (double)h.d.time, (double)kInf);
+    h.ft_dh = d.ok && smin < 1e30 && 2.0 * (double)h.d.time >= h.t_min_ft && tmax / dh < std::ldexp(1.0, 120);
+}
+
 int choose_R(int ny_max)
 {
     for (int R : {1, 2, 4, 8})
@@ -197,6 +269,7 @@ ModelDev make_dev(const rz_ctx *c, const ModelHost &h)
     d.rec_k = h.rec_k_d; d.rec_c = h.rec_c_d;
     d.uq_k = h.uq_k_d; d.uq_c = h.uq_c_d; d.uq_r = h.uq_r_d;
     d.max_iter = h.d.max_iter > 0 ? h.d.max_iter : 1000;
+    d.rdh_hi = h.rdh_hi; d.rdh_lo = h.rdh_lo; d.s_min_ft = h.s_min_ft;
     return d;
 }
 
@@ -208,7 +281,11 @@ int check_model(rz_ctx *c, int model)
     return RZ_OK;
 }
 
-int adj_variant(const rz_ctx *c) { return (c->fastdiv ? 1 : 0) | (c->chkinf ? 2 : 0); }
+int adj_variant(const rz_ctx *c)
+{
+    return (c->fastdiv ? kAdjFastDiv : 0) | (c->chkinf ? kAdjInfGuard : 0) |
+           (c->fastdiv && c->ft ? kAdjFloatT : 0);
+}
 
 Slots slots_of(const rz_ctx *c)
 {
@@ -655,6 +732,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         }
         h.fastdiv = fast_div_exact(d.dh);
         c->fastdiv = c->fastdiv && h.fastdiv;
+        set_ft(h);
         // T starts at 2*TIME and only decreases, so the guard TT >= INF
         // (INF = 9999999, fd.h:29-31) is dead unless 2*TIME reaches it.
         c->chkinf = c->chkinf || (float)(2.0 * (double)d.time) >= kInf;
@@ -743,6 +821,8 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
             c->ncu = 0;
         const char *env = getenv("RAJZEL_AMD_SPLIT");
         c->split = !(env && env[0] == '0');
+        const char *eft = getenv("RAJZEL_AMD_FLOAT_T");
+        c->ft_allowed = !(eft && eft[0] == '0');
     }
 
     // tables
@@ -819,11 +899,18 @@ int rz_set_slowness(rz_ctx *c, int model, const float *S, int on_device)
     int bad = 0;
     HIPCHK(c, hipMemcpyAsync(&bad, c->err_d, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (bad) {
+    if (bad & 1) {
         h.have_S = false;
         return fail(c, RZ_ERR_INVALID, "model %d: slowness must be finite and > 0 everywhere", model);
     }
     h.have_S = true;
+    h.ft_s = !(bad & 2);
+    bool ft = c->ft_allowed;
+    for (const ModelHost &x : c->m) ft = ft && x.ft_dh && (x.ft_s || !x.have_S);
+    if (ft != c->ft) {
+        c->ft = ft;
+        drop_graphs(c);   // recorded adjoint launches carry the variant
+    }
     ++c->s_epoch;
     return RZ_OK;
 }
@@ -1012,10 +1099,18 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     if (!T) return fail(c, RZ_ERR_INVALID, "null T");
     const size_t n = (size_t)h.d.nx * h.d.ny;
     bool big = false;
+    float tmin = INFINITY, tmax = 0.f;   // smallest nonzero and largest T (float T-only quotients)
     for (size_t i = 0; i < n; ++i) {
         if (!(T[i] >= 0.f)) return fail(c, RZ_ERR_INVALID, "T must be >= 0 (cell %zu)", i);
         big = big || T[i] >= kInf;
+        if (T[i] > 0.f && T[i] < tmin) tmin = T[i];
+        tmax = std::max(tmax, T[i]);
     }
+    // the caller's T, not the slowness, bounds the quotients' inputs here
+    int variant = (adj_variant(c) & ~kAdjFloatT) | (big ? kAdjInfGuard : 0);
+    if (c->ft_allowed && c->fastdiv && h.ft_dh && (double)tmin >= h.t_min_ft &&
+        (double)tmax / h.d.dh < std::ldexp(1.0, 120))
+        variant |= kAdjFloatT;
     ModelDev mh = make_dev(c, h);
     HIPCHK(c, hipMemcpyAsync(c->s_row, T, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     // receiver cells carry the sign bit in the adjoint's copy of T
@@ -1036,7 +1131,7 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     ShotOut o;
     o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1;
     HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d,
-                             c->s_tres, o, adj_variant(c) | (big ? 2 : 0), c->stream));
+                             c->s_tres, o, variant, c->stream));
     HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA, c->s_row, 0, c->stream));
     int it = 0;
     if (lam) HIPCHK(c, hipMemcpyAsync(lam, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));

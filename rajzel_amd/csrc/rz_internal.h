@@ -48,6 +48,11 @@ struct ModelDev {
     const int *uq_k, *uq_c, *uq_r; // interior receiver cells and their winning receiver
     int max_iter;
     int pad_;
+    // T-only quotients of the adjoint as fma(x, rdh_hi, x * rdh_lo) in float
+    // (adjoint variant 2); slowness below s_min_ft turns that variant off.
+    float rdh_hi, rdh_lo;
+    float s_min_ft;
+    int pad2_;
 };
 
 struct ShotDev {
@@ -89,7 +94,10 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
                           const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st);
 hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
                           const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
-                          int variant, hipStream_t st);   // variant bit 0: fast DH division, bit 1: INF guard
+                          int variant, hipStream_t st);
+// adjoint variant bits: 0 DH division by a double multiply, 1 INF guard,
+// 2 (with bit 0) T-only quotients by the float FMA of ModelDev::rdh_hi/lo
+constexpr int kAdjFastDiv = 1, kAdjInfGuard = 2, kAdjFloatT = 4;
 hipError_t launch_forward_split(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots,
                                 Slots slots, const long long *rec_off, const int *shot_base, ShotOut out,
                                 SplitSync ss, hipStream_t st);

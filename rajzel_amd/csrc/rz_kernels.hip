@@ -700,11 +700,22 @@ __device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh
 // while the double product is within 2^-52 of it.  The host also checks it
 // against the IEEE division for the model's DH before enabling it.
 // ---------------------------------------------------------------------------
-template <bool FD>
+template <int FD>
 __device__ __forceinline__ float qdh(float x, float dh, double rdh)
 {
-    if constexpr (FD) return (float)((double)x * rdh);
+    if constexpr (FD >= 1) return (float)((double)x * rdh);
     else return x / dh;
+}
+
+// x/DH for the T-only quotients (slopes and the terms of lhs).  FD == 2:
+// fma(x, rh, x * rl) with rh + rl ~ 1/DH, equal to the IEEE quotient for the
+// model's DH and for every input these quotients can see (the host checks
+// both before it selects this variant); otherwise as qdh.
+template <int FD>
+__device__ __forceinline__ float qdh_t(float x, float dh, double rdh, float rh, float rl)
+{
+    if constexpr (FD == 2) return __builtin_fmaf(x, rh, x * rl);
+    else return qdh<FD>(x, dh, rdh);
 }
 
 // (x + |x|)/2 and (x - |x|)/2 of sweep_adj.c:39-49 for finite |x| < 2^127:
@@ -738,13 +749,13 @@ __device__ __forceinline__ float neg_part(float x)
 // from T in every sweep: recomputing them costs less than the HBM traffic of
 // storing them.  The memory operations per step are fixed, as in fwd_sweep.
 // ---------------------------------------------------------------------------
-template <int R, int SX, int SY, bool LSAME, bool FD, bool CHKINF, int PACE = kPaceLock>
-__device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh_, double rdh_, float eps_,
-                                                      const float *Tf_, const float *Lin_, float *Lout_,
-                                                      float *stage_, const Pace pc_)
+template <int R, int SX, int SY, bool LSAME, int FD, bool CHKINF, int PACE = kPaceLock>
+__device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh_, double rdh_, float rh_,
+                                                      float rl_, float eps_, const float *Tf_, const float *Lin_,
+                                                      float *Lout_, float *stage_, const Pace pc_)
 {
     const WaveCtx w = uni_ctx(w_);
-    const float dh = uni(dh_), eps = uni(eps_);
+    const float dh = uni(dh_), eps = uni(eps_), rh = uni(rh_), rl = uni(rl_);
     const double rdh = uni(rdh_);
     const float *Tf = uni(Tf_), *Lin = uni(Lin_);
     float *Lout = uni(Lout_), *stage = uni(stage_);
@@ -905,7 +916,7 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
                     const float td = fabsf(r == R - 1 ? t_below : qt[u][r < R - 1 ? r + 1 : r]);
                     float app, amm;   // pos_part(ap), neg_part(am)
                     if (SX > 0) {
-                        const float ap = qdh<FD>(-(tr - tc), dh, rdh);
+                        const float ap = qdh_t<FD>(-(tr - tc), dh, rdh, rh, rl);
                         app = pos_part(ap);
                         apm[r] = neg_part(ap);
                         amp[r] = xcp[r];
@@ -913,7 +924,7 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
                         xqp[r] = app;
                         xqn[r] = apm[r];
                     } else {
-                        const float am = qdh<FD>(-(tc - tl), dh, rdh);
+                        const float am = qdh_t<FD>(-(tc - tl), dh, rdh, rh, rl);
                         amp[r] = pos_part(am);
                         amm = neg_part(am);
                         app = xcp[r];
@@ -921,7 +932,7 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
                         xqp[r] = amp[r];
                         xqn[r] = amm;
                     }
-                    const float bp = qdh<FD>(-(td - tc), dh, rdh);
+                    const float bp = qdh_t<FD>(-(td - tc), dh, rdh, rh, rl);
                     bpp[r] = pos_part(bp);
                     bpm[r] = neg_part(bp);
                     float bmm;        // neg_part(bm)
@@ -929,12 +940,12 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
                         bmp[r] = bpp[r > 0 ? r - 1 : 0];
                         bmm = bpm[r > 0 ? r - 1 : 0];
                     } else {
-                        const float bm = qdh<FD>(-(tc - tu), dh, rdh);
+                        const float bm = qdh_t<FD>(-(tc - tu), dh, rdh, rh, rl);
                         bmp[r] = pos_part(bm);
                         bmm = neg_part(bm);
                     }
                     const bool rec = __float_as_uint(traw) >> 31;
-                    lhs[r] = qdh<FD>(app - amm, dh, rdh) + qdh<FD>(bpp[r] - bmm, dh, rdh);
+                    lhs[r] = qdh_t<FD>(app - amm, dh, rdh, rh, rl) + qdh_t<FD>(bpp[r] - bmm, dh, rdh, rh, rl);
                     keep[r] = (!cvalid) | fcol | frow[r] | rec;
                     if constexpr (CHKINF) infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
                     else infv[r] = false;
@@ -1144,7 +1155,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
 // Adjoint kernel: one workgroup per shot (eikonal_adj.c:10-59 with
 // init_travel_adj.c as an O(ntr) scatter).
 // ---------------------------------------------------------------------------
-template <int R, bool FD, bool CHKINF>
+template <int R, int FD, bool CHKINF>
 __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
     const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, Slots slots,
     const long long *__restrict__ rec_off, const int *__restrict__ shot_base, const float *__restrict__ tres,
@@ -1190,11 +1201,11 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
         // 3+4 run as pairs (same x direction).
         double acc;
         if (w.group == 0) {
-            acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB, stage, lockstep(0, D));
-            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA, stage, lockstep(0, D));
+            acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, LA, LB, stage, lockstep(0, D));
+            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LA, stage, lockstep(0, D));
         } else {
-            acc = adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB, stage, lockstep(D, 0));
-            acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA, stage, lockstep(D, 0));
+            acc = adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LB, stage, lockstep(D, 0));
+            acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, LA, LA, stage, lockstep(D, 0));
         }
         const double tot = block_sum(acc, red, nwaves);
         lnorm = (float)tot / denom;
@@ -1391,7 +1402,7 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
     if (threadIdx.x == 0) out.it_f[shot_base[sh.model] + sh.shot] = it;
 }
 
-template <int R, bool FD, bool CHKINF>
+template <int R, int FD, bool CHKINF>
 __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) void k_adjoint_split(
     const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, int nshots, Slots slots,
     const long long *__restrict__ rec_off, const int *__restrict__ shot_base, const float *__restrict__ tres,
@@ -1442,19 +1453,19 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
         double acc;
         pc.base = phase * kPhaseStride;
         if (role == 0)
-            acc = adj_sweep<R, +1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LB,
+            acc = adj_sweep<R, +1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, LA, LB,
                                                                       stage, pc);
         else
-            acc = adj_sweep<R, +1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LB,
+            acc = adj_sweep<R, +1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LB,
                                                                       stage, pc);
         ok = rendezvous(cnt, role, phase++, ss.err);
         if (!ok) break;
         pc.base = phase * kPhaseStride;
         if (role == 0)
-            acc += adj_sweep<R, -1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.eps_adj, TB, LB, LA,
+            acc += adj_sweep<R, -1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LA,
                                                                        stage, pc);
         else
-            acc += adj_sweep<R, -1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.eps_adj, TA, LA, LA,
+            acc += adj_sweep<R, -1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, LA, LA,
                                                                        stage, pc);
         const double tot = pair_sum(acc, red, nwaves, part, cnt, role, phase++, ss.err, ok);
         if (!ok) break;
@@ -1629,6 +1640,7 @@ __global__ void k_prepare_slowness(const ModelDev *__restrict__ mds, int model, 
         if (k < md.ny && ca >= 0 && ca < md.nx) {
             va = md.S[(long long)k * md.nx + ca];
             if (!(va > 0.f) || !(va < INFINITY)) atomicOr(err_flag, 1);
+            if (va < md.s_min_ft) atomicOr(err_flag, 2);   // too small for the float T-only quotients
         }
         if (k < md.ny && cb >= 0 && cb < md.nx) vb = md.S[(long long)k * md.nx + cb];
         SA[p] = va;
@@ -1729,11 +1741,16 @@ hipError_t launch_adjoint_split(int R, int nb_max, const ModelDev *md, const Sho
 #define RZ_ADJ_LAUNCH(FD_, CI_)                                                                           \
     RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint_split<RR, FD_, CI_>), g, b, 0, st, md, shots, nshots, \
                                         slots, rec_off, shot_base, tres, out, ss))
-    switch (variant & 3) {
-    case 0: RZ_ADJ_LAUNCH(false, false); break;
-    case 1: RZ_ADJ_LAUNCH(true, false); break;
-    case 2: RZ_ADJ_LAUNCH(false, true); break;
-    default: RZ_ADJ_LAUNCH(true, true); break;
+    switch ((variant & kAdjFastDiv) ? ((variant & kAdjFloatT) ? 2 : 1) : 0) {   // DH division mode
+    case 0:
+        if (variant & kAdjInfGuard) { RZ_ADJ_LAUNCH(0, true); } else { RZ_ADJ_LAUNCH(0, false); }
+        break;
+    case 1:
+        if (variant & kAdjInfGuard) { RZ_ADJ_LAUNCH(1, true); } else { RZ_ADJ_LAUNCH(1, false); }
+        break;
+    default:
+        if (variant & kAdjInfGuard) { RZ_ADJ_LAUNCH(2, true); } else { RZ_ADJ_LAUNCH(2, false); }
+        break;
     }
 #undef RZ_ADJ_LAUNCH
     return hipGetLastError();
@@ -1748,11 +1765,16 @@ hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *
 #define RZ_ADJ_LAUNCH(FD_, CI_)                                                                    \
     RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, FD_, CI_>), g, b, 0, st, md, shots, slots, \
                                         rec_off, shot_base, tres, out))
-    switch (variant & 3) {
-    case 0: RZ_ADJ_LAUNCH(false, false); break;
-    case 1: RZ_ADJ_LAUNCH(true, false); break;
-    case 2: RZ_ADJ_LAUNCH(false, true); break;
-    default: RZ_ADJ_LAUNCH(true, true); break;
+    switch ((variant & kAdjFastDiv) ? ((variant & kAdjFloatT) ? 2 : 1) : 0) {   // DH division mode
+    case 0:
+        if (variant & kAdjInfGuard) { RZ_ADJ_LAUNCH(0, true); } else { RZ_ADJ_LAUNCH(0, false); }
+        break;
+    case 1:
+        if (variant & kAdjInfGuard) { RZ_ADJ_LAUNCH(1, true); } else { RZ_ADJ_LAUNCH(1, false); }
+        break;
+    default:
+        if (variant & kAdjInfGuard) { RZ_ADJ_LAUNCH(2, true); } else { RZ_ADJ_LAUNCH(2, false); }
+        break;
     }
 #undef RZ_ADJ_LAUNCH
     return hipGetLastError();
