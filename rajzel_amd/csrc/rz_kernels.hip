@@ -867,14 +867,13 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
 #pragma unroll
             for (int r = 0; r < R; ++r) nv[r] = ql[u][r];
 
-            // Split shots compute every step: an inactive step only has lanes
-            // that keep their value (cvalid is false), and without the branch
-            // the eight steps of an epoch form one block, in which the
-            // scheduler overlaps a step's T-only terms with the previous
-            // step's lambda chain (one wave per SIMD has no other wave to do
-            // that).  In a workgroup with both sweeps the other wave hides
-            // that latency, and skipping inactive steps saves their work.
-            if (PACE != kPaceLock || active) {
+            // Every step is computed: an inactive step only has lanes that
+            // keep their value (cvalid is false).  Without a branch the eight
+            // steps of an epoch form one block, in which the scheduler
+            // overlaps a step's T-only terms with the previous step's lambda
+            // chain, and the carried values need no copies at a join; both
+            // outweigh the work of the inactive steps (measured).
+            {
                 [[maybe_unused]] const bool fcol = (c <= 0) || (c >= nx - 1);
                 // T of the rows across the lane boundary, at column c
                 float t_above, t_below;
