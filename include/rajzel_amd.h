@@ -218,6 +218,21 @@ int rz_gauss_filt(int device, void *hip_stream, float *field, int nx, int ny, in
 int rz_set_split(rz_ctx *ctx, int on);
 int rz_get_split(const rz_ctx *ctx, int *active, int *failure);
 
+/* Pick file of one shot into Tobs[1..ntr] (1-based, as the reference's
+ * vectors): calc_FA_res.c:28-37, one fscanf("%e") per receiver, where a
+ * value that does not parse leaves Tobs[i] as it was.  The reference reads
+ * every shot's file on every evaluation and for every grid-search model
+ * (about 50 ms for 200 files of 1500 picks); this call keeps each file's
+ * parse and parses it again only when the file's device, inode, size or
+ * modification time, or ntr, changed (RAJZEL_AMD_PICK_CACHE=0 in the
+ * environment: always).  The values written are those a new parse writes.
+ * Host only, no context; returns 0, or -1 if the file cannot be opened
+ * (Tobs unchanged).  rz_picks_parses counts the parses so far;
+ * rz_picks_clear releases the kept values. */
+int rz_read_picks(const char *name, float *Tobs, int ntr);
+long rz_picks_parses(void);
+void rz_picks_clear(void);
+
 /* Block until the context's stream is idle. */
 int rz_sync(rz_ctx *ctx);
 

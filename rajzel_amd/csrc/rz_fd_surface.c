@@ -10,7 +10,8 @@
  *
  *   grad_obj    (fd.h:72, grad_obj.c:10-79)   every shot of this rank in one
  *               batched evaluation on the GPU; picks read from the
- *               <DATA_DIR>_<shot>.dat files as calc_FA_res.c:28-37 does;
+ *               <DATA_DIR>_<shot>.dat files as calc_FA_res.c:28-37 does
+ *               (each file parsed again only when it changed, rz_read_picks);
  *               MPI sum, frame copy and precond() as grad_obj.c:57-73
  *   forward     (fd.h:68, forward.c:10-26)    all shots of this rank in one
  *               launch; SNAP and SEISMO outputs per shot as eikonal.c:57-72
@@ -260,16 +261,13 @@ double grad_obj(float **grad, float **S, float **TT, float **lam, float *Tmod, f
         rzs_slowness(S);
         if (ntr > 0) {
             /* calc_FA_res.c:28-37: every evaluation reads every shot's pick
-             * file with fscanf("%e") into Tobs, in shot order */
+             * file with fscanf("%e") into Tobs, in shot order (a value that
+             * does not parse keeps what Tobs held) */
             float *tobs = rzs_alloc((size_t)ns * ntr * sizeof(float));
             char name[RZS_PATH];
             for (int s = 0; s < ns; ++s) {
                 snprintf(name, sizeof name, "%s_%i.dat", DATA_DIR, NSHOT1 + s);
-                FILE *fp = fopen(name, "r");
-                if (fp == NULL) err(" picks_?.dat could not be opened !");
-                for (int i = 1; i <= ntr; ++i)
-                    if (fscanf(fp, "%e", &Tobs[i]) != 1) { /* value of Tobs[i] stays, as in the reference */ }
-                fclose(fp);
+                if (rz_read_picks(name, Tobs, ntr) != 0) err(" picks_?.dat could not be opened !");
                 memcpy(tobs + (size_t)s * ntr, &Tobs[1], ntr * sizeof(float));
             }
             rzs_check(rz_set_picks(c, 0, tobs, 0), "picks");
@@ -376,10 +374,8 @@ static double rzs_calc_fa_res(const float *tm, const float *tn, float *Tobs, flo
     char name[RZS_PATH];
     double l2 = 0.0;
     snprintf(name, sizeof name, "%s_%i.dat", DATA_DIR, ishot);
-    FILE *fp = fopen(name, "r");
-    if (fp == NULL) err(" picks_?.dat could not be opened !");
+    if (rz_read_picks(name, Tobs, ntr) != 0) err(" picks_?.dat could not be opened !");
     for (int i = 1; i <= ntr; i++) {
-        if (fscanf(fp, "%e", &Tobs[i]) != 1) { /* Tobs[i] keeps its value, as in the reference */ }
         if (Tobs[i] >= 0.0) {
             Tres[i] = tm[i - 1] - Tobs[i];
             l2 += Tres[i] * Tres[i];
@@ -395,7 +391,6 @@ static double rzs_calc_fa_res(const float *tm, const float *tn, float *Tobs, flo
             }
         }
     }
-    fclose(fp);
     return l2;
 }
 

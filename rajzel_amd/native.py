@@ -25,6 +25,7 @@ EXPORTS = (
     "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
     "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb", "rz_set_split", "rz_get_split",
+    "rz_read_picks", "rz_picks_parses", "rz_picks_clear",
 )
 
 
@@ -103,6 +104,9 @@ def lib():
                 "rz_get_tnb": (i, [vp, i, vp, i]),
                 "rz_set_split": (i, [vp, i]),
                 "rz_get_split": (i, [vp, C.POINTER(i), C.POINTER(i)]),
+                "rz_read_picks": (i, [C.c_char_p, vp, i]),
+                "rz_picks_parses": (C.c_long, []),
+                "rz_picks_clear": (None, []),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -177,6 +181,28 @@ def gauss_filt(field, fsx, fsy, device=0, stream=None):
     if rc != RZ_OK:
         raise RzError(rc, lib().rz_last_error(None).decode())
     return out, x.value, y.value
+
+
+def read_picks(path, tobs):
+    """calc_FA_res.c:28-37 through rz_read_picks: the pick file at `path`
+    into tobs[1..ntr] (tobs: float32 numpy array of ntr + 1 values, index 0
+    unused, the reference's 1-based vector), in place.  Values that do not
+    parse leave their entries as they were.  Host only."""
+    if tobs.dtype != np.float32 or not tobs.flags.c_contiguous or tobs.ndim != 1 or tobs.size < 1:
+        raise ValueError("tobs must be a contiguous 1-D float32 array of ntr + 1 values")
+    rc = lib().rz_read_picks(os.fsencode(path), tobs.ctypes.data_as(C.c_void_p), tobs.size - 1)
+    if rc != 0:
+        raise OSError(f"pick file {path} could not be opened")
+    return tobs
+
+
+def picks_parses():
+    """Number of pick-file parses rz_read_picks has made in this process."""
+    return lib().rz_picks_parses()
+
+
+def picks_clear():
+    lib().rz_picks_clear()
 
 
 class Context:

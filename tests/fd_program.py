@@ -64,9 +64,10 @@ def make_workspace(sv, vp_model, invmat, picks=None, spatfilter=0, **kw):
     return ws
 
 
-def run(ws, lib, timeout=300):
-    """Run the program in ``ws``; returns the CompletedProcess."""
-    env = dict(os.environ)
+def run(ws, lib, timeout=300, env=None):
+    """Run the program in ``ws`` (``env``: extra environment variables);
+    returns the CompletedProcess."""
+    env = dict(os.environ, **(env or {}))
     return subprocess.run([LAUNCHER, lib, "rz.inp", "wf.inp"], cwd=ws, env=env, capture_output=True,
                           text=True, timeout=timeout)
 
@@ -107,8 +108,10 @@ def time_grid_search(nx=600, ny=200, dh=10.0, nshots=40, seed=31):
     """Wall time of the grid-search mode (INVMAT=2, grid_search.c) of the
     reference program and of the MI355X program on identical workspaces:
     a 200 x 600 grid, 40 shots, 600 receivers on row 2, a 4 x 4 grid of
-    (vp0, grad0) models.  Returns a dict with both times and whether the
-    grid-search tables are byte-identical."""
+    (vp0, grad0) models; the MI355X program also with every pick file
+    parsed on every read (RAJZEL_AMD_PICK_CACHE=0), as the reference does.
+    Returns a dict with the times and whether the grid-search tables are
+    byte-identical."""
     import time
     from oracle.bindings import Restatement
     sv = W.small_case(nx=nx, ny=ny, dh=dh, nshots=nshots, seed=seed, rec_row=2)
@@ -119,17 +122,60 @@ def time_grid_search(nx=600, ny=200, dh=10.0, nshots=40, seed=31):
     res = {"grid": f"{ny}x{nx}", "shots": nshots, "receivers": len(sv.rec_x_m), "models": 16}
     spaces = []
     try:
-        for key, lib in (("cpu_reference_s", CPU_LIB), ("gpu_s", GPU_LIB)):
+        # the last run parses every pick file on every read, as the reference does
+        for key, lib, flag in (("cpu_reference_s", CPU_LIB, "1"), ("gpu_s", GPU_LIB, "1"),
+                               ("gpu_parse_every_read_s", GPU_LIB, "0")):
             ws = make_workspace(sv, start_model(sv), invmat=2, picks=obs, **grid)
             spaces.append(ws)
             t0 = time.perf_counter()
-            r = run(ws, lib, timeout=600)
+            r = run(ws, lib, timeout=600, env={"RAJZEL_AMD_PICK_CACHE": flag})
             res[key] = time.perf_counter() - t0
             if r.returncode != 0:
                 raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
-        a, b = (outputs(ws)["gs.dat"] for ws in spaces)
-        res["tables_identical"] = a == b
+        a, b, c = (outputs(ws)["gs.dat"] for ws in spaces)
+        res["tables_identical"] = a == b == c
         res["speedup"] = res["cpu_reference_s"] / res["gpu_s"]
+    finally:
+        for ws in spaces:
+            remove(ws)
+    return res
+
+
+def time_fatt_picks(itermax=2):
+    """Wall time of the MI355X program's FATT inversion (INVMAT=1, L-BFGS
+    with the Wolfe line search) on the 1001 x 3001 workload of 200 shots and
+    1500 receivers, with the pick files parsed once per change (default)
+    and on every evaluation as the reference does (RAJZEL_AMD_PICK_CACHE=0).
+    Picks: the forward in the true model plus N(0, 1 ms), written as pick
+    files.  Returns both times, the number of evaluations (grad_obj calls,
+    counted from the program's output) and whether every output file of the
+    two runs is byte-identical."""
+    import time
+    from rajzel_amd import native as N
+    sv = W.c4_gradient()
+    with N.Context([dict(nx=sv.nx, ny=sv.ny, dh=sv.dh, time=sv.time, ttnorm=sv.ttnorm, eps_adj=sv.eps_adj,
+                         src_ix=sv.src_ix, src_iy=sv.src_iy, rec_ix=sv.rec_ix, rec_iy=sv.rec_iy)]) as ctx:
+        ctx.set_slowness(0, W.slowness(sv.vp_true))
+        ctx.forward()
+        tm = ctx.tmod(0)
+    obs = F.roundtrip_picks(tm + np.random.default_rng(sv.meta["seed"] + 1000).normal(0.0, sv.noise_s, tm.shape))
+    res = {"grid": f"{sv.ny}x{sv.nx}", "shots": sv.nshots, "receivers": sv.ntr, "itermax": itermax}
+    spaces, outs = [], []
+    try:
+        for key, flag in (("gpu_pick_cache_s", "1"), ("gpu_parse_every_evaluation_s", "0")):
+            ws = make_workspace(sv, sv.vp, invmat=1, picks=obs, ITERMAX=itermax, TIME=sv.time, SEISMO=0)
+            spaces.append(ws)
+            t0 = time.perf_counter()
+            r = run(ws, GPU_LIB, timeout=900, env={"RAJZEL_AMD_PICK_CACHE": flag})
+            res[key] = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            # one grad_obj per FATT iteration (fatt.c:144) and per Wolfe trial (wolfels.c:66-72)
+            res.setdefault("evaluations", r.stdout.count("FATT ITERATION") + r.stdout.count("gradient evaluation"))
+            outs.append(outputs(ws))
+        res["outputs_identical"] = outs[0] == outs[1]
+        res["saved_s_per_evaluation"] = ((res["gpu_parse_every_evaluation_s"] - res["gpu_pick_cache_s"])
+                                         / max(1, res["evaluations"]))
     finally:
         for ws in spaces:
             remove(ws)
@@ -141,5 +187,7 @@ if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["--time-grid-search"]:
         print(json.dumps(time_grid_search()), flush=True)
+    elif sys.argv[1:] == ["--time-fatt-picks"]:
+        print(json.dumps(time_fatt_picks()), flush=True)
     else:
-        sys.exit("usage: python -m tests.fd_program --time-grid-search")
+        sys.exit("usage: python -m tests.fd_program --time-grid-search | --time-fatt-picks")
