@@ -3,10 +3,10 @@
  * path, on top of the MI355X C ABI (rajzel_amd.h).
  *
  * Linked into the reference program in place of eikonal.c, sweep.c,
- * eikonal_adj.c, sweep_adj.c, ass_grad.c, grad_obj.c and forward.c, it
- * keeps their fd.h signatures, their globals and their side effects, so
- * that fatt.c, LBFGS.c, PCG.c, wolfels.c, parabolicls.c, step_length_est.c,
- * grid_search.c and rajzel.c call it unchanged:
+ * eikonal_adj.c, sweep_adj.c, ass_grad.c, grad_obj.c, forward.c,
+ * gauss_filt.c, grid_search.c and LBFGS.c, it keeps their fd.h signatures,
+ * their globals and their side effects, so that fatt.c, PCG.c, wolfels.c,
+ * parabolicls.c, step_length_est.c and rajzel.c call it unchanged:
  *
  *   grad_obj    (fd.h:72, grad_obj.c:10-79)   every shot of this rank in one
  *               batched evaluation on the GPU; picks read from the
@@ -26,6 +26,8 @@
  *               gradient (precond.c:35-37, SPATFILTER = 4) and of the
  *               search direction (fatt.c:179), on the GPU; linked in
  *               place of gauss_filt.c
+ *   LBFGS       (fd.h:88, LBFGS.c:12-226)  the reference's arithmetic,
+ *               with each history file read or written in one call
  *
  * sweep() and sweep_adj() (fd.h:145,147) have no callers outside
  * eikonal.c and eikonal_adj.c; their work happens inside the kernels.
@@ -46,7 +48,8 @@
 /* globals of the reference (globvar.h:12-29,38,69,96) */
 extern float DH, TIME, TTNORM, EPS_ADJ;
 extern int NX, NY, SEISMO, SNAP, MYID, INFO, NSHOT1, NSHOT2, SWS_TAPER_CIRCULAR_PER_SHOT;
-extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1;
+extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1, IDX, IDY;
+extern char JACOBIAN[];
 extern float VP0_1, VP0_2, DVP0, GRAD0_1, GRAD0_2, DGRAD0;
 extern char SNAP_FILE[], DATA_DIR[];
 
@@ -61,6 +64,8 @@ double calc_FA_res(float **TT, float *Tmod, float *Tobs, float *Tres, int ntr, i
 void calc_S(float **Vp, float **S);
 void model_gridsearch(float **Vp, float vp0, float grad0);
 void write_gridsearch(float L2, float vp0, float grad0, int count);
+float dotp(float *vec1, float *vec2, int n1, int n2, int sw);
+void exchange_grad_MPI(float **grad);
 
 #define RZS_PATH 74                 /* STRING_SIZE of fd.h:25 */
 
@@ -500,4 +505,124 @@ void grid_search(float **Vp, float **S, float **TT, float *Tmod, float *Tobs, fl
     if (batched && nobj > 0) rzs_grid_check(c, rz_get_field(c, c_models - 1, ns - 1, 0, &TT[1][1], 0), "traveltimes");
     rz_destroy(c);
     free(Sall); free(tm); free(tn); free(vp0s); free(grad0s);
+}
+
+/* ---------------------------------------------------------------------------
+ * LBFGS.c:12-226 (fd.h:88): the two-loop recursion of fatt.c:160 with the
+ * reference's arithmetic (its dotp() for every dot product, the same float
+ * operations in the same order on the caller's history arrays), and the
+ * previous model and gradient exchanged through the same files
+ * <JACOBIAN>_p_vp.old, _p.old and _c.old.  The reference reads and writes
+ * those files one float per fread/fwrite call, about 80 ms per 12 MB file;
+ * here each file is one fread or fwrite of a buffer in the same order (x
+ * slow, y fast, strides IDX and IDY).  A file shorter than the grid leaves
+ * the rest as the reference's fread leaves it: the last value read.
+ * --------------------------------------------------------------------------- */
+static size_t rzs_strided(void)
+{
+    return (size_t)((NX + IDX - 1) / IDX) * (size_t)((NY + IDY - 1) / IDY);
+}
+
+/* matrix <-> buffer in the reference's file order */
+static void rzs_gather(float **A, float *buf)
+{
+    size_t t = 0;
+    for (int i = 1; i <= NX; i += IDX)
+        for (int j = 1; j <= NY; j += IDY) buf[t++] = A[j][i];
+}
+
+static void rzs_scatter(const float *buf, float **A)
+{
+    size_t t = 0;
+    for (int i = 1; i <= NX; i += IDX)
+        for (int j = 1; j <= NY; j += IDY) A[j][i] = buf[t++];
+}
+
+static void rzs_read_old(const char *suffix, float *buf, size_t n)
+{
+    char name[RZS_PATH + 16];
+    snprintf(name, sizeof name, "%s%s", JACOBIAN, suffix);
+    FILE *fp = fopen(name, "rb");
+    if (fp == NULL) {
+        snprintf(g_msg, sizeof g_msg, " LBFGS: %s could not be opened", name);
+        err(g_msg);
+    }
+    const size_t got = fread(buf, sizeof(float), n, fp);
+    fclose(fp);
+    for (size_t t = got; t < n; ++t) buf[t] = got ? buf[got - 1] : 0.f;
+}
+
+static void rzs_write_old(const char *suffix, const float *buf, size_t n)
+{
+    char name[RZS_PATH + 16];
+    snprintf(name, sizeof name, "%s%s", JACOBIAN, suffix);
+    FILE *fp = fopen(name, "wb");
+    if (fp == NULL || fwrite(buf, sizeof(float), n, fp) != n) {
+        snprintf(g_msg, sizeof g_msg, " LBFGS: %s could not be written", name);
+        err(g_msg);
+    }
+    fclose(fp);
+}
+
+void LBFGS(float **Hgrad, float **grad, float **gradm, int iter, float *y_LBFGS, float *s_LBFGS, float *rho_LBFGS,
+           float *alpha_LBFGS, float **Vp, float *q_LBFGS, float *r_LBFGS, float *beta_LBFGS, int LBFGS_pointer,
+           int NLBFGS, int NLBFGS_vec)
+{
+    (void)beta_LBFGS;   /* the reference keeps beta in a local */
+    if (MYID == 0) {
+        const size_t n = rzs_strided();
+        float *buf = rzs_alloc(2 * n * sizeof(float));
+        float *old = buf + n;
+        if (iter > 1) {
+            /* y and s of this pair into the slot LBFGS_pointer (LBFGS.c:43-66) */
+            const int h0 = NLBFGS_vec * (LBFGS_pointer - 1) + 1;
+            rzs_read_old("_p.old", old, n);
+            rzs_gather(grad, buf);
+            for (size_t t = 0; t < n; ++t) y_LBFGS[h0 + t] = buf[t] - old[t];
+            rzs_read_old("_p_vp.old", old, n);
+            rzs_gather(Vp, buf);
+            for (size_t t = 0; t < n; ++t) s_LBFGS[h0 + t] = buf[t] - old[t];
+            /* initial Hessian scale and rho of every slot (LBFGS.c:68-96) */
+            const float gamma = dotp(y_LBFGS, s_LBFGS, h0, h0 + NLBFGS_vec - 1, 0) /
+                                dotp(y_LBFGS, y_LBFGS, h0, h0 + NLBFGS_vec - 1, 0);
+            printf("MYID = %d \n", MYID);
+            printf("gamma_LBFGS = %e \n", gamma);
+            for (int k = 1; k <= NLBFGS; ++k) {
+                const int a = NLBFGS_vec * (k - 1) + 1;
+                const float ys = dotp(y_LBFGS, s_LBFGS, a, a + NLBFGS_vec - 1, 0);
+                rho_LBFGS[k] = fabs(ys) > 0.0 ? (float)(1.0 / ys) : 0.0f;
+                printf("rho_LBFGS = %e of k = %d \n", rho_LBFGS[k], k);
+            }
+            /* first loop, newest slot last in the reference's order k = NLBFGS..1 (LBFGS.c:98-125) */
+            rzs_gather(gradm, q_LBFGS + 1);
+            for (int k = NLBFGS; k >= 1; --k) {
+                const int a = NLBFGS_vec * (k - 1) + 1;
+                alpha_LBFGS[k] = rho_LBFGS[k] * dotp(s_LBFGS, q_LBFGS, a, a + NLBFGS_vec - 1, 1);
+                const float *yk = y_LBFGS + a - 1;
+                for (int i = 1; i <= NLBFGS_vec; ++i) q_LBFGS[i] = q_LBFGS[i] - alpha_LBFGS[k] * yk[i];
+            }
+            for (int i = 1; i <= NLBFGS_vec; ++i) r_LBFGS[i] = gamma * q_LBFGS[i];   /* LBFGS.c:128-131 */
+            /* second loop (LBFGS.c:133-149) */
+            for (int k = 1; k <= NLBFGS; ++k) {
+                const int a = NLBFGS_vec * (k - 1) + 1;
+                const float beta = rho_LBFGS[k] * dotp(y_LBFGS, r_LBFGS, a, a + NLBFGS_vec - 1, 1);
+                const float *sk = s_LBFGS + a - 1;
+                for (int i = 1; i <= NLBFGS_vec; ++i) r_LBFGS[i] = r_LBFGS[i] + sk[i] * (alpha_LBFGS[k] - beta);
+            }
+            rzs_scatter(r_LBFGS + 1, Hgrad);                 /* LBFGS.c:151-161 */
+        } else if (iter == 1) {
+            rzs_gather(gradm, buf);                          /* LBFGS.c:165-174 */
+            rzs_scatter(buf, Hgrad);
+        }
+        /* this iteration's model, gradient and direction (LBFGS.c:178-216) */
+        rzs_gather(Vp, buf);
+        rzs_write_old("_p_vp.old", buf, n);
+        rzs_gather(grad, buf);
+        rzs_write_old("_p.old", buf, n);
+        rzs_gather(Hgrad, buf);
+        rzs_write_old("_c.old", buf, n);
+        free(buf);
+    }
+    MPI_Barrier(MPI_COMM_WORLD);                             /* LBFGS.c:221-223 */
+    exchange_grad_MPI(Hgrad);
 }

@@ -107,6 +107,21 @@ def test_fatt_lbfgs_wolfe():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kw", [dict(ITERMAX=5, NLBFGS=2), dict(ITERMAX=3, IDX=2, IDY=3)])
+def test_fatt_lbfgs_history(kw):
+    """The shim's LBFGS (LBFGS.c with bulk history files): slots that wrap
+    around (NLBFGS = 2 over 5 iterations) and strided history (IDX, IDY);
+    the history files, models and misfit log equal the reference's."""
+    sv = P.survey()
+    obs = _observed(sv)
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, **kw))
+    for f in ("jac/grad_p.old", "jac/grad_p_vp.old", "jac/grad_c.old"):
+        assert f in out
+    n = ((sv.nx + kw.get("IDX", 1) - 1) // kw.get("IDX", 1)) * ((sv.ny + kw.get("IDY", 1) - 1) // kw.get("IDY", 1))
+    assert len(out["jac/grad_c.old"]) == 4 * n
+
+
+@pytest.mark.gpu
 def test_fatt_pcg_parabolic_line_search():
     """PCG.c directions with the parabolic line search (parabolicls.c)."""
     sv = P.survey()
