@@ -4,9 +4,10 @@
  *
  * Linked into the reference program in place of eikonal.c, sweep.c,
  * eikonal_adj.c, sweep_adj.c, ass_grad.c, grad_obj.c, forward.c,
- * gauss_filt.c, grid_search.c and LBFGS.c, it keeps their fd.h signatures,
- * their globals and their side effects, so that fatt.c, PCG.c, wolfels.c,
- * parabolicls.c, step_length_est.c and rajzel.c call it unchanged:
+ * gauss_filt.c, grid_search.c, LBFGS.c, readmod.c and writemod.c, it keeps
+ * their fd.h signatures, their globals and their side effects, so that
+ * fatt.c, PCG.c, wolfels.c, parabolicls.c, step_length_est.c and rajzel.c
+ * call it unchanged:
  *
  *   grad_obj    (fd.h:72, grad_obj.c:10-79)   every shot of this rank in one
  *               batched evaluation on the GPU; picks read from the
@@ -28,6 +29,7 @@
  *               place of gauss_filt.c
  *   LBFGS       (fd.h:88, LBFGS.c:12-226)  the reference's arithmetic,
  *               with each history file read or written in one call
+ *   readmod, writemod (fd.h:127,170)  binary models in one call
  *
  * sweep() and sweep_adj() (fd.h:145,147) have no callers outside
  * eikonal.c and eikonal_adj.c; their work happens inside the kernels.
@@ -49,7 +51,8 @@
 extern float DH, TIME, TTNORM, EPS_ADJ;
 extern int NX, NY, SEISMO, SNAP, MYID, INFO, NSHOT1, NSHOT2, SWS_TAPER_CIRCULAR_PER_SHOT;
 extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1, IDX, IDY;
-extern char JACOBIAN[];
+extern char JACOBIAN[], MFILE[];
+extern FILE *FP;
 extern float VP0_1, VP0_2, DVP0, GRAD0_1, GRAD0_2, DGRAD0;
 extern char SNAP_FILE[], DATA_DIR[];
 
@@ -65,6 +68,7 @@ void calc_S(float **Vp, float **S);
 void model_gridsearch(float **Vp, float vp0, float grad0);
 void write_gridsearch(float L2, float vp0, float grad0, int count);
 float dotp(float *vec1, float *vec2, int n1, int n2, int sw);
+void writedsk(FILE *fp_out, float amp, int format);
 void exchange_grad_MPI(float **grad);
 
 #define RZS_PATH 74                 /* STRING_SIZE of fd.h:25 */
@@ -625,4 +629,60 @@ void LBFGS(float **Hgrad, float **grad, float **gradm, int iter, float *y_LBFGS,
     }
     MPI_Barrier(MPI_COMM_WORLD);                             /* LBFGS.c:221-223 */
     exchange_grad_MPI(Hgrad);
+}
+
+/* ---------------------------------------------------------------------------
+ * Model files: writemod.c:10-33 (fd.h:170) and readmod.c:10-49 (fd.h:127)
+ * with the same log lines and the same bytes, binary fields (format 3) in
+ * one fwrite or fread instead of one call per float (about 80 ms per 12 MB
+ * model; the program writes a model per iteration).  Other formats go
+ * through the reference's writedsk() value by value, as there.
+ * --------------------------------------------------------------------------- */
+void writemod(char modfile[], float **array, int format)
+{
+    fprintf(FP, "\n\n PE %d is writing model to \n", MYID);
+    fprintf(FP, "\t%s\n\n", modfile);
+    FILE *fp = fopen(modfile, "w");
+    if (fp == NULL) {
+        snprintf(g_msg, sizeof g_msg, " writemod: %s could not be opened", modfile);
+        err(g_msg);
+    }
+    if (format == 3) {
+        const size_t n = rzs_strided();
+        float *buf = rzs_alloc(n * sizeof(float));
+        rzs_gather(array, buf);
+        const size_t put = fwrite(buf, sizeof(float), n, fp);
+        free(buf);
+        if (put != n) {
+            snprintf(g_msg, sizeof g_msg, " writemod: %s could not be written", modfile);
+            err(g_msg);
+        }
+    } else {
+        for (int i = 1; i <= NX; i += IDX)
+            for (int j = 1; j <= NY; j += IDY) writedsk(fp, array[j][i], format);
+    }
+    fclose(fp);
+}
+
+void readmod(float **Vp)
+{
+    char filename[RZS_PATH + 16];
+    fprintf(FP, "\n... reading P-wave velocity models from file...\n");
+    fprintf(FP, "\t Vp:\n\t %s.vp\n\n", MFILE);
+    snprintf(filename, sizeof filename, "%s.vp", MFILE);
+    FILE *fp = fopen(filename, "r");
+    if (fp == NULL) err(" Could not open model file for Vp ! ");
+    /* every cell, x slow and y fast; past the end of a short file the
+     * reference's fread leaves the last value read */
+    const size_t n = (size_t)NX * NY;
+    float *buf = rzs_alloc(n * sizeof(float));
+    const size_t got = fread(buf, sizeof(float), n, fp);
+    fclose(fp);
+    for (size_t t = got; t < n; ++t) buf[t] = got ? buf[got - 1] : 0.f;
+    size_t t = 0;
+    for (int i = 1; i <= NX; ++i)
+        for (int j = 1; j <= NY; ++j) Vp[j][i] = buf[t++];
+    free(buf);
+    snprintf(filename, sizeof filename, "%s.rajzel.vp", MFILE);   /* readmod.c:45-46 */
+    writemod(filename, Vp, 3);
 }
