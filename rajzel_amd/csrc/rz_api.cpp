@@ -414,10 +414,22 @@ int plan_mixed(rz_ctx *c, int per)
         std::vector<int> ord(T);
         for (int i = 0; i < T; ++i) ord[i] = i;
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return (*h)[a] > (*h)[b]; });
-        std::vector<char> in(T, 0);
-        for (int i = 0; i < ns; ++i) in[ord[i]] = 1;
-        for (int i = 0; i < T; ++i) if (in[i]) ids.push_back(i);       // split, in shot order
-        for (int i = 0; i < T; ++i) if (!in[i]) ids.push_back(i);      // the rest
+        // Workgroup w of a launch runs on XCD w mod 8 (a split shot's two at
+        // list position j run on XCD j mod 8).  The slowest shot of the phase
+        // goes first, to XCD 0, and the one-workgroup list puts its fastest
+        // shots on the positions that land on XCD 0: they finish early, and
+        // the slowest shot shares that XCD's L2 with less work for the rest
+        // of its run (measured 0.5-1 ms per evaluation at 200 shots).  The
+        // order of shots in a launch does not change any result.
+        for (int i = 0; i < ns; ++i) ids.push_back(ord[i]);            // split, slowest first
+        const int nr = T - ns;
+        std::vector<int> pos(nr, -1);
+        int fast = T - 1;                                              // ord[ns, T): slowest to fastest
+        for (int p = 0; p < nr; p += 8) pos[p] = ord[fast--];
+        std::vector<int> others(ord.begin() + ns, ord.begin() + fast + 1);
+        std::sort(others.begin(), others.end());                       // the others in shot order
+        size_t o = 0;
+        for (int p = 0; p < nr; ++p) ids.push_back(pos[p] >= 0 ? pos[p] : others[o++]);
     }
     if (!c->side) {
         HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
