@@ -1125,13 +1125,14 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
         variant |= kAdjFloatT;
     ModelDev mh = make_dev(c, h);
     HIPCHK(c, hipMemcpyAsync(c->s_row, T, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    // receiver cells carry the sign bit in the adjoint's copy of T
+    // receiver and frame cells carry the sign bit in the adjoint's copy of T
+    // (as mark_kept_cells sets it after a forward)
     std::vector<float> marked(T, T + n);
-    for (size_t u = 0; u < h.uq_k.size(); ++u) {
-        float &v = marked[(size_t)h.uq_k[u] * h.d.nx + h.uq_c[u]];
-        v = -v;
-        if (v == 0.f) v = -0.f;
-    }
+    const int nx = h.d.nx, ny = h.d.ny;
+    auto keep = [&](size_t i) { marked[i] = -std::fabs(marked[i]); };   // T >= 0: -T, and -0 for 0
+    for (size_t u = 0; u < h.uq_k.size(); ++u) keep((size_t)h.uq_k[u] * nx + h.uq_c[u]);
+    for (int c = 0; c < nx; ++c) { keep((size_t)c); keep((size_t)(ny - 1) * nx + c); }
+    for (int k = 0; k < ny; ++k) { keep((size_t)k * nx); keep((size_t)k * nx + nx - 1); }
     HIPCHK(c, hipMemcpyAsync(c->s_row2, marked.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_rowmajor_to_field(&mh, c->s_row2, c->TA, 0, c->stream));
     HIPCHK(c, launch_rowmajor_to_field(&mh, c->s_row2, c->TB, 1, c->stream));

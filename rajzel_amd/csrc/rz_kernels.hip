@@ -740,7 +740,8 @@ __device__ __forceinline__ float neg_part(float x)
 // ---------------------------------------------------------------------------
 // Adjoint sweep (sweep_adj.c:10-77) in direction (SX, SY).
 //   Tf    converged traveltime in this sweep's family layout; the sign bit
-//         marks receiver cells (skipped, sweep_adj.c:30)
+//         marks the cells that keep their value: receivers
+//         (sweep_adj.c:30) and the frame (mark_kept_cells)
 //   Lin   lambda from the previous sweep, in this sweep's layout
 //   Lout  lambda written: in this sweep's layout (LSAME, Lout == Lin, the
 //         sweep updates it in place) or the other family's, through the
@@ -786,13 +787,9 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     const int sj = l % SP, sjl = ((SX > 0) == FA) ? SP - 1 : 0;
     const int dl = 2 * (sjl > sj ? sjl - sj : sj - sjl) + 1;
 
-    bool rvalid[R], frow[R];
+    bool rvalid[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int k = k0 + r;
-        rvalid[r] = k < ny;
-        frow[r] = (k == 0) || (k >= ny - 1);
-    }
+    for (int r = 0; r < R; ++r) rvalid[r] = k0 + r < ny;
     // Rows across the band boundaries: above lane 0 (band beta-1, lane 63,
     // r = R-1) and below lane 63 (band beta+1, lane 0, r = 0); each address
     // is the same for the whole wave.  T on both sides, lambda (old) on the
@@ -874,7 +871,6 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
             // chain, and the carried values need no copies at a join; both
             // outweigh the work of the inactive steps (measured).
             {
-                [[maybe_unused]] const bool fcol = (c <= 0) || (c >= nx - 1);
                 // T of the rows across the lane boundary, at column c
                 float t_above, t_below;
                 if (SY > 0) {   // lane l-1 is one step ahead, lane l+1 one step behind
@@ -943,9 +939,8 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
                         bmp[r] = pos_part(bm);
                         bmm = neg_part(bm);
                     }
-                    const bool rec = __float_as_uint(traw) >> 31;
                     lhs[r] = qdh_t<FD>(app - amm, dh, rdh, rh, rl) + qdh_t<FD>(bpp[r] - bmm, dh, rdh, rh, rl);
-                    keep[r] = (!cvalid) | fcol | frow[r] | rec;
+                    keep[r] = __float_as_uint(traw) >> 31;   // receiver or frame (mark_kept_cells)
                     if constexpr (CHKINF) infv[r] = fmaxf(fmaxf(fmaxf(tc, td), fmaxf(tu, tr)), tl) >= kInf;
                     else infv[r] = false;
                 }
@@ -1036,6 +1031,32 @@ __device__ void copy_a_to_b(const WaveCtx &w, const float *fa, float *fb)
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) stv<R>(dst + min(g + u, nkeys) * BR + l * R, v[u]);
+    }
+}
+
+// The cells whose adjoint value never changes carry the sign bit in the
+// adjoint's copy of T (T >= +0 everywhere, so the bit is free): interior
+// receiver cells (sweep_adj.c:30) and the frame, rows 0 and NY-1 and
+// columns 0 and NX-1 (sweep_adj.c:26-28, the loops run over the interior).
+// A sweep's keep test is then one sign test per cell.  Cells outside the
+// grid (skew keys, rows past NY) are never read by a grid cell's update, so
+// what the adjoint computes for them is never used.
+template <int R>
+__device__ void mark_kept_cells(const ModelDev &md, float *TA, float *TB)
+{
+    const int nx = md.nx, ny = md.ny, nframe = 2 * nx + 2 * ny;
+    for (int u = threadIdx.x; u < md.nuq + nframe; u += blockDim.x) {
+        int k, c;
+        if (u < md.nuq) {
+            k = md.uq_k[u]; c = md.uq_c[u];
+        } else {
+            const int v = u - md.nuq;
+            if (v < 2 * nx) { k = v < nx ? 0 : ny - 1; c = v < nx ? v : v - nx; }
+            else { const int w = v - 2 * nx; k = w < ny ? w : w - ny; c = w < ny ? 0 : nx - 1; }
+        }
+        const int pa = addr_a<R>(k, c, md.bstride), pb = addr_b<R>(k, c, md.bstride);
+        TA[pa] = __uint_as_float(__float_as_uint(TA[pa]) | 0x80000000u);
+        TB[pb] = __uint_as_float(__float_as_uint(TB[pb]) | 0x80000000u);
     }
 }
 
@@ -1140,13 +1161,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         out.tnb[ob + i] = TA[addr_a<R>(kn, c, md.bstride)];
     }
     __syncthreads();
-    // mark interior receiver cells for the adjoint (sign bit; T >= +0 everywhere)
-    for (int u = threadIdx.x; u < md.nuq; u += blockDim.x) {
-        const int k = md.uq_k[u], c = md.uq_c[u];
-        const int pa = addr_a<R>(k, c, md.bstride), pb = addr_b<R>(k, c, md.bstride);
-        TA[pa] = __uint_as_float(__float_as_uint(TA[pa]) | 0x80000000u);
-        TB[pb] = __uint_as_float(__float_as_uint(TB[pb]) | 0x80000000u);
-    }
+    mark_kept_cells<R>(md, TA, TB);
     if (threadIdx.x == 0) out.it_f[shot_base[sh.model] + sh.shot] = it;
 }
 
@@ -1392,12 +1407,7 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
         out.tnb[ob + i] = TA[addr_a<R>(kn, c, md.bstride)];
     }
     __syncthreads();
-    for (int u = threadIdx.x; u < md.nuq; u += blockDim.x) {
-        const int k = md.uq_k[u], c = md.uq_c[u];
-        const int pa = addr_a<R>(k, c, md.bstride), pb = addr_b<R>(k, c, md.bstride);
-        TA[pa] = __uint_as_float(__float_as_uint(TA[pa]) | 0x80000000u);
-        TB[pb] = __uint_as_float(__float_as_uint(TB[pb]) | 0x80000000u);
-    }
+    mark_kept_cells<R>(md, TA, TB);
     if (threadIdx.x == 0) out.it_f[shot_base[sh.model] + sh.shot] = it;
 }
 
