@@ -54,6 +54,38 @@ typedef float vf4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) vf2 gvf2;
 typedef __attribute__((address_space(1))) vf4 gvf4;
 
+// Buffer loads over a whole field: a raw buffer descriptor (gfx9 format,
+// dword 3 = 0x00020000) with the field's size, the lane's byte offset in a
+// VGPR and the wave-uniform part in an SGPR, which keeps per-step address
+// arithmetic 32-bit and scalar.  Callers pass in-range offsets only.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float ldb1(__amdgpu_buffer_rsrc_t r, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, 0, soff, 0));
+}
+
+template <int R>
+__device__ __forceinline__ void ldb(__amdgpu_buffer_rsrc_t r, int voff, int soff, float (&o)[R])
+{
+    if constexpr (R == 1) {
+        o[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    } else if constexpr (R == 2) {
+        const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        o[0] = __uint_as_float(t[0]); o[1] = __uint_as_float(t[1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; i += 4) {
+            const auto t = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 4 * i, soff, 0);
+            o[i] = __uint_as_float(t[0]); o[i + 1] = __uint_as_float(t[1]);
+            o[i + 2] = __uint_as_float(t[2]); o[i + 3] = __uint_as_float(t[3]);
+        }
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void ldv(const gfloat *p, float (&o)[R])
 {
@@ -776,9 +808,12 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     const int lag = SY > 0 ? l : 63 - l;
     const int off = (SY > 0 ? beta : nb - 1 - beta) * kBandOff;
     const long long boff = (long long)beta * w.bstride;
-    const gfloat *tband = gp(Tf) + boff;
-    const gfloat *lband = gp(Lin) + boff;
     gfloat *gLout = gpw(Lout) + boff;
+    // T and lambda (old) through buffer loads over the whole field: the
+    // lane's byte offset is fixed, the step's is scalar and 32-bit
+    const unsigned fbytes = 4u * (unsigned)(nb * w.bstride);
+    const __amdgpu_buffer_rsrc_t rT = field_rsrc(Tf, fbytes), rL = field_rsrc(Lin, fbytes);
+    const int vlane = 4 * l * R, sband = 4 * (int)boff;
     lfloat *lstage = (lfloat *)stage + l * R;
     const int k0 = beta * BR + l * R;
     // key of this lane's cells in the other family, and the delay of its
@@ -796,8 +831,8 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     // downstream side.
     const int lag0 = SY > 0 ? 0 : 63, lag63 = SY > 0 ? 63 : 0;
     const bool has_up = beta > 0, has_dn = beta + 1 < nb;
-    const long long up_base = has_up ? -w.bstride + 63 * R + R - 1 : 0;
-    const long long dn_base = has_dn ? w.bstride : 0;
+    const int up_base = has_up ? -(int)w.bstride + 63 * R + R - 1 : 0;
+    const int dn_base = has_dn ? (int)w.bstride : 0;
     const int ering = clampi(SY > 0 ? beta - 1 : beta + 1, 0, kMaxBands - 1);
     const bool ring_writer = (SY > 0 ? (l == 63) : (l == 0)) && (SY > 0 ? has_dn : has_up);
     lfloat *ring = (lfloat *)w.ring;
@@ -809,17 +844,15 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     do {                                                                                    \
         const int tau_ = (TAU);                                                             \
         const int key_ = clampi(SX > 0 ? tau_ : nx + 62 - tau_, 0, nkeys - 1);              \
-        ldv<R>(tband + key_ * BR + l * R, qt[SLOT]);                                        \
-        ldv<R>(lband + key_ * BR + l * R, ql[SLOT]);                                        \
+        ldb<R>(rT, vlane, sband + 4 * BR * key_, qt[SLOT]);                                 \
+        ldb<R>(rL, vlane, sband + 4 * BR * key_, ql[SLOT]);                                 \
         const int c0_ = clampi(SX > 0 ? tau_ - lag0 : nx - 1 - tau_ + lag0, 0, nx - 1);     \
         const int c63_ = clampi(SX > 0 ? tau_ - lag63 : nx - 1 - tau_ + lag63, 0, nx - 1);  \
-        const long long eu_ = up_base + (long long)(FA ? c0_ + 63 : c0_) * BR;              \
-        const long long ed_ = dn_base + (long long)(FA ? c63_ : c63_ + 63) * BR;            \
-        qeu[SLOT] = tband[eu_];                                                             \
-        qed[SLOT] = tband[ed_];                                                             \
-        const long long el_ = SY > 0 ? dn_base + (long long)(FA ? c63_ : c63_ + 63) * BR    \
-                                     : up_base + (long long)(FA ? c0_ + 63 : c0_) * BR;     \
-        qel[SLOT] = lband[el_];                                                             \
+        const int eu_ = sband + 4 * (up_base + (FA ? c0_ + 63 : c0_) * BR);                \
+        const int ed_ = sband + 4 * (dn_base + (FA ? c63_ : c63_ + 63) * BR);               \
+        qeu[SLOT] = ldb1(rT, eu_);                                                          \
+        qed[SLOT] = ldb1(rT, ed_);                                                          \
+        qel[SLOT] = ldb1(rL, SY > 0 ? ed_ : eu_);                                           \
     } while (0)
 
     // Slot u holds the blocks of step u (mod kPF).  A step reads its own
