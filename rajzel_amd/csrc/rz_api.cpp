@@ -716,6 +716,11 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         h.nb = bands_for(d.ny, c->R);
         h.bstride = (long long)(d.nx + 64) * kWave * c->R;   // keys 0..nx+62 + one dummy block
         field_max = std::max(field_max, (long long)h.nb * h.bstride);
+        // the adjoint reads a field through one buffer descriptor, whose
+        // byte range is a 31-bit count
+        if (4 * (long long)h.nb * h.bstride > 0x7fffffffLL)
+            return bail(fail(c, RZ_ERR_INVALID, "model %d: a %d x %d field exceeds 2 GiB in the band layout", i,
+                             d.nx, d.ny));
         for (int s = 0; s < d.nshots; ++s) {
             const int x = d.src_ix[s], y = d.src_iy[s];
             if (x < 0 || x >= d.nx || y < 0 || y >= d.ny)
