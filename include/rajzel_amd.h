@@ -205,6 +205,24 @@ int rz_set_graph(rz_ctx *ctx, int on);
  * reported by rz_last_error(NULL). */
 int rz_gauss_filt(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy, int on_device);
 
+/* rz_gauss_filt on a device field without any device allocation.  No
+ * counterpart in the reference: it serves callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The weights and the copy of the input live in a
+ * static device array of RZ_GAUSS_NOALLOC_FLOATS floats (32 MiB), which
+ * each device the call runs on holds until the process ends: weights at
+ * its start (at most RZ_GAUSS_NOALLOC_WEIGHTS of them, i.e. windows up to
+ * 127 x 127), the field after them, so NX*NY may be at most
+ * RZ_GAUSS_NOALLOC_FLOATS - RZ_GAUSS_NOALLOC_WEIGHTS cells (the
+ * 1001 x 3001 grid fits).  Larger inputs are rejected with RZ_ERR_INVALID
+ * and nothing changed.  Results, size updates and errors are otherwise
+ * those of rz_gauss_filt with on_device = 1.  All calls on one device share
+ * that device's array: calls that overlap on one device race (the call
+ * returns when its work on hip_stream is done). */
+#define RZ_GAUSS_NOALLOC_FLOATS (8u << 20)
+#define RZ_GAUSS_NOALLOC_WEIGHTS (16u << 10)
+int rz_gauss_filt_noalloc(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy);
+
 /* Split shots (on by default; RAJZEL_AMD_SPLIT=0 in the environment turns
  * the default off): a launch with at most one shot per two CUs runs each shot's
  * two paired sweeps in two workgroups on two CUs of one XCD, coupled

@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "../../include/rajzel_amd.h"
@@ -72,31 +73,28 @@ int fail_global(int code, const char *fmt, ...)
     return code;
 }
 
-}  // namespace
-}  // namespace rz
-
-extern "C" int rz_gauss_filt(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy,
-                             int on_device)
+// gauss_filt.c:27-50: a size of 0 leaves the field as it is (returns 0);
+// an even size is made odd.  Returns 1 with the half sizes set otherwise.
+int normalise_sizes(int *fsx, int *fsy, int &hx, int &hy)
 {
-    using namespace rz;
-    if (!field || !fsx || !fsy) return fail_global(RZ_ERR_INVALID, "gauss_filt: null pointer");
-    if (nx < 1 || ny < 1) return fail_global(RZ_ERR_INVALID, "gauss_filt: NX, NY must be >= 1");
-    if (ny > 65535 * kGY * kQY || (long long)nx * ny > (1LL << 31))
-        return fail_global(RZ_ERR_INVALID, "gauss_filt: grid too large");
-    // gauss_filt.c:27-41: a size of 0 returns; an even size is made odd
-    if (*fsx == 0) return RZ_OK;
+    if (*fsx == 0) return 0;
     if (!(*fsx % 2)) *fsx += *fsx > 0 ? 1 : -1;
-    if (*fsy == 0) return RZ_OK;
+    if (*fsy == 0) return 0;
     if (!(*fsy % 2)) *fsy += *fsy > 0 ? 1 : -1;
-    const int hx = std::abs(*fsx) / 2, hy = std::abs(*fsy) / 2;
-    if (hx > 4096 || hy > 4096) return fail_global(RZ_ERR_INVALID, "gauss_filt: filter size above 8193");
-    // gauss_filt.c:44-50, 101-120: sigma = hfs/2 (integer halves), weights
-    // exp(-ii^2/sx - jj^2/sy) in float, their float sum, and the division by
-    // it over 1..FILT_SIZE (none for negative sizes)
+    hx = std::abs(*fsx) / 2;
+    hy = std::abs(*fsy) / 2;
+    return 1;
+}
+
+// gauss_filt.c:44-50, 101-120: sigma = hfs/2 (integer halves), weights
+// exp(-ii^2/sx - jj^2/sy) in float, their float sum, and the division by it
+// over 1..FILT_SIZE (none for negative sizes).  w holds (2hx+1)*(2hy+1)
+// floats, row jj+hy, column ii+hx.
+void gauss_weights(int fsx, int fsy, int hx, int hy, float *w)
+{
     const float sigx = hx / 2, sigy = hy / 2;
     const float sx = 2.0 * sigx * sigx, sy = 2.0 * sigy * sigy;
-    const int wx = 2 * hx + 1, wy = 2 * hy + 1;
-    std::vector<float> w((size_t)wx * wy);
+    const int wx = 2 * hx + 1;
     float sum = 0.0;
     for (int ii = -hx; ii <= hx; ++ii)
         for (int jj = -hy; jj <= hy; ++jj) {
@@ -104,8 +102,70 @@ extern "C" int rz_gauss_filt(int device, void *hip_stream, float *field, int nx,
             k = ::exp((double)(-((ii * ii) / sx) - ((jj * jj) / sy)));
             sum += k;
         }
-    for (int i = 0; i < *fsx; ++i)
-        for (int j = 0; j < *fsy; ++j) w[(size_t)j * wx + i] /= sum;
+    for (int i = 0; i < fsx; ++i)
+        for (int j = 0; j < fsy; ++j) w[(size_t)j * wx + i] /= sum;
+}
+
+hipError_t launch_gauss_filt(const float *src, float *dst, const float *w, int nx, int ny, int hx, int hy,
+                             hipStream_t st)
+{
+    const dim3 block(kGX, kGY);
+    const dim3 grid((nx + kGX - 1) / kGX, (ny + kGY * kQY - 1) / (kGY * kQY));
+    hipLaunchKernelGGL(k_gauss_filt, grid, block, 0, st, src, dst, w, nx, ny, hx, hy);
+    return hipGetLastError();
+}
+
+int check_grid(const void *field, int nx, int ny, const int *fsx, const int *fsy)
+{
+    if (!field || !fsx || !fsy) return fail_global(RZ_ERR_INVALID, "gauss_filt: null pointer");
+    if (nx < 1 || ny < 1) return fail_global(RZ_ERR_INVALID, "gauss_filt: NX, NY must be >= 1");
+    if (ny > 65535 * kGY * kQY || (long long)nx * ny > (1LL << 31))
+        return fail_global(RZ_ERR_INVALID, "gauss_filt: grid too large");
+    return RZ_OK;
+}
+
+// Scratch of rz_gauss_filt_noalloc: the filter weights at [0, kNaWeights)
+// and the copy of the input field after them.  Every call on a device uses
+// that device's copy of this array; calls that overlap on one device race.
+constexpr size_t kNaFloats = RZ_GAUSS_NOALLOC_FLOATS;
+constexpr size_t kNaWeights = RZ_GAUSS_NOALLOC_WEIGHTS;
+static __device__ __align__(256) float scratch[kNaFloats];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+float *g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on `device` (the current device), looked up
+// once per device.
+int scratch_on(int device, float **out)
+{
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail_global(RZ_ERR_HIP, "gauss_filt_noalloc: scratch address on device %d: %s", device,
+                               hipGetErrorString(e));
+        g_na_ptr[device] = static_cast<float *>(p);
+    }
+    *out = g_na_ptr[device];
+    return RZ_OK;
+}
+
+}  // namespace
+}  // namespace rz
+
+extern "C" int rz_gauss_filt(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy,
+                             int on_device)
+{
+    using namespace rz;
+    if (const int rc = check_grid(field, nx, ny, fsx, fsy)) return rc;
+    int hx = 0, hy = 0;
+    if (!normalise_sizes(fsx, fsy, hx, hy)) return RZ_OK;
+    if (hx > 4096 || hy > 4096) return fail_global(RZ_ERR_INVALID, "gauss_filt: filter size above 8193");
+    const int wx = 2 * hx + 1, wy = 2 * hy + 1;
+    std::vector<float> w((size_t)wx * wy);
+    gauss_weights(*fsx, *fsy, hx, hy, w.data());
 
     hipStream_t st = (hipStream_t)hip_stream;
     if (hipSetDevice(device) != hipSuccess) return fail_global(RZ_ERR_HIP, "gauss_filt: no device %d", device);
@@ -126,14 +186,49 @@ extern "C" int rz_gauss_filt(int device, void *hip_stream, float *field, int nx,
     hipError_t e = hipMemcpyAsync(wd, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(src, field, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        const dim3 block(kGX, kGY);
-        const dim3 grid((nx + kGX - 1) / kGX, (ny + kGY * kQY - 1) / (kGY * kQY));
-        hipLaunchKernelGGL(k_gauss_filt, grid, block, 0, st, src, dst, wd, nx, ny, hx, hy);
-        e = hipGetLastError();
-    }
+    if (e == hipSuccess) e = launch_gauss_filt(src, dst, wd, nx, ny, hx, hy, st);
     if (e == hipSuccess && !on_device) e = hipMemcpyAsync(field, dst, bytes, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = st ? hipStreamSynchronize(st) : hipDeviceSynchronize();
     if (e != hipSuccess) return cleanup(fail_global(RZ_ERR_HIP, "gauss_filt: %s", hipGetErrorString(e)));
     return cleanup(RZ_OK);
+}
+
+extern "C" int rz_gauss_filt_noalloc(int device, void *hip_stream, float *field, int nx, int ny, int *fsx,
+                                     int *fsy)
+{
+    using namespace rz;
+    if (const int rc = check_grid(field, nx, ny, fsx, fsy)) return rc;
+    if (device < 0 || device >= kNaMaxDevices)
+        return fail_global(RZ_ERR_INVALID, "gauss_filt_noalloc: device %d out of range", device);
+    // checked before the sizes are touched: a rejected call changes nothing
+    const size_t ncell = (size_t)nx * ny;
+    if (ncell > kNaFloats - kNaWeights)
+        return fail_global(RZ_ERR_INVALID, "gauss_filt_noalloc: %d x %d grid above the %zu-cell scratch", nx, ny,
+                           (size_t)(kNaFloats - kNaWeights));
+    int tx = *fsx, ty = *fsy, hx = 0, hy = 0;
+    if (!normalise_sizes(&tx, &ty, hx, hy)) {
+        *fsx = tx;
+        *fsy = ty;
+        return RZ_OK;
+    }
+    const size_t nw = (size_t)(2 * hx + 1) * (2 * hy + 1);
+    if (hx > 4096 || hy > 4096 || nw > kNaWeights)
+        return fail_global(RZ_ERR_INVALID, "gauss_filt_noalloc: a %d x %d window exceeds %zu weights", std::abs(tx),
+                           std::abs(ty), (size_t)kNaWeights);
+    *fsx = tx;
+    *fsy = ty;
+    float w[kNaWeights];
+    gauss_weights(tx, ty, hx, hy, w);
+
+    hipStream_t st = (hipStream_t)hip_stream;
+    if (hipSetDevice(device) != hipSuccess) return fail_global(RZ_ERR_HIP, "gauss_filt_noalloc: no device %d", device);
+    float *sp = nullptr;
+    if (const int rc = scratch_on(device, &sp)) return rc;
+    float *src = sp + kNaWeights;
+    hipError_t e = hipMemcpyAsync(sp, w, nw * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(src, field, ncell * sizeof(float), hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = launch_gauss_filt(src, field, sp, nx, ny, hx, hy, st);
+    if (e == hipSuccess) e = st ? hipStreamSynchronize(st) : hipDeviceSynchronize();
+    if (e != hipSuccess) return fail_global(RZ_ERR_HIP, "gauss_filt_noalloc: %s", hipGetErrorString(e));
+    return RZ_OK;
 }

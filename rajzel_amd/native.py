@@ -25,7 +25,7 @@ EXPORTS = (
     "rz_sync", "rz_rows_per_lane", "rz_device_bytes", "rz_batch_shots", "rz_set_profiling",
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
     "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb", "rz_set_split", "rz_get_split",
-    "rz_read_picks", "rz_picks_parses", "rz_picks_clear",
+    "rz_read_picks", "rz_picks_parses", "rz_picks_clear", "rz_gauss_filt_noalloc",
 )
 
 
@@ -101,6 +101,7 @@ def lib():
                 "rz_set_graph": (i, [vp, i]),
                 "rz_autotune_rows": (i, [i, i, C.POINTER(ModelDesc), C.c_size_t, C.POINTER(i), C.POINTER(d)]),
                 "rz_gauss_filt": (i, [i, vp, vp, i, i, C.POINTER(i), C.POINTER(i), i]),
+                "rz_gauss_filt_noalloc": (i, [i, vp, vp, i, i, C.POINTER(i), C.POINTER(i)]),
                 "rz_get_tnb": (i, [vp, i, vp, i]),
                 "rz_set_split": (i, [vp, i]),
                 "rz_get_split": (i, [vp, C.POINTER(i), C.POINTER(i)]),
