@@ -326,6 +326,7 @@ def main():
         "phase_ms": phase_ms,
         "iterations": {"forward_mean": float(np.mean(itf)), "adjoint_mean": float(np.mean(ita)),
                        "forward_max": int(np.max(itf)), "adjoint_max": int(np.max(ita))},
+        "stop_test_passes": {k: int(np.sum(v)) for k, v in ctx.stop_passes(0).items() if not k.startswith("sum")},
     }
 
     if not args.no_plain:

@@ -21,11 +21,15 @@
  *     cause.  Nothing here aborts the process (the reference's err() does;
  *     the reference-surface shim maps failures onto it).
  *   - Results are bitwise those of the reference's single-rank float path
- *     (one MPI rank).  One decision is taken differently: the stopping
- *     test sums |T - Told| in double over the grid (norm1.c:21-29 sums in
- *     float, column by column), so the iteration count can differ from the
- *     reference's only when the mean change lies within float rounding of
- *     TTNORM.  A converged fixed point (no cell changes) is decided alike.
+ *     (one MPI rank), outer-iteration counts included: the stopping test
+ *     (eikonal.c:44, eikonal_adj.c:37) is decided from a double sum of the
+ *     per-sweep changes where a rigorous bound places norm1's float sum on
+ *     one side of the threshold, and otherwise from norm1's float sum taken
+ *     in the reference's order (norm1.c:21-29, x outer, y inner).
+ *     Between the two, a count of the cells that changed tightens the
+ *     bound.  RAJZEL_AMD_ORDERED_NORM=1 in the environment at rz_create
+ *     takes the ordered sum for every nonzero decision, =2 the count (tests
+ *     of those paths).
  *
  * A context is single-threaded: calls on one context must not overlap.
  */
@@ -155,6 +159,15 @@ int rz_get_grad(rz_ctx *ctx, int model, float *out, int on_device);      /* NY*N
 int rz_get_misfit(rz_ctx *ctx, int model, double *L2, double *l2_shot);
 /* Outer iterations per shot; -1 = hit max_iter.  Either pointer may be NULL. */
 int rz_get_iterations(rz_ctx *ctx, int model, int *it_f, int *it_a);
+/* Per shot, how often the stopping tests of the last run read the fields
+ * because the double sum could not decide: counts of changed cells
+ * (count_*) and norm1's float sum in the reference's order (ordered_*),
+ * forward and adjoint; sum_* is that float sum (norm1.c:21-25, before the
+ * division by NX*NY) of the last iteration's test where it was taken, else
+ * -1.  Any pointer may be NULL.  No counterpart in the reference (a cost
+ * diagnostic). */
+int rz_get_stop_passes(rz_ctx *ctx, int model, int *count_f, int *count_a, int *ordered_f, int *ordered_a,
+                       float *sum_f, float *sum_a);
 /* Tres array after the evaluation (the last shot's residuals). */
 int rz_get_tres_carry(rz_ctx *ctx, int model, float *out);
 /* Field of one shot, NY*NX: which = 0 traveltime T, 1 adjoint lambda.
@@ -259,6 +272,9 @@ int rz_rows_per_lane(const rz_ctx *ctx);
 size_t rz_device_bytes(const rz_ctx *ctx);
 /* Shots per launch of rz_gradient (all shots, or fewer under a budget). */
 int rz_batch_shots(const rz_ctx *ctx);
+/* Bytes of device memory one shot's fields take (T in two layouts, lambda
+ * in three buffers): the unit an hbm_budget is counted in. */
+size_t rz_slot_bytes(const rz_ctx *ctx);
 
 #ifdef __cplusplus
 }

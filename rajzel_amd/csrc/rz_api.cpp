@@ -36,6 +36,8 @@ struct ModelHost {
     bool ft_s = false;      // ... and the slowness keeps its inputs clear of underflow
     float rdh_hi = 0.f, rdh_lo = 0.f, s_min_ft = 0.f;
     double t_min_ft = 0.0;  // smallest nonzero T the float quotient allows
+    double stop_s = 0.0, band_lo = 0.0, band_hi = 0.0;   // stopping test (set_stop)
+    int stop_mode = 0;      // tests of the slower decisions (RAJZEL_AMD_ORDERED_NORM)
     // device
     float *S = nullptr, *SA = nullptr, *SB = nullptr;
     int *rec_k_d = nullptr, *rec_c_d = nullptr, *uq_k_d = nullptr, *uq_c_d = nullptr, *uq_r_d = nullptr;
@@ -89,16 +91,20 @@ struct rz_ctx {
     float *grad_d = nullptr;
     double *l2_shot_d = nullptr, *l2_sum_d = nullptr;
     int *it_f_d = nullptr, *it_a_d = nullptr;
+    int *lsel_d = nullptr;              // per shot: the converged lambda is in LC (1) or LA (0)
+    int *ord_f_d = nullptr, *ord_a_d = nullptr;   // per shot: passes of the stopping test over the fields
+    float *osum_f_d = nullptr, *osum_a_d = nullptr; // per shot: ordered sum of the last decision
     int *err_d = nullptr;
     // single-shot scratch
     float *s_tmod = nullptr, *s_tnb = nullptr, *s_tres = nullptr, *s_row = nullptr, *s_row2 = nullptr;
-    int *s_it = nullptr;
+    int *s_it = nullptr;                // iterations forward, adjoint; lambda location; stop passes f, a
+    float *s_osum = nullptr;            // ordered sums of the last decisions, forward and adjoint
     ShotDev *s_shot = nullptr;
     int *s_slot0 = nullptr;
     // field slots
     int nslots = 0;
     long long slot_floats = 0;
-    float *TA = nullptr, *TB = nullptr, *LA = nullptr, *LB = nullptr;
+    float *TA = nullptr, *TB = nullptr, *LA = nullptr, *LB = nullptr, *LC = nullptr;
     // host mirror of the shot table
     std::vector<ShotDev> shots_h;
     int last_b0 = -1, last_b1 = -1;   // global shot range resident in the slots
@@ -248,6 +254,61 @@ void set_ft(ModelHost &h)
     h.ft_dh = d.ok && smin < 1e30 && 2.0 * (double)h.d.time >= h.t_min_ft && tmax / dh < std::ldexp(1.0, 120);
 }
 
+// The stopping test of eikonal.c:44 / eikonal_adj.c:37 continues iff
+// (float)sum / (float)(NX*NY) >= TTNORM, sum being norm1's float sum of
+// |T - Told| (norm1.c:21-27).  A correctly rounded float division is
+// monotone in the dividend, so this holds iff sum >= stop_s, the least
+// non-negative float with that property (0 if every sum continues, +inf if
+// none does).  The kernels' double sum G of per-sweep decreases is within
+// 16u of the exact sum E (u = 2^-24: one float difference per cell and a
+// float sum over at most 8 rows per lane; the double sums add ~1e-12), and
+// norm1's float sum of NX*NY non-negative terms is within (1+u)(1+gamma) - 1
+// of E, gamma = m u / (1 - m u) with m = NX*NY.  band_lo and band_hi turn G
+// into bounds on norm1's sum.  Where they straddle stop_s the kernels count
+// the cells that changed (norm1's nonzero terms) and bound again with that
+// count, and only where that cannot decide either take the sum in the
+// reference's order.  With NX*NY*u >= 1/2 the first band is useless and
+// every nonzero decision goes to the count.
+void set_stop(ModelHost &h)
+{
+    const float denom = (float)(h.d.nx * h.d.ny);
+    const float ttn = h.d.ttnorm;
+    auto cont = [&](uint32_t bits) {
+        float s;
+        std::memcpy(&s, &bits, 4);
+        const volatile float q = s / denom;
+        return q >= ttn;
+    };
+    const uint32_t top = 0x7F7FFFFFu;   // FLT_MAX
+    if (cont(0)) {
+        h.stop_s = 0.0;
+    } else if (!cont(top)) {
+        h.stop_s = INFINITY;
+    } else {
+        uint32_t lo = 0, hi = top;      // cont(lo) false, cont(hi) true; bit order = value order here
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            (cont(mid) ? hi : lo) = mid;
+        }
+        float s;
+        std::memcpy(&s, &hi, 4);
+        h.stop_s = s;
+    }
+    const double u = std::ldexp(1.0, -24), mu = (double)h.d.nx * (double)h.d.ny * u;
+    // tests of the slower decisions: 1 = every nonzero decision takes the
+    // ordered sum, 2 = every one counts the changed cells first
+    const char *env = getenv("RAJZEL_AMD_ORDERED_NORM");
+    h.stop_mode = (env && env[0] == '1') ? 1 : (env && env[0] == '2') ? 2 : 0;
+    if (mu >= 0.5 || h.stop_mode != 0) {
+        h.band_lo = 0.0;
+        h.band_hi = INFINITY;
+        return;
+    }
+    const double gam = mu / (1.0 - mu), er = (1.0 + u) * (1.0 + gam) - 1.0, eg = 16.0 * u + 1e-9;
+    h.band_lo = (1.0 - er) / (1.0 + eg) * (1.0 - 1e-9);
+    h.band_hi = (1.0 + er) / (1.0 - eg) * (1.0 + 1e-9);
+}
+
 int choose_R(int ny_max)
 {
     for (int R : {1, 2, 4, 8})
@@ -270,6 +331,7 @@ ModelDev make_dev(const rz_ctx *c, const ModelHost &h)
     d.uq_k = h.uq_k_d; d.uq_c = h.uq_c_d; d.uq_r = h.uq_r_d;
     d.max_iter = h.d.max_iter > 0 ? h.d.max_iter : 1000;
     d.rdh_hi = h.rdh_hi; d.rdh_lo = h.rdh_lo; d.s_min_ft = h.s_min_ft;
+    d.stop_s = h.stop_s; d.band_lo = h.band_lo; d.band_hi = h.band_hi; d.stop_mode = h.stop_mode;
     return d;
 }
 
@@ -290,7 +352,7 @@ int adj_variant(const rz_ctx *c)
 Slots slots_of(const rz_ctx *c)
 {
     Slots s;
-    s.TA = c->TA; s.TB = c->TB; s.LA = c->LA; s.LB = c->LB;
+    s.TA = c->TA; s.TB = c->TB; s.LA = c->LA; s.LB = c->LB; s.LC = c->LC;
     s.slot_stride = c->slot_floats;
     return s;
 }
@@ -298,7 +360,8 @@ Slots slots_of(const rz_ctx *c)
 ShotOut outs_of(const rz_ctx *c)
 {
     ShotOut o;
-    o.tmod = c->tmod_d; o.tnb = c->tnb_d; o.it_f = c->it_f_d; o.it_a = c->it_a_d;
+    o.tmod = c->tmod_d; o.tnb = c->tnb_d; o.it_f = c->it_f_d; o.it_a = c->it_a_d; o.lsel = c->lsel_d;
+    o.ord_f = c->ord_f_d; o.ord_a = c->ord_a_d; o.osum_f = c->osum_f_d; o.osum_a = c->osum_a_d;
     return o;
 }
 
@@ -522,7 +585,8 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
         const int s1 = std::min(b1, h.shot_base + h.d.nshots) - h.shot_base;
         if (s1 <= s0) continue;
         ModelDev mh = make_dev(c, h);
-        HIPCHK(c, launch_gradient(&mh, (int)mi, c->md_d, c->slot_of_d + h.shot_base + s0, s1 - s0, sl,
+        HIPCHK(c, launch_gradient(&mh, (int)mi, c->md_d, c->slot_of_d + h.shot_base + s0,
+                                  c->lsel_d + h.shot_base + s0, s1 - s0, sl,
                                   c->grad_d + h.grad_off, s0 > 0 ? 1 : 0, c->stream));
     }
     TRY(mark(c, &m.e[4]));
@@ -747,6 +811,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
             h.uq_c.push_back((int)(p.first % d.nx));
             h.uq_r.push_back(p.second);
         }
+        set_stop(h);
         h.fastdiv = fast_div_exact(d.dh);
         c->fastdiv = c->fastdiv && h.fastdiv;
         set_ft(h);
@@ -802,7 +867,10 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         (rc = dalloc(c, &c->dres_d, rec_off)) ||
         (rc = dalloc(c, &c->grad_d, cells)) || (rc = dalloc(c, &c->l2_shot_d, shots)) ||
         (rc = dalloc(c, &c->l2_sum_d, nm)) || (rc = dalloc(c, &c->it_f_d, shots)) ||
-        (rc = dalloc(c, &c->it_a_d, shots)) || (rc = dalloc(c, &c->err_d, 1)))
+        (rc = dalloc(c, &c->it_a_d, shots)) || (rc = dalloc(c, &c->lsel_d, shots)) ||
+        (rc = dalloc(c, &c->ord_f_d, shots)) || (rc = dalloc(c, &c->ord_a_d, shots)) ||
+        (rc = dalloc(c, &c->osum_f_d, shots)) || (rc = dalloc(c, &c->osum_a_d, shots)) ||
+        (rc = dalloc(c, &c->err_d, 1)))
         return bail(rc);
     int ntr_max = 0, ncell_max = 0;
     for (auto &h : c->m) {
@@ -811,13 +879,13 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     }
     if ((rc = dalloc(c, &c->s_tmod, ntr_max)) || (rc = dalloc(c, &c->s_tnb, ntr_max)) ||
         (rc = dalloc(c, &c->s_tres, ntr_max)) || (rc = dalloc(c, &c->s_row, ncell_max)) ||
-        (rc = dalloc(c, &c->s_row2, ncell_max)) || (rc = dalloc(c, &c->s_it, 2)) ||
+        (rc = dalloc(c, &c->s_row2, ncell_max)) || (rc = dalloc(c, &c->s_it, 5)) || (rc = dalloc(c, &c->s_osum, 2)) ||
         (rc = dalloc(c, &c->s_shot, 1)) || (rc = dalloc(c, &c->s_slot0, 1)))
         return bail(rc);
 
-    // field slots (T and lambda in two layouts): all shots, or as many as
-    // the budget allows
-    const size_t per_slot = (size_t)c->slot_floats * 4 * sizeof(float);
+    // field slots (T in two layouts, lambda in three buffers): all shots, or
+    // as many as the budget allows
+    const size_t per_slot = (size_t)c->slot_floats * 5 * sizeof(float);
     int nslots = std::max(1, shots);
     if (hbm_budget > 0) {
         const size_t fit = hbm_budget / per_slot;
@@ -827,11 +895,11 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     c->nslots = nslots;
     const size_t sf = (size_t)c->slot_floats * nslots;
     if ((rc = dalloc(c, &c->TA, sf)) || (rc = dalloc(c, &c->TB, sf)) || (rc = dalloc(c, &c->LA, sf)) ||
-        (rc = dalloc(c, &c->LB, sf)))
+        (rc = dalloc(c, &c->LB, sf)) || (rc = dalloc(c, &c->LC, sf)))
         return bail(rc);
     {
         const size_t nsh = (size_t)std::max(1, shots);
-        if ((rc = dalloc(c, &c->ss.cnt, nsh * kSyncInts)) || (rc = dalloc(c, &c->ss.part, nsh * 2 * kMaxBands)) ||
+        if ((rc = dalloc(c, &c->ss.cnt, nsh * kSyncInts)) || (rc = dalloc(c, &c->ss.part, nsh * kSyncPart)) ||
             (rc = dalloc(c, &c->ss.err, 1)))
             return bail(rc);
         if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
@@ -1053,6 +1121,28 @@ int rz_get_iterations(rz_ctx *c, int model, int *it_f, int *it_a)
     return RZ_OK;
 }
 
+int rz_get_stop_passes(rz_ctx *c, int model, int *count_f, int *count_a, int *ordered_f, int *ordered_a,
+                       float *sum_f, float *sum_a)
+{
+    TRY(check_model(c, model));
+    const ModelHost &h = c->m[model];
+    const int ns = h.d.nshots;
+    std::vector<int> f(ns), a(ns);
+    if (ns) {
+        TRY(copy_out(c, f.data(), c->ord_f_d + h.shot_base, ns * sizeof(int), 0));
+        TRY(copy_out(c, a.data(), c->ord_a_d + h.shot_base, ns * sizeof(int), 0));
+        if (sum_f) TRY(copy_out(c, sum_f, c->osum_f_d + h.shot_base, ns * sizeof(float), 0));
+        if (sum_a) TRY(copy_out(c, sum_a, c->osum_a_d + h.shot_base, ns * sizeof(float), 0));
+    }
+    for (int i = 0; i < ns; ++i) {
+        if (count_f) count_f[i] = f[i] & 0xFFFF;
+        if (count_a) count_a[i] = a[i] & 0xFFFF;
+        if (ordered_f) ordered_f[i] = f[i] >> 16;
+        if (ordered_a) ordered_a[i] = a[i] >> 16;
+    }
+    return RZ_OK;
+}
+
 int rz_get_tres_carry(rz_ctx *c, int model, float *out)
 {
     TRY(check_model(c, model));
@@ -1073,9 +1163,12 @@ int rz_get_field(rz_ctx *c, int model, int shot, int which, float *out, int on_d
         return fail(c, RZ_ERR_STATE, "fields of shot %d are not resident (HBM budget batching)", shot);
     const int slot = g % c->nslots;
     ModelDev mh = make_dev(c, h);
-    const float *f = (which == 0 ? c->TA : c->LA) + (long long)slot * c->slot_floats;
+    const long long so = (long long)slot * c->slot_floats;
     const size_t n = (size_t)h.d.nx * h.d.ny;
-    HIPCHK(c, launch_field_to_rowmajor(&mh, f, c->s_row, which == 0 ? 1 : 0, c->stream));
+    if (which == 0)
+        HIPCHK(c, launch_field_to_rowmajor(&mh, c->TA + so, nullptr, nullptr, c->s_row, 1, c->stream));
+    else
+        HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA + so, c->LC + so, c->lsel_d + g, c->s_row, 0, c->stream));
     return copy_out(c, out, c->s_row, n * sizeof(float), on_device);
 }
 
@@ -1090,11 +1183,13 @@ int rz_eikonal_shot(rz_ctx *c, int model, int src_ix, int src_iy, float *T, floa
     sd.model = model; sd.shot = 0; sd.src_k = src_iy; sd.src_c = src_ix; sd.slot = 0;
     HIPCHK(c, hipMemcpyAsync(c->s_shot, &sd, sizeof sd, hipMemcpyHostToDevice, c->stream));
     ShotOut o;
-    o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1;
+    o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1; o.lsel = c->s_it + 2;
+    o.ord_f = c->s_it + 3; o.ord_a = c->s_it + 4;
+    o.osum_f = c->s_osum; o.osum_a = c->s_osum + 1;
     HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d, o,
                              c->stream));
     ModelDev mh = make_dev(c, h);
-    HIPCHK(c, launch_field_to_rowmajor(&mh, c->TA, c->s_row, 1, c->stream));
+    HIPCHK(c, launch_field_to_rowmajor(&mh, c->TA, nullptr, nullptr, c->s_row, 1, c->stream));
     int it = 0;
     const size_t n = (size_t)h.d.nx * h.d.ny;
     if (T) HIPCHK(c, hipMemcpyAsync(T, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -1147,10 +1242,12 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     sd.model = model; sd.slot = 0;
     HIPCHK(c, hipMemcpyAsync(c->s_shot, &sd, sizeof sd, hipMemcpyHostToDevice, c->stream));
     ShotOut o;
-    o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1;
+    o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1; o.lsel = c->s_it + 2;
+    o.ord_f = c->s_it + 3; o.ord_a = c->s_it + 4;
+    o.osum_f = c->s_osum; o.osum_a = c->s_osum + 1;
     HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d,
                              c->s_tres, o, variant, c->stream));
-    HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA, c->s_row, 0, c->stream));
+    HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA, c->LC, c->s_it + 2, c->s_row, 0, c->stream));
     int it = 0;
     if (lam) HIPCHK(c, hipMemcpyAsync(lam, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(&it, c->s_it + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1172,7 +1269,9 @@ int rz_ass_grad_shot(rz_ctx *c, int model, const float *lam, float *grad_shot, f
     ModelDev mh = make_dev(c, h);
     HIPCHK(c, hipMemcpyAsync(c->s_row, lam, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_rowmajor_to_field(&mh, c->s_row, c->LA, 0, c->stream));
-    HIPCHK(c, launch_gradient(&mh, model, c->md_d, c->s_slot0, 1, slots_of(c), c->s_row2, 0, c->stream));
+    // slot 0, lambda in LA: s_slot0 holds the 0 for both
+    HIPCHK(c, launch_gradient(&mh, model, c->md_d, c->s_slot0, c->s_slot0, 1, slots_of(c), c->s_row2, 0,
+                              c->stream));
     std::vector<float> gs(n);
     HIPCHK(c, hipMemcpyAsync(gs.data(), c->s_row2, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1282,5 +1381,6 @@ int rz_autotune_rows(int device, int nmodels, const rz_model_desc *models, size_
 }
 size_t rz_device_bytes(const rz_ctx *c) { return c ? c->bytes : 0; }
 int rz_batch_shots(const rz_ctx *c) { return c ? c->nslots : 0; }
+size_t rz_slot_bytes(const rz_ctx *c) { return c ? (size_t)c->slot_floats * 5 * sizeof(float) : 0; }
 
 }  // extern "C"

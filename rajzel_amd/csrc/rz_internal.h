@@ -52,7 +52,14 @@ struct ModelDev {
     // (adjoint variant 2); slowness below s_min_ft turns that variant off.
     float rdh_hi, rdh_lo;
     float s_min_ft;
-    int pad2_;
+    int stop_mode;                 // 0; tests: 1 every nonzero stop takes the ordered sum, 2 the count first
+    // Stopping test (norm1.c:17-31, eikonal.c:44, eikonal_adj.c:37): the
+    // reference continues iff its float sum of |T - Told| is >= stop_s.  A
+    // block's double sum G of per-sweep decreases brackets that float sum
+    // by [G*band_lo, G*band_hi]; when the bracket straddles stop_s the sum
+    // is taken in the reference's order (ordered_norm1).
+    double stop_s;
+    double band_lo, band_hi;
 };
 
 struct ShotDev {
@@ -63,8 +70,13 @@ struct ShotDev {
     int pad_;
 };
 
+// Per-shot fields.  A-family outputs alternate between two buffers from one
+// outer iteration to the next, so that the field at the start of an
+// iteration and the one at its end are both intact at the stopping test:
+// forward T in TA / LA (LA is free until the adjoint starts), adjoint
+// lambda in LC / LA (LA holds the initial lambda).
 struct Slots {
-    float *TA, *TB, *LA, *LB;      // base of slot 0; slot s at + s*slot_stride
+    float *TA, *TB, *LA, *LB, *LC; // base of slot 0; slot s at + s*slot_stride
     long long slot_stride;         // elements
 };
 
@@ -74,13 +86,18 @@ struct ShotOut {
     float *tmod;     // T at each receiver (calc_FA.c:15-17)
     float *tnb;      // T at the receiver's slope neighbour row (calc_FA_res.c:52-69)
     int *it_f, *it_a;
+    int *lsel;       // where the converged lambda is: 0 LA, 1 LC (indexed as it_a)
+    int *ord_f, *ord_a;  // passes of the stopping test over the fields (indexed as it_f): the
+                         // counts of changed cells in bits 0-15, the ordered sums above
+    float *osum_f, *osum_a;  // norm1's ordered float sum of the last decision, -1 if not taken
 };
 
 // Coupling of the two workgroups of split shots (rz_kernels.hip): per shot
-// kSyncInts ints of counters and 2*kMaxBands doubles of partial sums; err
+// kSyncInts ints of counters and kSyncPart doubles of partial sums; err
 // becomes nonzero when a launch cannot run split (1: the two workgroups of
 // a shot were on different XCDs, 2: a wait timed out).
 constexpr int kSyncInts = 64;
+constexpr int kSyncPart = 4 * kMaxBands;   // doubles per split shot: partial sums, then counts
 struct SplitSync {
     int *cnt;
     double *part;
@@ -110,9 +127,11 @@ hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s
 hipError_t launch_eval_init(float *carry, const float *carry0, int ntr_total, double *l2_sum, int nmodels,
                             float *grad, long long ncells, hipStream_t st);
 hipError_t launch_gradient(const ModelDev *md_host, int model, const ModelDev *md_dev, const int *slot_of,
-                           int nsh, Slots slots, float *grad, int accumulate, hipStream_t st);
-hipError_t launch_field_to_rowmajor(const ModelDev *md_host, const float *field, float *out, int strip,
-                                    hipStream_t st);
+                           const int *lsel, int nsh, Slots slots, float *grad, int accumulate, hipStream_t st);
+// Layout-A field -> row-major; with sel != nullptr the field is `alt` where
+// *sel is nonzero (a lambda that ended in LC).
+hipError_t launch_field_to_rowmajor(const ModelDev *md_host, const float *field, const float *alt, const int *sel,
+                                    float *out, int strip, hipStream_t st);
 hipError_t launch_rowmajor_to_field(const ModelDev *md_host, const float *in, float *field, int family,
                                     hipStream_t st);
 

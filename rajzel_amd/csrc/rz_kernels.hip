@@ -18,6 +18,8 @@
 // -ffp-contract=off): see godunov_sel(), qdh() and adj_sweep().  Any topological order
 // of a Gauss-Seidel sweep's dependency DAG gives the same bits, so each
 // sweep here equals the reference's sweep bit for bit.
+#include <climits>
+
 #include "rz_internal.h"
 
 namespace rz {
@@ -304,6 +306,21 @@ __device__ __forceinline__ double block_sum(double v, double *red, int nwaves)
     return t;
 }
 
+// What a forward sweep returns for its lane: the sum of decreases and an
+// upper bound on the cells it changed (R per step with any change).
+struct Dec {
+    double sum, cnt;
+};
+
+// block_sum of both parts.
+__device__ __forceinline__ Dec block_sum2(Dec v, double *red, int nwaves)
+{
+    Dec r;
+    r.sum = block_sum(v.sum, red, nwaves);
+    r.cnt = block_sum(v.cnt, red, nwaves);
+    return r;
+}
+
 // Barrier between step epochs of a sweep.  Within a sweep no wave reads
 // global memory another wave writes: new values cross bands through the LDS
 // ring, old values through loads that have already been consumed (waited
@@ -370,6 +387,356 @@ __device__ __forceinline__ WaveCtx uni_ctx(const WaveCtx &w)
     WaveCtx u = uni(w);
     u.lane = w.lane;
     return u;
+}
+
+// norm1.c:17-31 in the reference's order: the float sum s, x outer and y
+// inner, of the terms |Tend - Tstart| (a float difference) over the grid;
+// Tstart == nullptr stands for the initial forward field (tinit, 0 at the
+// source).  Both fields are in layout A.
+//
+// The sum is exact but not serial.  While s stays in one binade
+// [2^e, 2^(e+1)) (e >= -126, where the floats are the multiples of
+// u_e = 2^(e-23)), s + d is s plus d rounded to a multiple of u_e, whatever
+// s is, except where d/u_e lies exactly half-way between two integers: such
+// a tie rounds to the even neighbour, so it depends on the parity of s/u_e
+// only.  A run of terms is therefore a function of the incoming parity, to
+// (integer increment, outgoing parity), and those functions compose.  A
+// thread takes kNormPer consecutive rows of one column (the order is padded
+// with zero terms to whole blocks of rows per column: adding +0 changes no
+// float sum), so that it loads whole R-row groups; per chunk of blockDim
+// such blocks the workgroup scans the threads' functions, and each thread
+// walks its terms from its exact incoming sum.  The first term that takes
+// the sum to 2^(e+1) or beyond is added as a float by its thread, and the
+// walk restarts after it in the new binade: a sum crosses about as many
+// binades as it spans (a few dozen).  Increments stay below 2^24 within a
+// binade, so 32-bit unsigned arithmetic is exact up to the first such term;
+// what wraps after it is never used.
+constexpr int kNormPer = 16;
+
+// d in units of 2^(e-23), packed: bits 0-23 the floor, bit 29 a tie (the
+// rest is exactly one half), bit 30 above one half, bit 31 the term alone
+// takes a sum of this binade out of it
+constexpr unsigned kIncTie = 1u << 29, kIncUp = 1u << 30, kIncBig = 1u << 31;
+
+__device__ __forceinline__ unsigned term_inc(float d, int e, bool s_normal)
+{
+    const unsigned b = __float_as_uint(d);
+    const int E = (b >> 23) & 0xFF;
+    const unsigned M = (b & 0x7FFFFFu) | (E ? 0x800000u : 0u);
+    if (M == 0) return 0;
+    const int sh = (E ? E : 1) - 150 - e + 23;
+    // a normal d of at least u_e: M * 2^sh >= 2^23 units, and s >= 2^23 units
+    // when normal, so the sum leaves the binade
+    if (sh >= 24 || (sh >= 0 && (s_normal || (M << sh) >= (1u << 24)))) return kIncBig;
+    if (sh >= 0) return M << sh;
+    const int k = -sh;
+    if (k > 25) return 0;                 // below a quarter unit: rounds to 0
+    const unsigned rem = M & ((1u << k) - 1u), half = 1u << (k - 1);
+    return (M >> k) | (rem == half ? kIncTie : 0u) | (rem > half ? kIncUp : 0u);
+}
+
+// the increment of a packed term on a running sum of parity p (ties to even)
+__device__ __forceinline__ unsigned inc_at(unsigned v, unsigned p)
+{
+    const unsigned f = v & 0xFFFFFFu;
+    return f + (((v & kIncUp) || ((v & kIncTie) && ((p + f) & 1u))) ? 1u : 0u);
+}
+
+// A run of terms as a function of the incoming parity p: increment t[p];
+// outgoing parity q[p] in bit p of q.
+struct ParFn {
+    unsigned t0, t1, q;
+};
+
+// first F, then G
+__device__ __forceinline__ ParFn compose(const ParFn &F, const ParFn &G)
+{
+    const unsigned f0 = F.q & 1u, f1 = (F.q >> 1) & 1u;
+    ParFn H;
+    H.t0 = F.t0 + (f0 ? G.t1 : G.t0);
+    H.t1 = F.t1 + (f1 ? G.t1 : G.t0);
+    H.q = ((G.q >> f0) & 1u) | (((G.q >> f1) & 1u) << 1);
+    return H;
+}
+
+// Workgroup-wide exclusive scan of the threads' functions in thread order;
+// *all gets the composition of every thread's.
+__device__ ParFn block_exscan_fn(const ParFn &mine, ParFn *all)
+{
+    __shared__ ParFn wtot[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const ParFn id{0u, 0u, 2u};
+    ParFn inc = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        ParFn y;
+        y.t0 = __shfl_up(inc.t0, o, 64);
+        y.t1 = __shfl_up(inc.t1, o, 64);
+        y.q = __shfl_up(inc.q, o, 64);
+        if (lane >= o) inc = compose(y, inc);
+    }
+    ParFn ex;
+    ex.t0 = __shfl_up(inc.t0, 1, 64);
+    ex.t1 = __shfl_up(inc.t1, 1, 64);
+    ex.q = __shfl_up(inc.q, 1, 64);
+    if (lane == 0) ex = id;
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    ParFn base = id, tot = id;
+    for (int i = 0; i < nw; ++i) {
+        if (i < wave) base = compose(base, wtot[i]);
+        tot = compose(tot, wtot[i]);
+    }
+    __syncthreads();
+    *all = tot;
+    return compose(base, ex);
+}
+
+__device__ int block_min(int v)
+{
+    __shared__ int m_s;
+    if (threadIdx.x == 0) m_s = INT_MAX;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) atomicMin(&m_s, v);
+    __syncthreads();
+    const int r = m_s;
+    __syncthreads();
+    return r;
+}
+
+template <int R>
+__device__ __attribute__((noinline)) float ordered_norm1(int nx_, int ny_, long long bstride_, const float *Tend_,
+                                                         const float *Tstart__, float tinit_, int src_k_, int src_c_)
+{
+    constexpr int BR = kWave * R;
+    static_assert(kNormPer % R == 0, "whole row groups per thread");
+    __shared__ float s_s;
+    const int nx = uni(nx_), ny = uni(ny_), src_k = uni(src_k_), src_c = uni(src_c_);
+    const long long bstride = uni(bstride_);
+    const float tinit = uni(tinit_);
+    const float *Tstart_ = uni(Tstart__);
+    const gfloat *Tend = gp(uni(Tend_)), *Tstart = gp(Tstart_);
+    const int nseg = (ny + kNormPer - 1) / kNormPer;   // row blocks per column
+    const int nblk = nx * nseg, t = threadIdx.x, nt = blockDim.x;
+    float s = 0.f;                                      // the running sum, the same in every thread
+    for (int g0 = 0; g0 < nblk; g0 += nt) {
+        // this thread's terms: rows k0 .. k0 + kNormPer - 1 of column c
+        const int g = g0 + t, c = g / nseg, k0 = (g - c * nseg) * kNormPer;
+        const bool live = g < nblk;
+        float d[kNormPer];
+#pragma unroll
+        for (int j = 0; j < kNormPer; j += R) {
+            const int k = k0 + j, bb = k / BR, l = (k % BR) / R;
+            float te[R], ts[R];
+            if (live && k < ny) {
+                const long long a = bb * bstride + (long long)(c + l) * BR + l * R;
+                ldv<R>(Tend + a, te);
+                if (Tstart_) ldv<R>(Tstart + a, ts);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float t0 = Tstart_ ? ts[r] : ((k + r == src_k && c == src_c) ? 0.f : tinit);
+                d[j + r] = (live && k + r < ny) ? fabsf(te[r] - t0) : 0.f;
+            }
+        }
+        const int mine = t * kNormPer;                 // chunk index of this thread's first term
+        int start = 0;                                 // chunk index of the first term not yet added
+        for (;;) {
+            if (s == 0.f) {                            // 0 + d = d: the first nonzero term
+                int f = INT_MAX;
+#pragma unroll
+                for (int i = kNormPer - 1; i >= 0; --i)
+                    if (mine + i >= start && d[i] != 0.f) f = mine + i;
+                const int fmin = block_min(f);
+                if (fmin == INT_MAX) break;
+#pragma unroll
+                for (int i = 0; i < kNormPer; ++i)
+                    if (mine + i == fmin) s_s = d[i];
+                __syncthreads();
+                s = s_s;
+                __syncthreads();
+                start = fmin + 1;
+                continue;
+            }
+            // s = N * 2^(e-23), 2^23 <= N < 2^24 (N < 2^23 below the normal range)
+            const unsigned sb = __float_as_uint(s);
+            const int Es = (sb >> 23) & 0xFF;
+            const int e = (Es ? Es : 1) - 127;
+            const unsigned N = (sb & 0x7FFFFFu) | (Es ? 0x800000u : 0u);
+            unsigned v[kNormPer];
+            ParFn F{0u, 0u, 2u};
+#pragma unroll
+            for (int i = 0; i < kNormPer; ++i) {
+                v[i] = mine + i >= start ? term_inc(d[i], e, Es != 0) : 0u;
+                const unsigned a0 = inc_at(v[i], F.q & 1u), a1 = inc_at(v[i], (F.q >> 1) & 1u);
+                F.t0 += a0;
+                F.t1 += a1;
+                F.q = (((F.q & 1u) + a0) & 1u) | (((((F.q >> 1) & 1u) + a1) & 1u) << 1);
+            }
+            ParFn all;
+            const ParFn pre = block_exscan_fn(F, &all);
+            const unsigned base = N + ((N & 1u) ? pre.t1 : pre.t0);
+            const unsigned pin = (base & 1u);
+            // this thread's run leaves the binade iff a term is big or its
+            // sum from `base` reaches 2^24 units (base itself is exact while
+            // every earlier run stays inside)
+            bool big = false;
+#pragma unroll
+            for (int i = 0; i < kNormPer; ++i) big |= (v[i] & kIncBig) != 0;
+            const unsigned mt = pin ? F.t1 : F.t0;
+            const bool leaves = big || base >= (1u << 24) || mt >= (1u << 24) - base;
+            if (!__syncthreads_or(leaves)) {           // the rest of the chunk stays in the binade
+                s = __builtin_ldexpf((float)(N + ((N & 1u) ? all.t1 : all.t0)), e - 23);
+                break;
+            }
+            // the first term that leaves the binade, in order
+            unsigned run = base;
+            int evc = INT_MAX;
+            unsigned at = 0;
+            if (leaves) {
+#pragma unroll
+                for (int i = 0; i < kNormPer; ++i) {
+                    const unsigned a = inc_at(v[i], run & 1u);
+                    if (evc == INT_MAX && mine + i >= start) {
+                        if ((v[i] & kIncBig) || run + a >= (1u << 24)) { evc = mine + i; at = run; }
+                        else run += a;
+                    }
+                }
+            }
+            const int j = block_min(evc);
+            if (j == evc) {                            // its thread adds the term that leaves as a float
+                const float sj = __builtin_ldexpf((float)at, e - 23);
+#pragma unroll
+                for (int i = 0; i < kNormPer; ++i)
+                    if (mine + i == j) s_s = sj + d[i];
+            }
+            __syncthreads();
+            s = s_s;
+            __syncthreads();
+            start = j + 1;
+        }
+    }
+    return s;
+}
+
+// The stopping test after an outer iteration: the reference continues iff
+// (float)sum / (float)(NX*NY) >= TTNORM with sum its float norm1 sum, i.e.
+// iff sum >= stop_s.  G, the double sum of the iteration's per-sweep
+// decreases, differs from the exact sum E of |T - Told| by at most 16 float
+// units of E (a float difference per cell and a float sum of at most 8 rows
+// per lane before the double sums), and the reference's float sum differs
+// from E by at most (1+u)(1+gamma_{NX*NY}) - 1: band_lo/band_hi hold those
+// bounds (host, set_stop).  G = 0 means no cell changed, where the float sum
+// is exactly 0.  Returns 1 (continue), 0 (stop) or -1 (undecided: count the
+// changed cells, stop_counted, and if that cannot decide either, take
+// ordered_norm1; see decide()).  EXACT: the decreases do not bound the
+// change (the INF guard can raise a value), so every decision takes the
+// ordered sum.
+template <bool EXACT = false>
+__device__ __forceinline__ int stop_fast(const ModelDev &md, double G)
+{
+    if (EXACT) return -1;
+    if (G == 0.0) return 0.0 >= md.stop_s ? 1 : 0;
+    if (G * md.band_hi < md.stop_s) return 0;
+    if (G * md.band_lo >= md.stop_s) return 1;
+    return -1;
+}
+
+// Cells whose value differs between two fields of layout A (Tstart ==
+// nullptr: the initial forward field): norm1's nonzero terms.  A coalesced
+// pass of the whole workgroup over both fields, one lane's R rows at one key
+// per load, eight loads of each field in flight per thread.
+template <int R>
+__device__ int count_changed(const ModelDev &md, const float *Tend_, const float *Tstart_, float tinit, int src_k,
+                             int src_c)
+{
+    constexpr int BR = kWave * R, U = 8;
+    __shared__ int cnt_s;
+    if (threadIdx.x == 0) cnt_s = 0;
+    __syncthreads();
+    const gfloat *Tend = gp(Tend_), *Tstart = gp(Tstart_);
+    const int gpb = (md.nx + 64) * kWave;   // groups of R rows per band
+    const int nt = blockDim.x;
+    int n = 0;
+    for (int b = 0; b < md.nb; ++b) {
+        const gfloat *e = Tend + (long long)b * md.bstride, *o = Tstart + (long long)b * md.bstride;
+        for (int q0 = threadIdx.x; q0 < gpb; q0 += U * nt) {
+            float te[U][R], ts[U][R];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = min(q0 + u * nt, gpb - 1);
+                ldv<R>(e + q * R, te[u]);
+                if (Tstart_) ldv<R>(o + q * R, ts[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = q0 + u * nt, key = q / kWave, l = q % kWave, c = key - l;
+                const bool colok = q < gpb && c >= 0 && c < md.nx;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int k = b * BR + l * R + r;
+                    const float t0 = Tstart_ ? ts[u][r] : ((k == src_k && c == src_c) ? 0.f : tinit);
+                    n += (colok && k < md.ny && te[u][r] != t0) ? 1 : 0;
+                }
+            }
+        }
+    }
+    atomicAdd(&cnt_s, n);
+    __syncthreads();
+    const int r = cnt_s;
+    __syncthreads();
+    return r;
+}
+
+// stop_fast's test with the bound of m nonzero terms instead of NX*NY.
+__device__ __forceinline__ int stop_counted(const ModelDev &md, double G, int m)
+{
+    const double u = 0x1p-24, mu = (double)m * u;
+    if (mu >= 0.5) return -1;
+    const double gam = mu / (1.0 - mu), er = (1.0 + u) * (1.0 + gam) - 1.0, eg = 16.0 * u + 1e-9;
+    if (G * ((1.0 + er) / (1.0 - eg) * (1.0 + 1e-9)) < md.stop_s) return 0;
+    if (G * ((1.0 - er) / (1.0 + eg) * (1.0 - 1e-9)) >= md.stop_s) return 1;
+    return -1;
+}
+
+__device__ __forceinline__ bool stop_exact(const ModelDev &md, float sum)
+{
+    const float denom = (float)(md.nx * md.ny);
+    return sum / denom >= md.ttnorm;   // norm1.c:27, eikonal.c:44
+}
+
+// The stopping decision after an outer iteration whose fields at the start
+// (Wi; nullptr: the initial forward field) and at the end (Wo) are intact:
+// the fast test, then the bound with `mbound` changed cells when the
+// sweeps counted them (mbound >= 0: the forward's count over its sweeps of
+// R cells per lane step with a change, at least the number of cells that
+// changed), then the bound with the exact
+// count of changed cells, then the ordered sum.  Every thread of the
+// workgroup calls it with the same G and gets the same answer.  `passes`
+// is set when the fields were read (a split shot's partner must not
+// overwrite Wi before both are done); nord counts the passes: 1 per count
+// of changed cells, 1 << 16 per ordered sum; osum is the ordered sum when
+// one was taken, else -1.  md.stop_mode 1 and 2 (tests) skip to the ordered
+// sum, resp. to the count, for every nonzero G.
+template <int R, bool EXACT>
+__device__ bool decide(const ModelDev &md, double G, double mbound, const float *Wo, const float *Wi, float tinit,
+                       int src_k, int src_c, bool &passes, int &nord, float &osum)
+{
+    osum = -1.f;
+    int d = stop_fast<EXACT>(md, G);
+    if (d < 0 && !EXACT && md.stop_mode == 0 && mbound >= 0.0) d = stop_counted(md, G, (int)fmin(mbound, 1e9));
+    passes = d < 0;
+    if (d >= 0) return d != 0;
+    if (!EXACT && md.stop_mode != 1) {
+        ++nord;
+        d = stop_counted(md, G, count_changed<R>(md, Wo, Wi, tinit, src_k, src_c));
+    }
+    if (d >= 0) return d != 0;
+    nord += 1 << 16;
+    osum = ordered_norm1<R>(md.nx, md.ny, md.bstride, Wo, Wi, tinit, src_k, src_c);
+    return stop_exact(md, osum);
 }
 
 // Steps of a sweep: the last band's keys plus `hold` steps to drain staged
@@ -548,7 +915,8 @@ __device__ __forceinline__ void pace_idle(const Pace &pc, int nep)
 //          through the staged stores of Stage
 //   stage  this wave's LDS delay line (Stage<R>::WAVE_FLOATS floats)
 //   INIT   read the initial field (2*TIME, 0 at the source) instead of Tin
-// Returns this lane's sum of (old - new) over its valid cells.
+// Returns this lane's sum of (old - new) over its valid cells and the number
+// of cells the wave changed.
 //
 // All loads are whole 64*R-float blocks of this sweep's layout and all
 // stores whole segments (Stage) of the other layout.  Garbage (values of
@@ -561,7 +929,7 @@ __device__ __forceinline__ void pace_idle(const Pace &pc, int nep)
 // vmcnt instead of draining the queue.
 // ---------------------------------------------------------------------------
 template <int R, int SX, int SY, bool INIT, int PACE = kPaceLock>
-__device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh_, double two_dh2_,
+__device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, double two_dh2_,
                                                       const float *Tin_, const float *Sf_, float *Tout_,
                                                       float *stage_, int src_k_, int src_c_, float tinit_,
                                                       const Pace pc_)
@@ -582,7 +950,7 @@ __device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh
     const int nep = gend / kPF;
     if (!w.active) {
         pace_idle<PACE>(pc, nep);
-        return 0.0;
+        return Dec{0.0, 0.0};
     }
     pace_begin<PACE>(pc);
     const int lag = SY > 0 ? l : 63 - l;
@@ -651,6 +1019,7 @@ __device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh
 #pragma unroll
     for (int r = 0; r < R; ++r) pnew[r] = tinit;
     double acc = 0.0;
+    unsigned chg = 0;        // steps at which some of this lane's cells changed
     float ring_next = 0.f;   // first consumer step is >= kBandOff steps away
 
     for (int g0 = 0; g0 < gend; g0 += kPF) {
@@ -705,6 +1074,7 @@ __device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh
                                            rvalid, cvalid, dh, two_dh2, nv, fsum, need);
                 }
                 acc += (double)fsum;
+                chg += fsum > 0.f ? 1u : 0u;   // a change makes the (non-negative) sum positive
 #pragma unroll
                 for (int r = 0; r < R; ++r) pnew[r] = nv[r];
             }
@@ -722,7 +1092,7 @@ __device__ __attribute__((noinline)) double fwd_sweep(const WaveCtx w_, float dh
     }
     pace_end<PACE>(pc, nep);
 #undef RZ_FWD_LOAD
-    return acc;
+    return Dec{acc, (double)chg * R};
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,10 +1412,11 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
 // Copy a field from layout A to layout B (one wave per band, 8 keys per
 // round): each lane gathers its R cells of B key `key` from A key
 // key + 2l - 63 (the A dummy block for cells outside the grid), then whole B
-// blocks are stored.
+// blocks are stored.  With fa2 != nullptr the A blocks are also copied to
+// fa2 (the converged field ended in the other A buffer).
 // ---------------------------------------------------------------------------
 template <int R>
-__device__ void copy_a_to_b(const WaveCtx &w, const float *fa, float *fb)
+__device__ void copy_a_to_b(const WaveCtx &w, const float *fa, float *fb, float *fa2 = nullptr)
 {
     constexpr int BR = kWave * R;
     if (!w.active) return;
@@ -1064,6 +1435,16 @@ __device__ void copy_a_to_b(const WaveCtx &w, const float *fa, float *fb)
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) stv<R>(dst + min(g + u, nkeys) * BR + l * R, v[u]);
+    }
+    if (fa2) {
+        gfloat *dst2 = gpw(fa2) + boff;
+        for (int g = 0; g < nkeys; g += 8) {
+            float v[8][R];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ldv<R>(src + min(g + u, nkeys) * BR + l * R, v[u]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) stv<R>(dst2 + min(g + u, nkeys) * BR + l * R, v[u]);
+        }
     }
 }
 
@@ -1131,19 +1512,23 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
     const int D = pair_delay(md.nb, Stage<R>::HOLD);
     const long long so = (long long)sh.slot * slots.slot_stride;
     float *TA = slots.TA + so, *TB = slots.TB + so;
+    // A-family output of iteration it: W[it & 1] (TA, then LA, then TA ...)
+    float *const W[2] = {TA, slots.LA + so};
     const float tinit = (float)(2.0 * (double)md.time);          // zero_travel.c:22
-    float lnorm = (float)(10.0 * (double)md.time);                // eikonal.c:39
-    const float denom = (float)(md.nx * md.ny);
+    bool cont = (float)(10.0 * (double)md.time) >= md.ttnorm;    // eikonal.c:39,44
     // Sweep order (+,+) A, (-,+) B, (-,-) A, (+,-) B (eikonal.c:50-53); each
     // reads its own family's layout and writes the next sweep's.  Sweeps 2+3 and
     // sweep 4 + the next iteration's sweep 1 run as pairs (same x
     // direction); that next sweep 1 is run ahead of the stopping test and
     // simply dropped if the loop ends.
-    int it = 0;
-    double acc_next = 0.0;   // group 1: decrease of the sweep 1 run ahead
-    while (lnorm >= md.ttnorm) {                                  // eikonal.c:44
+    int it = 0, last = -1, nord = 0;
+    float osum = -1.f;   // ordered sum of the last decision, if it took one
+    Dec acc_next{0.0, 0.0};  // group 1: the sweep 1 run ahead
+    while (cont) {
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
-        double acc = 0.0;
+        float *Wo = W[it & 1];
+        const float *Wi = it > 0 ? W[(it - 1) & 1] : nullptr;   // T at the start (null: the initial field)
+        Dec acc{0.0, 0.0};
         if (it == 0) {
             if (w.group == 0)
                 acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
@@ -1153,27 +1538,32 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         } else {
             acc = acc_next;
         }
-        if (w.group == 0)
-            acc += fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k, sh.src_c,
-                                               tinit, lockstep(0, D));
-        else
-            acc += fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
-                                               tinit, lockstep(D, 0));
+        const Dec a2 = w.group == 0
+            ? fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c, tinit,
+                                          lockstep(0, D))
+            : fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k, sh.src_c, tinit,
+                                          lockstep(D, 0));
+        acc.sum += a2.sum;
+        acc.cnt += a2.cnt;
         if (w.group == 0) {
-            acc += fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k, sh.src_c,
-                                               tinit, lockstep(0, D));
-            acc_next = 0.0;
+            const Dec a4 = fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k,
+                                                       sh.src_c, tinit, lockstep(0, D));
+            acc.sum += a4.sum;
+            acc.cnt += a4.cnt;
+            acc_next = Dec{0.0, 0.0};
         } else {
-            acc_next = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
+            acc_next = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k, sh.src_c,
                                                    tinit, lockstep(D, 0));
         }
-        const double tot = block_sum(acc, red, nwaves);
-        lnorm = (float)tot / denom;                               // norm1.c:27
-        ++it;
+        const Dec tot = block_sum2(acc, red, nwaves);
+        last = it++;
+        bool passes;                                              // norm1.c:27, eikonal.c:44
+        cont = decide<R, false>(md, tot.sum, tot.cnt, Wo, Wi, tinit, sh.src_k, sh.src_c, passes, nord, osum);
     }
     const int nfield = md.nb * (int)md.bstride;
-    if (it != 0) {  // the last sweep wrote layout A; the adjoint also needs B
-        if (w.group == 0) copy_a_to_b<R>(w, TA, TB);
+    if (last >= 0) {  // the last sweep wrote W[last & 1]; the adjoint reads TA and TB
+        float *F = W[last & 1];
+        if (w.group == 0) copy_a_to_b<R>(w, F, TB, F == TA ? nullptr : TA);
         __syncthreads();
     } else {        // loop never entered: the field is the initial one in both layouts
         for (int p = threadIdx.x; p < nfield; p += blockDim.x) {
@@ -1195,7 +1585,11 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
     }
     __syncthreads();
     mark_kept_cells<R>(md, TA, TB);
-    if (threadIdx.x == 0) out.it_f[shot_base[sh.model] + sh.shot] = it;
+    if (threadIdx.x == 0) {
+        out.it_f[shot_base[sh.model] + sh.shot] = it;
+        out.ord_f[shot_base[sh.model] + sh.shot] = nord;
+        out.osum_f[shot_base[sh.model] + sh.shot] = osum;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1237,28 +1631,40 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
         LA[addr_a<R>(md.uq_k[u], md.uq_c[u], md.bstride)] = tres[ob + md.uq_r[u]];
     __syncthreads();
 
-    float lnorm = big;                                            // eikonal_adj.c:34
-    const float denom = (float)(md.nx * md.ny);
-    int it = 0;
-    while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
+    // A-family output of iteration it: W[it & 1] (LC, then LA, then LC ...);
+    // the first iteration starts from the initial lambda in LA
+    float *const W[2] = {slots.LC + so, LA};
+    bool cont = big >= md.ttnorm;                                 // eikonal_adj.c:34,37
+    int it = 0, last = -1, nord = 0;
+    float osum = -1.f;   // ordered sum of the last decision, if it took one
+    while (cont) {
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
+        float *Wo = W[it & 1];
+        const float *Wi = it > 0 ? W[(it - 1) & 1] : LA;           // lambda at the start
         // (+,+) A, (+,-) B, (-,+) B, (-,-) A (eikonal_adj.c:43-46): sweep 1
         // reads lambda in A and writes B, sweep 2 updates B in place, sweep 3
         // reads B and writes A, sweep 4 updates A in place; sweeps 1+2 and
         // 3+4 run as pairs (same x direction).
         double acc;
         if (w.group == 0) {
-            acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, LA, LB, stage, lockstep(0, D));
-            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LA, stage, lockstep(0, D));
+            acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wi, LB, stage, lockstep(0, D));
+            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, Wo, stage, lockstep(0, D));
         } else {
             acc = adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LB, stage, lockstep(D, 0));
-            acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, LA, LA, stage, lockstep(D, 0));
+            acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wo, Wo, stage, lockstep(D, 0));
         }
         const double tot = block_sum(acc, red, nwaves);
-        lnorm = (float)tot / denom;
-        ++it;
+        last = it++;
+        bool passes;                                              // norm1.c:27, eikonal_adj.c:37
+        cont = decide<R, CHKINF>(md, tot, -1.0, Wo, Wi, 0.f, -1, -1, passes, nord, osum);
     }
-    if (threadIdx.x == 0) out.it_a[shot_base[sh.model] + sh.shot] = it;
+    if (threadIdx.x == 0) {
+        const int g = shot_base[sh.model] + sh.shot;
+        out.it_a[g] = it;
+        out.lsel[g] = (last >= 0 && (last & 1) == 0) ? 1 : 0;   // the converged lambda is in LC
+        out.ord_a[g] = nord;
+        out.osum_a[g] = osum;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1339,6 +1745,24 @@ __device__ double pair_sum(double v, double *red, int nwaves, double *part, int 
     return t;
 }
 
+// pair_sum of the decreases, with the waves' counts summed alongside (part
+// holds the counts after the 2*kMaxBands partial sums).
+__device__ Dec pair_sum2(Dec v, double *red, int nwaves, double *part, int *cnt, int role, int phase, int *err,
+                         bool &ok)
+{
+    double c = v.cnt;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) part[2 * kMaxBands + role * kMaxBands + (threadIdx.x >> 6)] = c;
+    Dec r;
+    r.sum = pair_sum(v.sum, red, nwaves, part, cnt, role, phase, err, ok);
+    double t = 0.0;
+    for (int i = 0; i < nwaves; ++i) t += part[2 * kMaxBands + i] + part[3 * kMaxBands + i];
+    __syncthreads();
+    r.cnt = t;
+    return r;
+}
+
 __device__ __forceinline__ bool split_role(int nshots, int &shot, int &role)
 {
     const int b = blockIdx.x;
@@ -1361,7 +1785,7 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
     const ModelDev md = mds[sh.model];
     const int nwaves = blockDim.x >> 6;
     int *cnt = ss.cnt + (long long)s * kSyncInts;
-    double *part = ss.part + (long long)s * 2 * kMaxBands;
+    double *part = ss.part + (long long)s * kSyncPart;
     if (!pair_handshake(cnt, role, ss.err)) return;
     const WaveCtx w = make_wave_split(md, ring, role);
     float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
@@ -1371,18 +1795,21 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
     pc.D = pair_delay(md.nb, Stage<R>::HOLD);
     const long long so = (long long)sh.slot * slots.slot_stride;
     float *TA = slots.TA + so, *TB = slots.TB + so;
+    float *const W[2] = {TA, slots.LA + so};                      // as in k_forward
     const float tinit = (float)(2.0 * (double)md.time);          // zero_travel.c:22
-    float lnorm = (float)(10.0 * (double)md.time);                // eikonal.c:39
-    const float denom = (float)(md.nx * md.ny);
+    bool cont = (float)(10.0 * (double)md.time) >= md.ttnorm;    // eikonal.c:39,44
     // The sweeps and pairs of k_forward: role 0 runs sweep 1 of the first
     // iteration alone, then sweeps 2 and 4; role 1 runs sweeps 3 and the
     // next iteration's sweep 1.
-    int it = 0, phase = 0;
+    int it = 0, last = -1, phase = 0, nord = 0;
+    float osum = -1.f;
     bool ok = true;
-    double acc_next = 0.0;
-    while (lnorm >= md.ttnorm) {                                  // eikonal.c:44
+    Dec acc_next{0.0, 0.0};
+    while (cont) {
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
-        double acc = 0.0;
+        float *Wo = W[it & 1];
+        const float *Wi = it > 0 ? W[(it - 1) & 1] : nullptr;
+        Dec acc{0.0, 0.0};
         if (it == 0) {
             if (role == 0) {
                 pc.base = phase * kPhaseStride;
@@ -1395,32 +1822,43 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
             acc = acc_next;
         }
         pc.base = phase * kPhaseStride;
-        if (role == 0)
-            acc += fwd_sweep<R, -1, +1, false, kPaceLead>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k,
-                                                          sh.src_c, tinit, pc);
-        else
-            acc += fwd_sweep<R, -1, -1, false, kPaceTrail>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k,
-                                                           sh.src_c, tinit, pc);
+        const Dec a2 = role == 0
+            ? fwd_sweep<R, -1, +1, false, kPaceLead>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k,
+                                                     sh.src_c, tinit, pc)
+            : fwd_sweep<R, -1, -1, false, kPaceTrail>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k,
+                                                      sh.src_c, tinit, pc);
+        acc.sum += a2.sum;
+        acc.cnt += a2.cnt;
         ok = rendezvous(cnt, role, phase++, ss.err);
         if (!ok) break;
         pc.base = phase * kPhaseStride;
         if (role == 0) {
-            acc += fwd_sweep<R, +1, -1, false, kPaceLead>(w, md.dh, md.two_dh2, TB, md.SB, TA, stage, sh.src_k,
-                                                          sh.src_c, tinit, pc);
-            acc_next = 0.0;
+            const Dec a4 = fwd_sweep<R, +1, -1, false, kPaceLead>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage,
+                                                                  sh.src_k, sh.src_c, tinit, pc);
+            acc.sum += a4.sum;
+            acc.cnt += a4.cnt;
+            acc_next = Dec{0.0, 0.0};
         } else {
-            acc_next = fwd_sweep<R, +1, +1, false, kPaceTrail>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage,
+            acc_next = fwd_sweep<R, +1, +1, false, kPaceTrail>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage,
                                                                sh.src_k, sh.src_c, tinit, pc);
         }
-        const double tot = pair_sum(acc, red, nwaves, part, cnt, role, phase++, ss.err, ok);
+        const Dec tot = pair_sum2(acc, red, nwaves, part, cnt, role, phase++, ss.err, ok);
         if (!ok) break;
-        lnorm = (float)tot / denom;                               // norm1.c:27
-        ++it;
+        last = it++;
+        bool passes;                                              // norm1.c:27, eikonal.c:44
+        cont = decide<R, false>(md, tot.sum, tot.cnt, Wo, Wi, tinit, sh.src_k, sh.src_c, passes, nord, osum);
+        if (passes) {
+            // both workgroups read the whole grid alike; they meet before the
+            // next iteration's sweep 2 overwrites Wi
+            ok = rendezvous(cnt, role, phase++, ss.err);
+            if (!ok) break;
+        }
     }
     if (role != 0) return;
     const int nfield = md.nb * (int)md.bstride;
-    if (it != 0) {  // the last sweep wrote layout A; the adjoint also needs B
-        copy_a_to_b<R>(w, TA, TB);
+    if (last >= 0) {  // the last sweep wrote W[last & 1]; the adjoint reads TA and TB
+        float *F = W[last & 1];
+        copy_a_to_b<R>(w, F, TB, F == TA ? nullptr : TA);
         __syncthreads();
     } else {        // loop never entered: the field is the initial one in both layouts
         for (int p = threadIdx.x; p < nfield; p += blockDim.x) {
@@ -1441,7 +1879,11 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
     }
     __syncthreads();
     mark_kept_cells<R>(md, TA, TB);
-    if (threadIdx.x == 0) out.it_f[shot_base[sh.model] + sh.shot] = it;
+    if (threadIdx.x == 0) {
+        out.it_f[shot_base[sh.model] + sh.shot] = it;
+        out.ord_f[shot_base[sh.model] + sh.shot] = nord;
+        out.osum_f[shot_base[sh.model] + sh.shot] = osum;
+    }
 }
 
 template <int R, int FD, bool CHKINF>
@@ -1460,7 +1902,7 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
     const ModelDev md = mds[sh.model];
     const int nwaves = blockDim.x >> 6;
     int *cnt = ss.cnt + (long long)s * kSyncInts;
-    double *part = ss.part + (long long)s * 2 * kMaxBands;
+    double *part = ss.part + (long long)s * kSyncPart;
     if (!pair_handshake(cnt, role, ss.err)) return;
     const WaveCtx w = make_wave_split(md, ring, role);
     float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
@@ -1486,16 +1928,19 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
             LA[addr_a<R>(md.uq_k[u], md.uq_c[u], md.bstride)] = tres[ob + md.uq_r[u]];
         __syncthreads();
     }
-    float lnorm = big;                                            // eikonal_adj.c:34
-    const float denom = (float)(md.nx * md.ny);
-    int it = 0, phase = 0;
+    float *const W[2] = {slots.LC + so, LA};                      // as in k_adjoint
+    bool cont = big >= md.ttnorm;                                 // eikonal_adj.c:34,37
+    int it = 0, last = -1, phase = 0, nord = 0;
+    float osum = -1.f;
     bool ok = true;
-    while (lnorm >= md.ttnorm) {                                  // eikonal_adj.c:37
+    while (cont) {
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
+        float *Wo = W[it & 1];
+        const float *Wi = it > 0 ? W[(it - 1) & 1] : LA;
         double acc;
         pc.base = phase * kPhaseStride;
         if (role == 0)
-            acc = adj_sweep<R, +1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, LA, LB,
+            acc = adj_sweep<R, +1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wi, LB,
                                                                       stage, pc);
         else
             acc = adj_sweep<R, +1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LB,
@@ -1504,17 +1949,28 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
         if (!ok) break;
         pc.base = phase * kPhaseStride;
         if (role == 0)
-            acc += adj_sweep<R, -1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LA,
+            acc += adj_sweep<R, -1, +1, false, FD, CHKINF, kPaceLead>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, Wo,
                                                                        stage, pc);
         else
-            acc += adj_sweep<R, -1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, LA, LA,
+            acc += adj_sweep<R, -1, -1, true, FD, CHKINF, kPaceTrail>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wo, Wo,
                                                                        stage, pc);
         const double tot = pair_sum(acc, red, nwaves, part, cnt, role, phase++, ss.err, ok);
         if (!ok) break;
-        lnorm = (float)tot / denom;
-        ++it;
+        last = it++;
+        bool passes;                                              // norm1.c:27, eikonal_adj.c:37
+        cont = decide<R, CHKINF>(md, tot, -1.0, Wo, Wi, 0.f, -1, -1, passes, nord, osum);
+        if (passes) {   // as in k_forward_split
+            ok = rendezvous(cnt, role, phase++, ss.err);
+            if (!ok) break;
+        }
     }
-    if (role == 0 && threadIdx.x == 0) out.it_a[shot_base[sh.model] + sh.shot] = it;
+    if (role == 0 && threadIdx.x == 0) {
+        const int g = shot_base[sh.model] + sh.shot;
+        out.it_a[g] = it;
+        out.lsel[g] = (last >= 0 && (last & 1) == 0) ? 1 : 0;
+        out.ord_a[g] = nord;
+        out.osum_a[g] = osum;
+    }
 }
 
 // Zero the split shots' counters before a launch (a kernel, so that graphs
@@ -1638,7 +2094,7 @@ __global__ void k_eval_init(float *carry, const float *carry0, int ntr_total, do
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ void k_gradient(const ModelDev *__restrict__ mds, int model, const int *__restrict__ slot_of,
-                           int nsh, Slots slots, float *grad, int accumulate)
+                           const int *__restrict__ lsel, int nsh, Slots slots, float *grad, int accumulate)
 {
     constexpr int BR = kWave * R;
     const ModelDev md = mds[model];
@@ -1656,7 +2112,8 @@ __global__ void k_gradient(const ModelDev *__restrict__ mds, int model, const in
         const double s3 = s * s * s;
         float g = accumulate ? grad[(long long)k * md.nx + c] : 0.f;
         for (int j = 0; j < nsh; ++j) {
-            const float lam = slots.LA[(long long)slot_of[j] * slots.slot_stride + pos];
+            const float *L = lsel[j] ? slots.LC : slots.LA;   // where the shot's lambda converged
+            const float lam = L[(long long)slot_of[j] * slots.slot_stride + pos];
             g += (float)((double)lam * s3);
         }
         grad[(long long)k * md.nx + c] = g;
@@ -1692,9 +2149,10 @@ __global__ void k_prepare_slowness(const ModelDev *__restrict__ mds, int model, 
 
 // Layout-A field -> row-major (optionally clearing the receiver sign bits).
 template <int R>
-__global__ void k_field_to_rowmajor(int nx, int ny, long long bstride, const float *field, float *out,
-                                    int strip)
+__global__ void k_field_to_rowmajor(int nx, int ny, long long bstride, const float *field, const float *alt,
+                                    const int *sel, float *out, int strip)
 {
+    if (sel && *sel) field = alt;
     const long long n = (long long)nx * ny;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
@@ -1846,20 +2304,21 @@ hipError_t launch_eval_init(float *carry, const float *carry0, int ntr_total, do
     return hipGetLastError();
 }
 
-hipError_t launch_gradient(const ModelDev *mh, int model, const ModelDev *md, const int *slot_of, int nsh,
-                           Slots slots, float *grad, int accumulate, hipStream_t st)
+hipError_t launch_gradient(const ModelDev *mh, int model, const ModelDev *md, const int *slot_of, const int *lsel,
+                           int nsh, Slots slots, float *grad, int accumulate, hipStream_t st)
 {
     const long long n = (long long)mh->nb * mh->bstride;
     RZ_DISPATCH_R(mh->R, hipLaunchKernelGGL(k_gradient<RR>, dim3(grid_for(n, 256)), dim3(256), 0, st, md, model,
-                                            slot_of, nsh, slots, grad, accumulate));
+                                            slot_of, lsel, nsh, slots, grad, accumulate));
     return hipGetLastError();
 }
 
-hipError_t launch_field_to_rowmajor(const ModelDev *mh, const float *field, float *out, int strip, hipStream_t st)
+hipError_t launch_field_to_rowmajor(const ModelDev *mh, const float *field, const float *alt, const int *sel,
+                                    float *out, int strip, hipStream_t st)
 {
     const long long n = (long long)mh->nx * mh->ny;
     RZ_DISPATCH_R(mh->R, hipLaunchKernelGGL(k_field_to_rowmajor<RR>, dim3(grid_for(n, 256)), dim3(256), 0, st,
-                                            mh->nx, mh->ny, mh->bstride, field, out, strip));
+                                            mh->nx, mh->ny, mh->bstride, field, alt, sel, out, strip));
     return hipGetLastError();
 }
 

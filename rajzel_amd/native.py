@@ -26,6 +26,7 @@ EXPORTS = (
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
     "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb", "rz_set_split", "rz_get_split",
     "rz_read_picks", "rz_picks_parses", "rz_picks_clear", "rz_gauss_filt_noalloc",
+    "rz_slot_bytes", "rz_get_stop_passes",
 )
 
 
@@ -93,6 +94,8 @@ def lib():
                 "rz_rows_per_lane": (i, [vp]),
                 "rz_device_bytes": (C.c_size_t, [vp]),
                 "rz_batch_shots": (i, [vp]),
+                "rz_slot_bytes": (C.c_size_t, [vp]),
+                "rz_get_stop_passes": (i, [vp, i, vp, vp, vp, vp, vp, vp]),
                 "rz_set_profiling": (i, [vp, i]),
                 "rz_get_profile": (i, [vp, vp, vp]),
                 "rz_set_residuals": (i, [vp, i, vp, i]),
@@ -345,6 +348,18 @@ class Context:
         self._chk(lib().rz_get_iterations(self.h, model, _ptr(f), _ptr(a)))
         return f[:ns], a[:ns]
 
+    def stop_passes(self, model=0):
+        """Per shot, the stopping decisions of the last run that read the
+        fields: counts of changed cells ("count_f", "count_a"), norm1's
+        ordered float sums ("ordered_f", "ordered_a"), and the ordered sum of
+        the last iteration's decision where one was taken, else -1 ("sum_f",
+        "sum_a"); forward and adjoint."""
+        ns = self.shapes[model][2]
+        keys = ("count_f", "count_a", "ordered_f", "ordered_a", "sum_f", "sum_a")
+        out = {k: np.empty(max(ns, 1), np.float32 if k.startswith("sum") else np.int32) for k in keys}
+        self._chk(lib().rz_get_stop_passes(self.h, model, *(_ptr(out[k]) for k in keys)))
+        return {k: v[:ns] for k, v in out.items()}
+
     def tres_carry(self, model=0):
         ntr = self.shapes[model][3]
         out = np.empty(max(ntr, 1), np.float32)
@@ -417,3 +432,8 @@ class Context:
     @property
     def batch_shots(self):
         return lib().rz_batch_shots(self.h)
+
+    @property
+    def slot_bytes(self):
+        """Device bytes of one shot's fields: the unit of an hbm_budget."""
+        return lib().rz_slot_bytes(self.h)

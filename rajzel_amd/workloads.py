@@ -238,6 +238,72 @@ def small_case(nx=53, ny=37, dh=10.0, nshots=3, seed=7, ntr=None, rec_row=2):
     )
 
 
+def edge_case(nx, ny, dh=10.0, nshots=3, seed=61, sigma=3.0, ttnorm=1e-4, rec_step=7):
+    """Receiver edge cases on a rough model, for fixtures where the stopping
+    test ends the solves before a fixed point.
+
+    Velocity 1500 + 3000*minmax(gaussian_filter(U(0,1), sigma)) (rough for a
+    small sigma, so that several outer iterations change cells), picks from a
+    2%-faster copy.  Receivers, 0-based (row, column):
+      * row 1 (the reference's row 2) every ``rec_step`` columns;
+      * two receivers on one cell of row 1 and two on one cell at depth
+        (init_recflag.c:24-30, init_travel_adj.c:49-55: the last one wins);
+      * the top row 0 and the bottom row NY-1 (frame rows: no adjoint
+        source, init_travel_adj.c:44; the bottom row's residual takes the
+        backward slope, calc_FA_res.c:62-68);
+      * the frame columns 0 and NX-1, and a line at depth NY/2.
+    Sources at random surface columns (row 0) and one at depth."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    f = gaussian_filter(rng.uniform(0.0, 1.0, (ny, nx)), sigma=sigma)
+    f = (f - f.min()) / (f.max() - f.min())
+    vp = (1500.0 + 3000.0 * f).astype(np.float32)
+    cols = list(range(1, nx - 1, rec_step))
+    rows = [1] * len(cols)
+    dup_c, deep = cols[len(cols) // 2], ny // 2
+    cells = [(1, dup_c), (deep, 2), (deep, 2), (1, dup_c),        # duplicates, interleaved
+             (0, 3), (0, nx // 2), (ny - 1, 4), (ny - 1, nx - 3), # frame rows
+             (ny // 3, 0), (2 * ny // 3, nx - 1)]                  # frame columns
+    cells += [(deep, c) for c in range(3, nx - 1, max(1, 3 * rec_step))]
+    rows += [k for k, _ in cells]
+    cols += [c for _, c in cells]
+    rk = np.asarray(rows, np.float64) + 1.0                     # 1-based
+    rc = np.asarray(cols, np.float64) + 1.0
+    scol = rng.integers(2, nx, size=nshots - 1)
+    sx = np.concatenate([scol, [nx // 2]]).astype(np.float64)
+    sz = np.concatenate([np.ones(nshots - 1), [ny // 2 + 1]])
+    return Survey(
+        name=f"edge_{ny}x{nx}", nx=nx, ny=ny, dh=dh, vp=vp,
+        vp_true=(vp * np.float32(1.02)).astype(np.float32),
+        src_x_m=(sx * dh).astype(np.float32), src_z_m=(sz * dh).astype(np.float32),
+        rec_x_m=(rc * dh).astype(np.float32), rec_z_m=(rk * dh).astype(np.float32),
+        ttnorm=ttnorm, noise_s=0.001, meta={"seed": seed},
+    )
+
+
+def c5_gradient(nshots=400, ntr=3000, seed=5, nx=6000, ny=2000, dh=2.5):
+    """2000 x 6000, DH = 2.5 m.  Truth = 1-D gradient 1500 + 0.6 z plus 40
+    seeded Gaussians (U(-400, 400) m/s, radius U(50, 300) m); the current
+    model is the 1-D gradient.  400 shots every 37.5 m (x = 37.5 k - 18.75
+    m) at z = 2.5 m (row 1); 3000 receivers every 5 m at z = 5 m (row 2).
+    Picks = truth times + N(0, 1 ms)."""
+    rng = np.random.default_rng(seed)
+    X, Z = _grid_coords(nx, ny, dh)
+    base = 1500.0 + 0.6 * Z
+    anom = _gaussians(rng, X, Z, 40, -400.0, 400.0, 50.0, 300.0, dh, nx, ny)
+    return Survey(
+        name=f"c5_gradient_{ny}x{nx}", nx=nx, ny=ny, dh=dh,
+        vp=base.astype(np.float32),
+        vp_true=np.maximum(base + anom, 1000.0).astype(np.float32),
+        src_x_m=(37.5 * np.arange(1, nshots + 1) - 18.75).astype(np.float32),
+        src_z_m=np.full(nshots, dh, np.float32),
+        rec_x_m=(5.0 * np.arange(1, ntr + 1)).astype(np.float32),
+        rec_z_m=np.full(ntr, 2 * dh, np.float32),
+        noise_s=0.001, meta={"seed": seed},
+    )
+
+
 def synthesize_picks(survey: Survey, forward_fn, seed=None):
     """Fill ``survey.tobs`` with forward times in ``vp_true`` plus seeded
     Gaussian noise.  ``forward_fn(S, survey) -> (nshots, ntr) float32``."""

@@ -19,6 +19,12 @@ Cases
   eval_bands.npz  the same on a 150x70 grid (several row bands on the GPU)
   eval_surface.npz  surface receivers only (row 2), as the reference's own
                   example acquisition
+  eval_early_wide.npz, eval_early_wide6.npz, eval_early_tall.npz
+                  grad_obj's shot loop on rough models (workloads.edge_case)
+                  at TTNORM 1e-4 / 1e-6 (200x600) and 1e-6 (1001x29): the
+                  forward and adjoint solves stop before a fixed point;
+                  receivers duplicated on one cell, on the top and bottom
+                  frame rows, on the frame columns and at depth
   c1_forward.npz  eikonal() of the 200x600 forward configuration: receiver
                   times, iterations and the full traveltime grid
   gauss_filt.npz  gauss_filt() of a random 37x53 field and of a gradient-like
@@ -104,6 +110,29 @@ def evaluation(ref, name, nx, ny, nshots, seed, mute, rec_row):
         it_f=out["it_f"], it_a=out["it_a"])
 
 
+def evaluation_survey(ref, name, sv, mute, seed):
+    """grad_obj on a workload Survey (its TTNORM), picks from the reference's
+    forward in the survey's true model plus N(0, noise), muted picks."""
+    ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    St = W.slowness(sv.vp_true)
+    tm = np.stack([ref.eikonal(St, float(sv.src_x_m[s]), float(sv.src_z_m[s]), sv.rec_ix, sv.rec_iy)[1]
+                   for s in range(sv.nshots)])
+    rng = np.random.default_rng(seed)
+    tobs = (tm + rng.normal(0, sv.noise_s, tm.shape)).astype(np.float32)
+    for (s, i) in mute:
+        tobs[s, i] = -1.0
+    tres0 = np.linspace(-1e-3, 1e-3, sv.ntr).astype(np.float32)
+    out = ref.grad_obj(sv.S, sv.src_x_m, sv.src_z_m, sv.rec_ix, sv.rec_iy, tobs, tres0=tres0)
+    np.savez_compressed(
+        os.path.join(OUT, f"{name}.npz"),
+        nx=sv.nx, ny=sv.ny, dh=np.float32(sv.dh), time=np.float32(sv.time), ttnorm=np.float32(sv.ttnorm),
+        eps_adj=np.float32(sv.eps_adj), S=sv.S, src_x_m=sv.src_x_m, src_z_m=sv.src_z_m,
+        src_ix=sv.src_ix, src_iy=sv.src_iy, rec_ix=sv.rec_ix, rec_iy=sv.rec_iy, tres0=tres0,
+        tobs=out["tobs"], grad=out["grad"], T_last=out["T"], lam_last=out["lam"], tmod=out["tmod"],
+        tres=out["tres"], tres_carry=out["tres_carry"], l2_shot=out["l2_shot"], L2=np.float64(out["L2"]),
+        it_f=out["it_f"], it_a=out["it_a"])
+
+
 def c1_forward(ref):
     sv = W.c1_forward()
     ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
@@ -138,6 +167,11 @@ def main():
     evaluation(ref, "eval_small", 53, 37, 4, 31, [(1, 5), (2, 5), (2, 11)], (2, 36))
     evaluation(ref, "eval_bands", 70, 150, 3, 41, [(0, 0), (2, 69)], (2, 120, 149))
     evaluation(ref, "eval_surface", 53, 37, 3, 51, [(1, 7)], 2)
+    # solves that stop before a fixed point, with receiver edge cases
+    evaluation_survey(ref, "eval_early_wide", W.edge_case(600, 200, ttnorm=1e-4), [(0, 3), (1, 3), (2, 40)], 71)
+    evaluation_survey(ref, "eval_early_wide6", W.edge_case(600, 200, ttnorm=1e-6, seed=62), [(1, 0)], 72)
+    evaluation_survey(ref, "eval_early_tall", W.edge_case(29, 1001, rec_step=3, ttnorm=1e-6, sigma=1.0),
+                      [(2, 5)], 73)
     c1_forward(ref)
     gauss_filt(ref)
     for f in sorted(os.listdir(OUT)):

@@ -19,12 +19,20 @@ def _model(d, **kw):
     return m
 
 
-@pytest.mark.parametrize("name,R", [("eval_small", 1), ("eval_small", 4), ("eval_bands", 1), ("eval_bands", 2),
-                                    ("eval_surface", 4)])
-def test_evaluation_matches_reference(name, R):
+@pytest.mark.parametrize("name,R,ordered", [
+    ("eval_small", 1, 0), ("eval_small", 4, 0), ("eval_bands", 1, 0), ("eval_bands", 2, 0),
+    ("eval_surface", 4, 0),
+    # solves stopped before a fixed point; `ordered` 1: every stopping
+    # decision takes norm1's float sum in the reference's order, 2: every one
+    # counts the changed cells first
+    ("eval_early_wide", 1, 0), ("eval_early_wide", 4, 1), ("eval_early_wide", 2, 2), ("eval_early_wide6", 2, 0),
+    ("eval_early_wide6", 2, 1), ("eval_early_wide6", 1, 2), ("eval_early_tall", 4, 0), ("eval_early_tall", 8, 1),
+    ("eval_early_tall", 2, 1), ("eval_early_tall", 4, 2)])
+def test_evaluation_matches_reference(name, R, ordered, monkeypatch):
     from rajzel_amd import native as N
     d = np.load(os.path.join(GOLD, f"{name}.npz"))
-    print(f"[case] golden {name} R={R}", flush=True)
+    print(f"[case] golden {name} R={R} ordered={ordered}", flush=True)
+    monkeypatch.setenv("RAJZEL_AMD_ORDERED_NORM", str(int(ordered)))
     with N.Context([_model(d)], rows_per_lane=R) as ctx:
         ctx.set_slowness(0, d["S"])
         ctx.set_picks(0, d["tobs"])

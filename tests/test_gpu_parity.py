@@ -133,11 +133,10 @@ def test_per_shot_and_budget_equal_batched(ora):
         g_p = ctx.grad(0)
         assert bits_equal(g_b, g_p)
         _check_grad(ora, sv, ctx)
-    # a budget that fits two shots' fields (4 float and 2 double fields of
-    # nb*(nx+64)*64R elements): 3 batches
+    # a budget that fits two shots' fields: 3 batches
     R = 1
-    nb = -(-sv.ny // (64 * R))
-    slot = -(-(nb * (sv.nx + 64) * 64 * R) // 64) * 64 * (4 * 4)
+    with N.Context([model_dict(sv, max_iter=50)], rows_per_lane=R) as ctx:
+        slot = ctx.slot_bytes
     with N.Context([model_dict(sv, max_iter=50)], rows_per_lane=R, hbm_budget=2 * slot) as ctx:
         assert ctx.batch_shots == 2
         ctx.set_slowness(0, sv.S)

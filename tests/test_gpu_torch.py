@@ -44,10 +44,7 @@ def test_backward_recomputes_under_hbm_budget(ora):
     TF.misfit(vp, full, tobs).backward()
     g_full = vp.grad.clone()
     vp.grad = None
-    R = full.ctx.rows_per_lane
-    nb = -(-sv.ny // (64 * R))
-    slot = -(-(nb * (sv.nx + 64) * 64 * R) // 64) * 64 * (4 * 4)
-    small = TF.Solver(sv, hbm_budget=2 * slot)
+    small = TF.Solver(sv, hbm_budget=2 * full.ctx.slot_bytes)
     assert small.ctx.batch_shots == 2
     TF.misfit(vp, small, tobs).backward()
     assert bits_equal(vp.grad.cpu().numpy(), g_full.cpu().numpy())

@@ -39,7 +39,10 @@ def test_gradient_assembly(ora):
     assert bits_equal(gs, d["grad_shot"]) and bits_equal(g, d["grad"])
 
 
-@pytest.mark.parametrize("name", ["eval_small", "eval_bands", "eval_surface"])
+EVALS = ["eval_small", "eval_bands", "eval_surface", "eval_early_wide", "eval_early_wide6", "eval_early_tall"]
+
+
+@pytest.mark.parametrize("name", EVALS)
 def test_evaluation(ora, name):
     d = load(f"{name}.npz")
     o = ora.grad_obj(d["S"], d["src_ix"], d["src_iy"], d["rec_ix"], d["rec_iy"], d["tobs"], float(d["dh"]),
@@ -49,6 +52,24 @@ def test_evaluation(ora, name):
         assert bits_equal(o[k], d[kr]), f"{k}: {ndiff(o[k], d[kr])} values differ"
     assert o["L2"] == float(d["L2"])
     assert np.array_equal(o["it_f"], d["it_f"]) and np.array_equal(o["it_a"], d["it_a"])
+
+
+@pytest.mark.parametrize("name", ["eval_early_wide", "eval_early_wide6", "eval_early_tall"])
+def test_early_fixtures_stop_before_a_fixed_point(ora, name):
+    """The reference's stopping test ends these solves while cells still
+    change: with TTNORM 1e-30 (iterate to a fixed point) every forward solve
+    and at least one adjoint solve take more outer iterations than the
+    fixture records.  The fixtures also hold duplicate receivers, frame-row
+    and frame-column receivers."""
+    d = load(f"{name}.npz")
+    o = ora.grad_obj(d["S"], d["src_ix"], d["src_iy"], d["rec_ix"], d["rec_iy"], d["tobs"], float(d["dh"]),
+                     float(d["time"]), 1e-30, float(d["eps_adj"]), tres0=d["tres0"])
+    assert np.all(o["it_f"] > d["it_f"]) and np.any(o["it_a"] > d["it_a"])
+    cells = list(zip(d["rec_iy"].tolist(), d["rec_ix"].tolist()))
+    ny, nx = d["S"].shape
+    assert len(set(cells)) < len(cells)                                   # duplicates
+    assert any(k == 0 for k, _ in cells) and any(k == ny - 1 for k, _ in cells)
+    assert any(c == 0 for _, c in cells) and any(c == nx - 1 for _, c in cells)
 
 
 def test_c1_forward(ora):
