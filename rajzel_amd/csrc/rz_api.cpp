@@ -897,6 +897,14 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     if ((rc = dalloc(c, &c->TA, sf)) || (rc = dalloc(c, &c->TB, sf)) || (rc = dalloc(c, &c->LA, sf)) ||
         (rc = dalloc(c, &c->LB, sf)) || (rc = dalloc(c, &c->LC, sf)))
         return bail(rc);
+    {   // RAJZEL_AMD_POISON=1: every field starts as NaN, so that a grid cell
+        // that a kernel reads before any kernel wrote it shows in the results
+        const char *env = getenv("RAJZEL_AMD_POISON");
+        if (env && env[0] == '1')
+            for (float *f : {c->TA, c->TB, c->LA, c->LB, c->LC})
+                if (hipMemset(f, 0xFF, sf * sizeof(float)) != hipSuccess)
+                    return bail(fail(c, RZ_ERR_HIP, "poisoning the fields failed"));
+    }
     {
         const size_t nsh = (size_t)std::max(1, shots);
         if ((rc = dalloc(c, &c->ss.cnt, nsh * kSyncInts)) || (rc = dalloc(c, &c->ss.part, nsh * kSyncPart)) ||
@@ -1227,9 +1235,13 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     HIPCHK(c, hipMemcpyAsync(c->s_row, T, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     // receiver and frame cells carry the sign bit in the adjoint's copy of T
     // (as mark_kept_cells sets it after a forward)
-    std::vector<float> marked(T, T + n);
+    // |T| everywhere (a -0 passes the check above and must not read as a
+    // mark), then the sign bit on the cells that keep their value
+    std::vector<float> marked(n);
+    for (size_t i = 0; i < n; ++i) marked[i] = std::fabs(T[i]);
     const int nx = h.d.nx, ny = h.d.ny;
-    auto keep = [&](size_t i) { marked[i] = -std::fabs(marked[i]); };   // T >= 0: -T, and -0 for 0
+    // -|T| (-0 for 0); idempotent, since the frame corners are marked twice
+    auto keep = [&](size_t i) { marked[i] = -std::fabs(marked[i]); };
     for (size_t u = 0; u < h.uq_k.size(); ++u) keep((size_t)h.uq_k[u] * nx + h.uq_c[u]);
     for (int c = 0; c < nx; ++c) { keep((size_t)c); keep((size_t)(ny - 1) * nx + c); }
     for (int k = 0; k < ny; ++k) { keep((size_t)k * nx); keep((size_t)k * nx + nx - 1); }

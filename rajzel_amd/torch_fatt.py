@@ -16,7 +16,10 @@ which is what the reference's descent direction (descent.c:22) relies on.
 
 The forward keeps the converged traveltime fields of every shot on the
 device; when they do not fit the context's HBM budget the backward
-recomputes them batch by batch.
+recomputes them batch by batch.  A Solver holds the fields of its last
+forward only: a backward whose forward was followed by another one on the
+same Solver uploads its own slowness again, and the adjoint recomputes the
+fields that belong to it.
 
 torch is imported before the HIP library so that both use one HIP runtime
 (torch ships its own copy of libamdhip64.so.7).
@@ -43,6 +46,8 @@ class Solver:
         self.nshots, self.ntr = survey.nshots, survey.ntr
         self.device = torch.device("cuda", device)
         self.ctx = native.Context([model], device=device, rows_per_lane=rows_per_lane, hbm_budget=hbm_budget)
+        self._forwards = 0     # forwards run so far
+        self._current = 0      # the forward whose slowness the context holds
 
     def _bind_stream(self):
         self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
@@ -65,16 +70,24 @@ class _Traveltimes(torch.autograd.Function):
         S = slowness(vp).contiguous()
         solver.ctx.set_slowness(0, S.data_ptr())
         solver.ctx.forward()
+        solver._forwards += 1
+        solver._current = solver._forwards
         tmod = torch.empty((solver.nshots, solver.ntr), dtype=torch.float32, device=vp.device)
         solver.ctx.tmod(0, device_ptr=tmod.data_ptr())
         fctx.solver = solver
         fctx.S = S
+        fctx.token = solver._current
         return tmod
 
     @staticmethod
     def backward(fctx, grad_out):
         solver = fctx.solver
         solver._bind_stream()
+        if solver._current != fctx.token:
+            # another forward ran since: restore this one's slowness, so that
+            # the adjoint recomputes this forward's fields
+            solver.ctx.set_slowness(0, fctx.S.data_ptr())
+            solver._current = fctx.token
         g = grad_out.contiguous().float()
         solver.ctx.set_residuals(0, g.data_ptr())
         solver.ctx.adjoint()

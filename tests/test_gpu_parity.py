@@ -181,6 +181,28 @@ def test_single_shot_entry_points(ora):
         assert bits_equal(gs_g, gs_o) and bits_equal(g_g, g_o)
 
 
+def test_single_shot_adjoint_negative_zero_traveltime(ora):
+    """rz_eikonal_adj_shot with T = -0.0 at the source (an interior cell,
+    not a receiver): the reference treats -0 as +0 and updates the cell, so
+    lambda equals the +0 run's bit for bit."""
+    sv = _grad_case(ora, 48, 70, 1, seed=22)
+    N = _native()
+    print("[case] single-shot adjoint with -0.0 in T", flush=True)
+    sy, sx = int(sv.src_iy[0]) + 20, int(sv.src_ix[0])   # a source inside the grid
+    T, _ = ora.eikonal(sv.S, sx, sy, sv.dh)
+    assert T[sy, sx] == 0 and 0 < sy < sv.ny - 1 and 0 < sx < sv.nx - 1
+    Tn = T.copy()
+    Tn[sy, sx] = np.float32(-0.0)
+    tres = (np.arange(sv.ntr, dtype=np.float32) - sv.ntr / 2) * np.float32(1e-4)
+    with N.Context([model_dict(sv, max_iter=50)], rows_per_lane=1) as ctx:
+        ctx.set_slowness(0, sv.S)
+        lam_p, it_p = ctx.eikonal_adj_shot(0, T, tres)
+        lam_n, it_n = ctx.eikonal_adj_shot(0, Tn, tres)
+    lam_o, it_o = ora.eikonal_adj(T, sv.rec_ix, sv.rec_iy, tres, sv.dh)
+    assert it_p == it_n == it_o
+    assert bits_equal(lam_n, lam_p) and bits_equal(lam_p, lam_o)
+
+
 def test_adjoint_inf_guard(ora):
     """sweep_adj.c:68: lambda = 0 next to traveltimes >= INF (9999999)."""
     sv = _grad_case(ora, 40, 30, 1, seed=23)

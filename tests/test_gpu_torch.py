@@ -34,6 +34,32 @@ def test_forward_and_backward_equal_grad_obj(ora):
     solver.close()
 
 
+def test_backward_after_a_second_forward_on_one_solver(ora):
+    """Two forwards (vp1, then vp2) on one Solver before loss1.backward(): the
+    gradient is the one of vp1, bit for bit as with a single forward."""
+    from rajzel_amd import torch_fatt as TF
+    sv = _survey(ora, rec_row=(2, 39))
+    print("[case] torch two forwards, one backward", flush=True)
+    tobs = torch.tensor(sv.tobs, device="cuda:0")
+    vp1 = torch.tensor(sv.vp, device="cuda:0", requires_grad=True)
+    vp2 = torch.tensor(sv.vp * np.float32(1.05), device="cuda:0", requires_grad=True)
+    ref = TF.Solver(sv)
+    TF.misfit(vp1, ref, tobs).backward()
+    g_ref = vp1.grad.clone()
+    vp1.grad = None
+    solver = TF.Solver(sv)
+    loss1 = TF.misfit(vp1, solver, tobs)
+    loss2 = TF.misfit(vp2, solver, tobs)
+    loss1.backward()
+    assert bits_equal(vp1.grad.cpu().numpy(), g_ref.cpu().numpy())
+    loss2.backward()                       # and the later one still gets its own
+    vp2.grad, g2 = None, vp2.grad.clone()
+    TF.misfit(vp2, ref, tobs).backward()
+    assert bits_equal(vp2.grad.cpu().numpy(), g2.cpu().numpy())
+    ref.close()
+    solver.close()
+
+
 def test_backward_recomputes_under_hbm_budget(ora):
     from rajzel_amd import torch_fatt as TF
     sv = _survey(ora, nshots=5, rec_row=(2, 39))
