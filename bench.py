@@ -11,7 +11,9 @@ picks = times in the model with 20 Gaussian anomalies + N(0, 1 ms) noise
 Printed: one JSON line with the shot-gradients/s of the batched launch,
 the plain one-shot-per-launch form on the same GPU, the HBM roofline of the
 dominant kernel (phase times from HIP events on the work stream) and the
-reference CPU path timed on a bounded sample of the same evaluation.
+reference CPU path (its own grad_obj, compiled from its sources) on the
+host cores over a bounded sample of the same evaluation, and over the
+200 x 600 / 120-shot evaluation in full.
 """
 from __future__ import annotations
 
@@ -184,40 +186,74 @@ def measured_traffic(path, phase, workload):
     return float(k["hbm_bytes"])
 
 
-def cpu_baseline(sv, S, tobs, nsample, gpu_tmod, gpu_tres, gpu_itf, gpu_ita):
-    """Reference C path (oracle/_ref) on the first nsample shots, one thread."""
-    from oracle.bindings import Reference, Restatement
-    kind = "reference"
-    try:
-        ref = Reference()
-        ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
-        t0 = time.perf_counter()
-        with tempfile.TemporaryDirectory(dir="/tmp") as d:
-            out = ref.grad_obj(S, sv.src_x_m[:nsample], sv.src_z_m[:nsample], sv.rec_ix, sv.rec_iy,
-                               tobs[:nsample], workdir=d)
-        dt = time.perf_counter() - t0
-    except OSError as e:  # the reference build needs libmpi at load time
-        log(f"reference CPU build unavailable ({e}); timing the restatement instead")
-        kind = "port"
-        ora = Restatement()
-        t0 = time.perf_counter()
-        out = ora.grad_obj(S, sv.src_ix[:nsample], sv.src_iy[:nsample], sv.rec_ix, sv.rec_iy,
-                           tobs[:nsample], sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
-        dt = time.perf_counter() - t0
-    parity = {
-        "shots": nsample,
-        "tmod_bitwise": bool(np.array_equal(out["tmod"].view(np.uint32), gpu_tmod[:nsample].view(np.uint32))),
-        "tres_bitwise": bool(np.array_equal(out["tres"].view(np.uint32), gpu_tres[:nsample].view(np.uint32))),
-        "iterations_equal": bool(np.array_equal(out["it_f"], gpu_itf[:nsample])
-                                 and np.array_equal(out["it_a"], gpu_ita[:nsample])),
-        "ref_it_f": out["it_f"].tolist(), "ref_it_a": out["it_a"].tolist(),
-    }
+def _cpu_worker(sv, shots, noise_rows, barrier, queue):
+    """One process of the CPU baseline: the reference's forward in the true
+    model for its shots plus the workload's noise gives the picks (untimed);
+    then, after every process is ready, the reference's grad_obj of those
+    shots (pick files written and read as the reference does) is timed."""
+    from oracle.bindings import Reference
+    ref = Reference()
+    ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    St = W_slowness(sv.vp_true)
+    tm = np.stack([ref.eikonal(St, float(sv.src_x_m[s]), float(sv.src_z_m[s]), sv.rec_ix, sv.rec_iy)[1]
+                   for s in shots])
+    tobs = (tm + noise_rows).astype(np.float32)
+    barrier.wait()
+    t0 = time.perf_counter()
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        out = ref.grad_obj(sv.S, sv.src_x_m[shots], sv.src_z_m[shots], sv.rec_ix, sv.rec_iy, tobs, workdir=d)
+    queue.put((list(shots), time.perf_counter() - t0, out["tmod"], out["tres"], out["it_f"], out["it_a"]))
+
+
+def W_slowness(vp):
+    from rajzel_amd import workloads as W
+    return W.slowness(vp)
+
+
+def cpu_reference_rate(sv, shots, workers):
+    """The reference's grad_obj (oracle/_ref, compiled from its sources) over
+    `shots` of the workload, split into `workers` single-rank processes
+    (consecutive shot ranges, as init_MPIshot.c does for MPI ranks).
+    Returns (shot-gradients/s over the slowest process's wall time, the
+    per-shot results).  Must run before this process touches the GPU: the
+    workers are forked."""
+    import multiprocessing as mp
+    ctx = mp.get_context("fork")
+    rng = np.random.default_rng(sv.meta.get("seed_noise", sv.meta.get("seed", 0) + 1000))
+    noise = rng.normal(0.0, sv.noise_s, (sv.nshots, sv.ntr)) if sv.noise_s > 0 else np.zeros((sv.nshots, sv.ntr))
+    parts = [p for p in np.array_split(np.asarray(shots), workers) if len(p)]
+    barrier, queue = ctx.Barrier(len(parts)), ctx.Queue()
+    procs = [ctx.Process(target=_cpu_worker, args=(sv, p, noise[p], barrier, queue)) for p in parts]
+    for pr in procs:
+        pr.start()
+    res = [queue.get() for _ in procs]
+    for pr in procs:
+        pr.join()
+    wall = max(r[1] for r in res)
+    rows = {}
+    for sh, _, tm, tr, itf, ita in res:
+        for j, s in enumerate(sh):
+            rows[s] = (tm[j], tr[j], int(itf[j]), int(ita[j]))
+    return len(shots) / wall, wall, rows
+
+
+def cpu_baseline(sv_c4, c4_shots, sv_c2, cores):
+    """The reference C path on the host cores (BASELINE.md's plan): the C4
+    sample on all cores and on one, and the C2 evaluation in full on all
+    cores and on a 40-shot sample on one."""
+    c4_all, c4_wall, rows = cpu_reference_rate(sv_c4, c4_shots, cores)
+    c4_one, c4_one_wall, _ = cpu_reference_rate(sv_c4, c4_shots[:2], 1)
+    c2_all, c2_wall, _ = cpu_reference_rate(sv_c2, list(range(sv_c2.nshots)), cores)
+    c2_one, c2_one_wall, _ = cpu_reference_rate(sv_c2, list(range(40)), 1)
     return {
-        "value": nsample / dt, "unit": "shot-gradients/s", "cores": 1, "kind": kind,
-        "sample": f"first {nsample} of {sv.nshots} shots of the same evaluation "
-                  f"({sv.ny}x{sv.nx}, {sv.ntr} receivers), forward+adjoint+gradient, 1 thread, "
-                  f"{dt:.1f} s",
-    }, parity
+        "value": c4_all, "unit": "shot-gradients/s", "cores": cores, "kind": "reference",
+        "sample": f"{len(c4_shots)} of the {sv_c4.nshots} shots of the same {sv_c4.ny}x{sv_c4.nx} evaluation "
+                  f"({sv_c4.ntr} receivers), forward+adjoint+gradient, reference grad_obj in {cores} single-rank "
+                  f"processes over consecutive shot ranges, {c4_wall:.1f} s",
+        "one_core": {"value": c4_one, "shots": 2, "seconds": c4_one_wall},
+        "c2_full": {"value": c2_all, "cores": cores, "shots": sv_c2.nshots, "seconds": c2_wall,
+                    "one_core_value": c2_one, "one_core_shots": 40, "one_core_seconds": c2_one_wall},
+    }, rows
 
 
 def main():
@@ -228,7 +264,10 @@ def main():
     ap.add_argument("--shots", type=int, default=200)
     ap.add_argument("--plain-shots", type=int, default=10,
                     help="shots timed in the one-shot-per-launch form (rate is per shot)")
-    ap.add_argument("--cpu-shots", type=int, default=3, help="shots of the reference CPU sample")
+    ap.add_argument("--cpu-shots", type=int, default=0,
+                    help="shots of the reference CPU sample (default: two per host core)")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="processes of the reference CPU baseline (default: the host cores, at most 16)")
     ap.add_argument("--rows-per-lane", type=int, default=-1,
                     help="rows of a band per lane (1, 2, 4, 8); 0 = by band count, -1 = autotuned")
     ap.add_argument("--no-cpu", action="store_true")
@@ -260,6 +299,18 @@ def main():
     S_true = W.slowness(sv.vp_true)
     log(f"[bench] workload {sv.name}: {sv.nshots} shots, {sv.ntr} receivers "
         f"({time.perf_counter() - t_setup:.1f} s to generate)")
+    cpu = None
+    if not args.no_cpu and not args.concurrency_scan:
+        # the reference on the host cores, before this process touches the GPU
+        cores = args.cpu_cores or min(len(os.sched_getaffinity(0)), 16)
+        nsample = min(args.cpu_shots or 2 * cores, sv.nshots)
+        picks_ = np.linspace(0, sv.nshots - 1, nsample).round().astype(int).tolist()
+        try:
+            cpu = cpu_baseline(sv, picks_, W.c2_gradient(), cores)
+            log(f"[bench] cpu baseline: {cpu[0]['value']:.3f} shot-gradients/s on {cores} cores "
+                f"(one core {cpu[0]['one_core']['value']:.3f}); C2 {cpu[0]['c2_full']['value']:.1f}/s")
+        except OSError as e:  # the reference build needs libmpi at load time
+            log(f"reference CPU build unavailable ({e})")
     if args.concurrency_scan:
         print(json.dumps({"workload": f"{sv.ny}x{sv.nx} grid, {sv.ntr} receivers, picks 0.5 s",
                           "concurrency": concurrency_scan(sv, args.rows_per_lane)}), flush=True)
@@ -347,12 +398,20 @@ def main():
         out["speedup_vs_plain"] = value / plain
         log(f"[bench] plain one-shot-per-launch: {plain:.1f} shot-gradients/s")
 
-    if not args.no_cpu:
+    if cpu is not None:
+        cb, rows = cpu
         gpu_tmod, gpu_tres = ctx.tmod(0), ctx.tres(0)
-        cb, parity = cpu_baseline(sv, S, sv.tobs, min(args.cpu_shots, sv.nshots), gpu_tmod, gpu_tres, itf, ita)
+        shots = sorted(rows)
         out["cpu_baseline"] = cb
-        out["parity_sample"] = parity
-        log(f"[bench] cpu baseline: {cb['value']:.3f} shot-gradients/s; parity {parity}")
+        out["parity_sample"] = {
+            "shots": len(shots),
+            "tmod_bitwise": all(np.array_equal(rows[s][0].view(np.uint32), gpu_tmod[s].view(np.uint32))
+                                for s in shots),
+            "tres_bitwise": all(np.array_equal(rows[s][1].view(np.uint32), gpu_tres[s].view(np.uint32))
+                                for s in shots),
+            "iterations_equal": all(rows[s][2] == itf[s] and rows[s][3] == ita[s] for s in shots),
+        }
+        log(f"[bench] parity of the CPU sample: {out['parity_sample']}")
     ctx.close()
     if not args.no_precond:
         out["precond_gauss_filt"] = gauss_filter_timing(sv.nx, sv.ny)

@@ -97,6 +97,10 @@ class Restatement:
         self._fwd.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, fr,
                               C.c_int, _i32, _i32, C.c_int, _i32, _i32, fr, fr, o_i]
         self._fwd.restype = None
+        self._taper = fn("rzo_set_shot_taper")
+        self._taper.argtypes = [C.c_int, C.c_float, C.c_int, C.c_int, _f32, _f32, C.c_int]
+        self._taper.restype = None
+        self._taper_keep = None
         self._gauss = fn("rzo_gauss_filt")
         self._gauss.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), fr]
         self._gauss.restype = C.c_int
@@ -166,6 +170,16 @@ class Restatement:
         ny, nx = A.shape
         self._frame(nx, ny, A)
         return A
+
+    def set_shot_taper(self, mode, radius=80.0, filtsize=1, src_x_m=(0.0,), src_z_m=(0.0,), first_shot=0):
+        """Per-shot gradient taper of grad_obj (ass_grad.c:31-37): mode =
+        SWS_TAPER_CIRCULAR_PER_SHOT, radius = SRTRADIUS (m), filtsize =
+        FILTSIZE; src_x_m/src_z_m the sources of every shot of the survey
+        (metres), of which grad_obj's shots start at first_shot."""
+        sx = np.ascontiguousarray(src_x_m, np.float32)
+        sz = np.ascontiguousarray(src_z_m, np.float32)
+        self._taper_keep = (sx, sz)
+        self._taper(int(mode), float(radius), int(filtsize), sx.size, sx, sz, int(first_shot))
 
     def gauss_filt(self, A, fsx, fsy):
         """gauss_filt.c on a copy of A; returns (filtered, fsx, fsy as made odd)."""
@@ -242,10 +256,17 @@ class Reference:
         L.rzref_grad_obj.restype = C.c_double
         L.rzref_gauss_filt.argtypes = [_f32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.rzref_gauss_filt.restype = None
+        L.rzref_set_taper.argtypes = [C.c_int, C.c_float, C.c_int]
+        L.rzref_set_taper.restype = None
 
     def setup(self, nx, ny, dh, time=6.0, ttnorm=1e-10, eps_adj=1e-9):
         self.nx, self.ny = nx, ny
         self.lib.rzref_setup(nx, ny, dh, time, ttnorm, eps_adj)
+
+    def set_taper(self, sws, srtradius=80.0, filtsize=1):
+        """SWS_TAPER_CIRCULAR_PER_SHOT, SRTRADIUS (m), FILTSIZE for grad_obj
+        (reset to 0 by setup)."""
+        self.lib.rzref_set_taper(sws, srtradius, filtsize)
 
     @staticmethod
     def _f(x):

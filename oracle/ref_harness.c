@@ -38,6 +38,15 @@ void rzref_setup(int nx, int ny, float dh, float time, float ttnorm, float eps_a
     SWS_TAPER_CIRCULAR_PER_SHOT = 0;
 }
 
+/* Per-shot gradient taper of ass_grad.c:31-37: the reference's globals
+ * SWS_TAPER_CIRCULAR_PER_SHOT, SRTRADIUS and FILTSIZE (read_par.c:183-192). */
+void rzref_set_taper(int sws, float srtradius, int filtsize)
+{
+    SWS_TAPER_CIRCULAR_PER_SHOT = sws;
+    SRTRADIUS = srtradius;
+    FILTSIZE = filtsize;
+}
+
 static float **mat_from(const float *src)
 {
     float **m = matrix(1, NY, 1, NX);

@@ -293,6 +293,81 @@ void NAME(rzo_ass_grad)(int nx, int ny, const real *S, const real *lam, real *gs
 }
 
 /* ---------------------------------------------------------------------------
+ * Per-shot gradient tapers of ass_grad.c:31-37 (SWS_TAPER_CIRCULAR_PER_SHOT),
+ * applied to grad_shot after the frame copy.  Settings (rzo_set_shot_taper):
+ *   mode 1  taper_grad_shot.c:63-145: grad_shot *= taper at every cell within
+ *           `radius` (SRTRADIUS, m) of the shot's source, then of each
+ *           receiver in turn; taper = f0 + (1-f0)*(0.5 - 0.5 cos(pi r/R))^a
+ *           with f0 = 1e-3, a = -(log(0.1f - f0) - log(1 - f0))/log 2, cell
+ *           (ix, iy) at (ix*DH, iy*DH) (1-based), the source at its position
+ *           in metres, receiver n at recpos*DH.  x, y, r, a and the taper are
+ *           floats; the square root, cosine and power are taken in double.
+ *   mode 2  taper_grad_shot1.c:159-216: grad_shot = 1e-5 on the
+ *           (2*FILTSIZE+1)^2 cells around every source of the survey (all
+ *           shots, iround(pos/DH)) and every receiver.  The reference does no
+ *           bounds check: a column index off the grid lands, in the
+ *           reference's contiguous matrix, on the adjacent row's cell of the
+ *           same linear address, which is reproduced; a row off the grid
+ *           reaches outside the matrix there (undefined) and is skipped.
+ * ------------------------------------------------------------------------- */
+static struct {
+    int mode, filtsize, nsrc, first;
+    float radius;
+    const float *sx, *sz;   /* all sources of the survey, metres */
+} g_taper;
+
+void NAME(rzo_set_shot_taper)(int mode, float radius, int filtsize, int nsrc_all, const float *src_x_m,
+                              const float *src_z_m, int first_shot)
+{
+    g_taper.mode = mode; g_taper.radius = radius; g_taper.filtsize = filtsize;
+    g_taper.nsrc = nsrc_all; g_taper.sx = src_x_m; g_taper.sz = src_z_m; g_taper.first = first_shot;
+}
+
+/* taper_grad_shot.c:100-110 for one centre (cx, cy) in metres */
+static void NAME(taper_disc)(int nx, int ny, float dh, float R, float a, float cx, float cy, real *g)
+{
+    const float f0 = 1e-3f;
+    for (int iy = 1; iy <= ny; ++iy)
+        for (int ix = 1; ix <= nx; ++ix) {
+            const float x = ix * dh, y = iy * dh;
+            const float r = (float)sqrt(pow((double)(x - cx), 2.0) + pow((double)(y - cy), 2.0));
+            float taper = 1.0f;
+            if (r <= R) taper = (float)(f0 + (1.0 - f0) * pow(0.5 - 0.5 * cos(3.141592653589793 * r / R), a));
+            g[IDX(iy - 1, ix - 1)] *= (real)taper;
+        }
+}
+
+static void NAME(taper_shot)(int nx, int ny, float dh, int shot, int ntr, const int *rx, const int *ry, real *g)
+{
+    if (g_taper.mode == 1) {
+        const float frh = 0.1f, f0 = 1e-3f, R = g_taper.radius;
+        const float a = (float)(-(log(frh - f0) - log(1.0 - f0)) / log(2.0));
+        const int n = g_taper.first + shot;
+        NAME(taper_disc)(nx, ny, dh, R, a, g_taper.sx[n], g_taper.sz[n], g);
+        for (int i = 0; i < ntr; ++i) NAME(taper_disc)(nx, ny, dh, R, a, (rx[i] + 1) * dh, (ry[i] + 1) * dh, g);
+    } else if (g_taper.mode == 2) {
+        const int F = g_taper.filtsize;
+        const size_t n = (size_t)nx * (size_t)ny;
+        for (int c = 0; c < g_taper.nsrc + ntr; ++c) {
+            int i1, j1;   /* 1-based centre */
+            if (c < g_taper.nsrc) {
+                i1 = (int)floor(g_taper.sx[c] / dh + 0.5);
+                j1 = (int)floor(g_taper.sz[c] / dh + 0.5);
+            } else {
+                i1 = rx[c - g_taper.nsrc] + 1;
+                j1 = ry[c - g_taper.nsrc] + 1;
+            }
+            for (int i = i1 - F; i <= i1 + F; ++i)
+                for (int j = j1 - F; j <= j1 + F; ++j) {
+                    if (j < 1 || j > ny) continue;
+                    const long long L = (long long)(j - 1) * nx + (i - 1);
+                    if (L >= 0 && (size_t)L < n) g[L] = (real)1e-5f;
+                }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
  * One objective + gradient evaluation over shots 0..nshots-1 on one rank
  * (grad_obj.c:10-79 without precond, which is the identity for
  * SPATFILTER=0 and no tapers).
@@ -324,7 +399,19 @@ double NAME(rzo_grad_obj)(int nx, int ny, float dh, float time, float ttnorm, fl
         const double l2 = NAME(rzo_calc_fa_res)(nx, ny, dh, T, ntr, rx, ry, tmod, tobs + (size_t)s * ntr, tres);
         l2sum += l2;
         const int ita = NAME(rzo_eikonal_adj)(nx, ny, dh, time, ttnorm, eps_adj, maxit, T, ntr, rx, ry, tres, flag, lam);
-        NAME(rzo_ass_grad)(nx, ny, S, lam, gshot, grad);
+        if (g_taper.mode == 0) {
+            NAME(rzo_ass_grad)(nx, ny, S, lam, gshot, grad);
+        } else {   /* ass_grad.c:20-46 with the per-shot taper between frame copy and sum */
+            memset(gshot, 0, n * sizeof(real));
+            for (int h = 1; h < nx - 1; ++h)
+                for (int k = 1; k < ny - 1; ++k) {
+                    const double sv = (double)S[IDX(k, h)];
+                    gshot[IDX(k, h)] = (real)((double)lam[IDX(k, h)] * (sv * sv * sv));
+                }
+            NAME(rzo_cp_grad_frame)(nx, ny, gshot);
+            NAME(taper_shot)(nx, ny, dh, s, ntr, rx, ry, gshot);
+            for (size_t i = 0; i < n; ++i) grad[i] += gshot[i];
+        }
         if (tmod_all) memcpy(tmod_all + (size_t)s * ntr, tmod, (size_t)ntr * sizeof(real));
         if (tres_all) memcpy(tres_all + (size_t)s * ntr, tres, (size_t)ntr * sizeof(real));
         if (l2_shot) l2_shot[s] = l2;

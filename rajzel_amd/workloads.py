@@ -282,6 +282,28 @@ def edge_case(nx, ny, dh=10.0, nshots=3, seed=61, sigma=3.0, ttnorm=1e-4, rec_st
     )
 
 
+def taper_case(nx=60, ny=34, dh=10.0, nshots=4, seed=81):
+    """A small survey for the per-shot gradient tapers (ass_grad.c:31-37):
+    sources at depth 2*DH between the receivers, receivers on row 2 at both
+    edge columns (1 and NX: a taper_grad_shot1.c window there runs off the
+    grid sideways) and every 3 columns between, and a line at depth NY/2.
+    Every (2*FILTSIZE+1)-row window of FILTSIZE <= 1 stays on the grid
+    (the reference's behaviour for rows off the grid is undefined)."""
+    rng = np.random.default_rng(seed)
+    X, Z = _grid_coords(nx, ny, dh)
+    vtrue = 1500.0 + 0.5 * Z + _gaussians(rng, X, Z, 3, -300.0, 300.0, 3 * dh, 8 * dh, dh, nx, ny)
+    rc = np.concatenate([[1], np.arange(4, nx, 3), [nx], np.arange(3, nx, 7)]).astype(np.float64)
+    rr = np.concatenate([np.full(len(rc) - len(np.arange(3, nx, 7)), 2), np.full(len(np.arange(3, nx, 7)), ny // 2)])
+    sx = rng.uniform(3 * dh, (nx - 3) * dh, nshots)
+    return Survey(
+        name=f"taper_{ny}x{nx}", nx=nx, ny=ny, dh=dh,
+        vp=gradient_model(nx, ny, dh, 1500.0, 0.5), vp_true=vtrue.astype(np.float32),
+        src_x_m=sx.astype(np.float32), src_z_m=np.full(nshots, 2 * dh, np.float32),
+        rec_x_m=(rc * dh).astype(np.float32), rec_z_m=(rr * dh).astype(np.float32),
+        noise_s=0.002, meta={"seed": seed},
+    )
+
+
 def c5_gradient(nshots=400, ntr=3000, seed=5, nx=6000, ny=2000, dh=2.5):
     """2000 x 6000, DH = 2.5 m.  Truth = 1-D gradient 1500 + 0.6 z plus 40
     seeded Gaussians (U(-400, 400) m/s, radius U(50, 300) m); the current

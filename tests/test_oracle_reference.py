@@ -119,3 +119,30 @@ def test_gauss_filt(ora, ref, shape, sizes):
     o, ox, oy = ora.gauss_filt(A, *sizes)
     assert (ox, oy) == (rx, ry)
     assert bits_equal(o, r), f"{ndiff(o, r)} cells differ"
+
+
+@pytest.mark.parametrize("sws,radius,filtsize", [(1, 35.0, 1), (1, 80.0, 1), (2, 80.0, 1), (2, 80.0, 0)])
+def test_grad_obj_with_per_shot_taper(ora, ref, sws, radius, filtsize):
+    """ass_grad.c:31-37: taper_grad_shot (radial taper at the shot's source
+    and every receiver) and taper_grad_shot1 (windows at every source and
+    receiver set to 1e-5), with windows running off the grid sideways."""
+    sv = W.taper_case()
+    ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    St = W.slowness(sv.vp_true)
+    tm = np.stack([ref.eikonal(St, float(sv.src_x_m[s]), float(sv.src_z_m[s]), sv.rec_ix, sv.rec_iy)[1]
+                   for s in range(sv.nshots)])
+    tobs = (tm + np.random.default_rng(6).normal(0, 0.002, tm.shape)).astype(np.float32)
+    ref.set_taper(sws, radius, filtsize)
+    b = ref.grad_obj(sv.S, sv.src_x_m, sv.src_z_m, sv.rec_ix, sv.rec_iy, tobs, workdir="/tmp")
+    ref.set_taper(0)
+    ora.set_shot_taper(sws, radius, filtsize, sv.src_x_m, sv.src_z_m)
+    try:
+        a = ora.grad_obj(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, b["tobs"], sv.dh, sv.time, sv.ttnorm,
+                         sv.eps_adj)
+    finally:
+        ora.set_shot_taper(0)
+    assert bits_equal(a["grad"], b["grad"]), f"grad: {ndiff(a['grad'], b['grad'])} values differ"
+    assert a["L2"] == b["L2"]
+    plain = ora.grad_obj(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, b["tobs"], sv.dh, sv.time, sv.ttnorm,
+                         sv.eps_adj)
+    assert ndiff(plain["grad"], a["grad"]) > 0                 # the taper does something
