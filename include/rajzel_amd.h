@@ -180,11 +180,42 @@ int rz_get_field(rz_ctx *ctx, int model, int shot, int which, float *out, int on
  *   rz_eikonal_shot      eikonal.c:10-76 for a source cell: T, receiver
  *                        times, outer iterations.
  *   rz_eikonal_adj_shot  eikonal_adj.c:10-59 for a given T and Tres.
- *   rz_ass_grad_shot     ass_grad.c:11-49 (no per-shot taper):
+ *   rz_ass_grad_shot     ass_grad.c:11-49 without the per-shot taper
+ *                        (rz_ass_grad_shot_at below applies it):
  *                        grad_shot = frame-copied lambda*S^3, grad += it. */
 int rz_eikonal_shot(rz_ctx *ctx, int model, int src_ix, int src_iy, float *T, float *tmod, int *iters);
 int rz_eikonal_adj_shot(rz_ctx *ctx, int model, const float *T, const float *tres, float *lam, int *iters);
 int rz_ass_grad_shot(rz_ctx *ctx, int model, const float *lam, float *grad_shot, float *grad);
+
+/* Per-shot gradient taper of ass_grad.c:31-37 (SWS_TAPER_CIRCULAR_PER_SHOT)
+ * for `model`: each shot's frame-copied lambda*S^3 is tapered before it joins
+ * the shot sum, and rz_gradient / rz_adjoint then copy the frame of the sum
+ * again (grad_obj.c:70), bitwise as the reference.
+ *   mode 0   none (the default)
+ *   mode 1   taper_grad_shot.c:43-84, radius_m = SRTRADIUS (> 0): the
+ *            gradient is multiplied by f0 + (1-f0)(0.5 - 0.5 cos(PI r/R))^a
+ *            (f0 = 1e-3, a from frh = 0.1) within r <= R of the shot's source
+ *            and then of each receiver in turn.  Shot s of the model uses
+ *            source first_shot + s, which must be < nsrc.
+ *   mode 2   taper_grad_shot1.c:32-62, filtsize = FILTSIZE: cells of the
+ *            (2F+1) x (2F+1) windows around all nsrc sources (iround(pos/DH))
+ *            and all receivers take 1e-5, in every shot.  Columns outside the
+ *            grid wrap onto the neighbouring row, as in the reference's
+ *            contiguous matrix; rows outside the grid, and cells before the
+ *            first or past the last, are left alone (the reference writes
+ *            outside its matrix there).
+ * src_x_m, src_z_m are the survey's nsrc source positions in metres
+ * (srcpos[1][1..nsrc], srcpos[2][1..nsrc], sources.c:68-71).  The factors are
+ * computed here, once per geometry, on the host with the C library's pow
+ * and cos (as the reference does), and kept on the device: at most 2^28 of
+ * them (RZ_ERR_INVALID beyond, e.g. for a radius that covers most of a large
+ * grid); a call with the same settings keeps them. */
+int rz_set_shot_taper(rz_ctx *ctx, int model, int mode, float radius_m, int filtsize, int nsrc, const float *src_x_m,
+                      const float *src_z_m, int first_shot);
+/* rz_ass_grad_shot with the model's per-shot taper for source `src` (the
+ * reference's ishot - 1, an index into rz_set_shot_taper's sources):
+ * ass_grad.c:11-49 in full. */
+int rz_ass_grad_shot_at(rz_ctx *ctx, int model, int src, const float *lam, float *grad_shot, float *grad);
 
 /* Phase timing: with profiling on, rz_forward / rz_gradient record HIP
  * events on the work stream around each phase; rz_get_profile returns the

@@ -258,6 +258,8 @@ class Reference:
         L.rzref_gauss_filt.restype = None
         L.rzref_set_taper.argtypes = [C.c_int, C.c_float, C.c_int]
         L.rzref_set_taper.restype = None
+        L.rzref_ass_grad_at.argtypes = [_f32, _f32, C.c_int, _f32, _f32, C.c_int, _i32, _i32, C.c_int, _f32, _f32]
+        L.rzref_ass_grad_at.restype = None
 
     def setup(self, nx, ny, dh, time=6.0, ttnorm=1e-10, eps_adj=1e-9):
         self.nx, self.ny = nx, ny
@@ -308,6 +310,17 @@ class Reference:
         g = self._f(grad).copy()
         gs = np.empty((self.ny, self.nx), np.float32)
         self.lib.rzref_ass_grad(self._f(S), self._f(lam), gs, g)
+        return gs, g
+
+    def ass_grad_at(self, S, lam, grad, src_x_m, src_z_m, rec_ix, rec_iy, ishot):
+        """The reference's ass_grad() for shot ishot (0-based) of the given
+        sources (metres) and receivers (0-based cells), with the per-shot
+        taper of set_taper."""
+        g = self._f(grad).copy()
+        gs = np.empty((self.ny, self.nx), np.float32)
+        sx, sz = self._f(src_x_m), self._f(src_z_m)
+        self.lib.rzref_ass_grad_at(self._f(S), self._f(lam), len(sx), sx, sz, len(rec_ix), self._i1(rec_ix),
+                                   self._i1(rec_iy), int(ishot) + 1, gs, g)
         return gs, g
 
     def gauss_filt(self, A, fsx, fsy):

@@ -147,6 +147,23 @@ void rzref_ass_grad(const float *S, const float *lam, float *grad_shot_out, floa
     free_matrix(gs, 1, NY, 1, NX); free_matrix(srcpos, 1, 8, 1, 1); free_imatrix(rec, 1, 3, 1, 1);
 }
 
+/* ass_grad() for shot ishot (1-based) of nshots sources (metres) and ntr
+ * receivers (1-based cells): the per-shot taper set by rzref_set_taper
+ * applies. */
+void rzref_ass_grad_at(const float *S, const float *lam, int nshots, const float *srcx, const float *srcy, int ntr,
+                       const int *rx1, const int *ry1, int ishot, float *grad_shot_out, float *grad_io)
+{
+    float **s = mat_from(S), **l = mat_from(lam), **g = mat_from(grad_io), **gs = mat_from(NULL);
+    float **srcpos = matrix(1, 8, 1, nshots);
+    for (int i = 1; i <= nshots; ++i) { srcpos[1][i] = srcx[i - 1]; srcpos[2][i] = srcy[i - 1]; }
+    int **rec = recpos_from(ntr, rx1, ry1);
+    ass_grad(g, gs, s, l, srcpos, nshots, rec, ntr, ishot);
+    mat_to(g, grad_io); mat_to(gs, grad_shot_out);
+    free_matrix(s, 1, NY, 1, NX); free_matrix(l, 1, NY, 1, NX); free_matrix(g, 1, NY, 1, NX);
+    free_matrix(gs, 1, NY, 1, NX); free_matrix(srcpos, 1, 8, 1, nshots);
+    free_imatrix(rec, 1, 3, 1, ntr > 0 ? ntr : 1);
+}
+
 static void write_picks_file(const char *stem, int shot1, int ntr, const float *v)
 {
     char fn[STRING_SIZE2];

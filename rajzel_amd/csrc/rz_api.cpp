@@ -41,6 +41,15 @@ struct ModelHost {
     // device
     float *S = nullptr, *SA = nullptr, *SB = nullptr;
     int *rec_k_d = nullptr, *rec_c_d = nullptr, *uq_k_d = nullptr, *uq_c_d = nullptr, *uq_r_d = nullptr;
+    // per-shot gradient taper (rz_set_shot_taper): settings, and the device
+    // tables built from them
+    int tap_mode = 0, tap_filt = 0, tap_first = 0, tap_nsrc = 0;
+    float tap_radius = 0.f;
+    std::vector<float> tap_sx, tap_sz;
+    int *tap_box = nullptr;
+    long long *tap_off = nullptr;
+    float *tap_fsrc = nullptr, *tap_frec = nullptr;
+    unsigned *tap_start = nullptr;
 };
 
 }  // namespace
@@ -111,6 +120,7 @@ struct rz_ctx {
     bool have_grad = false, have_fwd = false;
     size_t bytes = 0;
     std::vector<void *> allocs;
+    std::vector<size_t> alloc_bytes;
     std::string err;
     // phase timing with events on the work stream (rz_set_profiling)
     bool prof = false;
@@ -153,10 +163,28 @@ int dalloc(rz_ctx *c, T **p, size_t n)
         return fail(c, RZ_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
     }
     c->allocs.push_back(q);
+    c->alloc_bytes.push_back(n * sizeof(T));
     c->bytes += n * sizeof(T);
     *p = static_cast<T *>(q);
     return RZ_OK;
 }
+
+// Free one allocation of the context (the stream must be idle).
+void dfree(rz_ctx *c, void *p)
+{
+    if (!p) return;
+    for (size_t i = 0; i < c->allocs.size(); ++i)
+        if (c->allocs[i] == p) {
+            c->allocs.erase(c->allocs.begin() + i);
+            c->bytes -= c->alloc_bytes[i];
+            c->alloc_bytes.erase(c->alloc_bytes.begin() + i);
+            (void)hipFree(p);
+            return;
+        }
+}
+
+// Factors the per-shot taper tables may hold (1 GiB of floats).
+constexpr size_t kTaperMaxEntries = (size_t)1 << 28;
 
 #define TRY(expr)                 \
     do {                          \
@@ -314,6 +342,17 @@ int choose_R(int ny_max)
     for (int R : {1, 2, 4, 8})
         if (bands_for(ny_max, R) <= 4) return R;
     return 8;
+}
+
+// The taper of model h for a launch whose first shot uses source src0.
+TaperDev taper_of(const ModelHost &h, int src0)
+{
+    TaperDev t{};
+    t.mode = h.tap_mode;
+    t.src0 = src0;
+    t.box = h.tap_box; t.off = h.tap_off; t.fsrc = h.tap_fsrc;
+    t.start = h.tap_start; t.frec = h.tap_frec;
+    return t;
 }
 
 ModelDev make_dev(const rz_ctx *c, const ModelHost &h)
@@ -587,8 +626,12 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
         ModelDev mh = make_dev(c, h);
         HIPCHK(c, launch_gradient(&mh, (int)mi, c->md_d, c->slot_of_d + h.shot_base + s0,
                                   c->lsel_d + h.shot_base + s0, s1 - s0, sl,
-                                  c->grad_d + h.grad_off, s0 > 0 ? 1 : 0, c->stream));
+                                  c->grad_d + h.grad_off, s0 > 0 ? 1 : 0, taper_of(h, h.tap_first + s0),
+                                  c->stream));
     }
+    if (b1 == c->total_shots)   // grad_obj.c:70: with a per-shot taper the sum's frame is copied again
+        for (const ModelHost &h : c->m)
+            if (h.tap_mode != 0) HIPCHK(c, launch_grad_frame(h.d.nx, h.d.ny, c->grad_d + h.grad_off, c->stream));
     TRY(mark(c, &m.e[4]));
     return RZ_OK;
 }
@@ -1271,10 +1314,12 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     return RZ_OK;
 }
 
-int rz_ass_grad_shot(rz_ctx *c, int model, const float *lam, float *grad_shot, float *grad)
+static int ass_grad_shot(rz_ctx *c, int model, int src, const float *lam, float *grad_shot, float *grad)
 {
     TRY(check_model(c, model));
     ModelHost &h = c->m[model];
+    if (src >= 0 && h.tap_mode == 1 && src >= h.tap_nsrc)
+        return fail(c, RZ_ERR_INVALID, "source %d out of range: the taper holds %d sources", src, h.tap_nsrc);
     if (!h.have_S) return fail(c, RZ_ERR_STATE, "slowness not set");
     if (!lam) return fail(c, RZ_ERR_INVALID, "null lambda");
     const size_t n = (size_t)h.d.nx * h.d.ny;
@@ -1283,7 +1328,7 @@ int rz_ass_grad_shot(rz_ctx *c, int model, const float *lam, float *grad_shot, f
     HIPCHK(c, launch_rowmajor_to_field(&mh, c->s_row, c->LA, 0, c->stream));
     // slot 0, lambda in LA: s_slot0 holds the 0 for both
     HIPCHK(c, launch_gradient(&mh, model, c->md_d, c->s_slot0, c->s_slot0, 1, slots_of(c), c->s_row2, 0,
-                              c->stream));
+                              src >= 0 ? taper_of(h, src) : TaperDev{}, c->stream));
     std::vector<float> gs(n);
     HIPCHK(c, hipMemcpyAsync(gs.data(), c->s_row2, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1292,6 +1337,78 @@ int rz_ass_grad_shot(rz_ctx *c, int model, const float *lam, float *grad_shot, f
     if (grad_shot) std::memcpy(grad_shot, gs.data(), n * sizeof(float));
     if (grad)
         for (size_t i = 0; i < n; ++i) grad[i] += gs[i];   // ass_grad.c:40-46
+    return RZ_OK;
+}
+
+int rz_ass_grad_shot(rz_ctx *c, int model, const float *lam, float *grad_shot, float *grad)
+{
+    return ass_grad_shot(c, model, -1, lam, grad_shot, grad);
+}
+
+int rz_ass_grad_shot_at(rz_ctx *c, int model, int src, const float *lam, float *grad_shot, float *grad)
+{
+    if (src < 0) return fail(c, RZ_ERR_INVALID, "negative source index %d", src);
+    return ass_grad_shot(c, model, src, lam, grad_shot, grad);
+}
+
+int rz_set_shot_taper(rz_ctx *c, int model, int mode, float radius_m, int filtsize, int nsrc, const float *src_x_m,
+                      const float *src_z_m, int first_shot)
+{
+    TRY(check_model(c, model));
+    ModelHost &h = c->m[model];
+    if (mode < 0 || mode > 2) return fail(c, RZ_ERR_INVALID, "taper mode %d (0, 1 or 2)", mode);
+    if (nsrc < 0 || (nsrc > 0 && (!src_x_m || !src_z_m)))
+        return fail(c, RZ_ERR_INVALID, "source positions: %d sources, null pointer", nsrc);
+    if (mode == 1) {
+        if (!(radius_m > 0.f) || !std::isfinite(radius_m))
+            return fail(c, RZ_ERR_INVALID, "taper radius %g m: must be finite and > 0", (double)radius_m);
+        if (first_shot < 0 || (long long)first_shot + h.d.nshots > nsrc)
+            return fail(c, RZ_ERR_INVALID, "shots %d..%d of model %d are not in the %d sources", first_shot,
+                        first_shot + h.d.nshots - 1, model, nsrc);
+    }
+    if (mode == 0) nsrc = 0;
+    const std::vector<float> sx(src_x_m, src_x_m + nsrc), sz(src_z_m, src_z_m + nsrc);
+    if (mode == h.tap_mode && (mode == 0 || (sx == h.tap_sx && sz == h.tap_sz && first_shot == h.tap_first &&
+                                             (mode == 1 ? radius_m == h.tap_radius : filtsize == h.tap_filt))))
+        return RZ_OK;   // unchanged geometry: keep the tables
+    TaperTables t;
+    if (mode != 0) {
+        const std::string why = build_taper(mode, h.d.nx, h.d.ny, h.d.dh, radius_m, filtsize, nsrc, sx.data(),
+                                            sz.data(), h.d.ntr, h.rec_c.data(), h.rec_k.data(), kTaperMaxEntries, t);
+        if (!why.empty()) return fail(c, RZ_ERR_INVALID, "per-shot taper: %s", why.c_str());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (void *p : {(void *)h.tap_box, (void *)h.tap_off, (void *)h.tap_fsrc, (void *)h.tap_frec,
+                    (void *)h.tap_start})
+        dfree(c, p);
+    h.tap_box = nullptr; h.tap_off = nullptr; h.tap_fsrc = nullptr; h.tap_frec = nullptr; h.tap_start = nullptr;
+    h.tap_mode = 0;
+    drop_graphs(c);   // a recorded gradient has no final frame copy, or one too many
+    if (mode != 0) {
+        TRY(dalloc(c, &h.tap_start, t.start.size()));
+        HIPCHK(c, hipMemcpy(h.tap_start, t.start.data(), t.start.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+        if (mode == 1) {
+            TRY(dalloc(c, &h.tap_box, t.box.size()));
+            TRY(dalloc(c, &h.tap_off, t.off.size()));
+            TRY(dalloc(c, &h.tap_fsrc, t.fsrc.size()));
+            TRY(dalloc(c, &h.tap_frec, t.frec.size()));
+            auto up = [&](void *dst, const void *src, size_t bytes) {
+                return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+            };
+            HIPCHK(c, up(h.tap_box, t.box.data(), t.box.size() * sizeof(int)));
+            HIPCHK(c, up(h.tap_off, t.off.data(), t.off.size() * sizeof(long long)));
+            HIPCHK(c, up(h.tap_fsrc, t.fsrc.data(), t.fsrc.size() * sizeof(float)));
+            HIPCHK(c, up(h.tap_frec, t.frec.data(), t.frec.size() * sizeof(float)));
+        }
+    }
+    h.tap_mode = mode;
+    h.tap_radius = radius_m;
+    h.tap_filt = filtsize;
+    h.tap_first = first_shot;
+    h.tap_nsrc = nsrc;
+    h.tap_sx = sx;
+    h.tap_sz = sz;
+    c->have_grad = false;
     return RZ_OK;
 }
 

@@ -50,7 +50,8 @@
 /* globals of the reference (globvar.h:12-29,38,69,96) */
 extern float DH, TIME, TTNORM, EPS_ADJ;
 extern int NX, NY, SEISMO, SNAP, MYID, INFO, NSHOT1, NSHOT2, SWS_TAPER_CIRCULAR_PER_SHOT;
-extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1, IDX, IDY;
+extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1, IDX, IDY, FILTSIZE;
+extern float SRTRADIUS;
 extern char JACOBIAN[], MFILE[];
 extern FILE *FP;
 extern float VP0_1, VP0_2, DVP0, GRAD0_1, GRAD0_2, DGRAD0;
@@ -209,16 +210,40 @@ void eikonal_adj(float **TT, float *Tres, float **lam, int **recpos, int ntr, in
     rzs_check(rz_eikonal_adj_shot(c, 0, &TT[1][1], ntr > 0 ? &Tres[1] : NULL, &lam[1][1], &iters), "eikonal_adj");
 }
 
+/* The per-shot gradient taper of ass_grad.c:31-37 on context c, whose shot
+ * s is the reference's shot NSHOT1 + s: SWS_TAPER_CIRCULAR_PER_SHOT 1
+ * (taper_grad_shot.c, SRTRADIUS) or 2 (taper_grad_shot1.c, FILTSIZE) over the
+ * nshots sources of srcpos; other values apply none, as in ass_grad.c.  The
+ * library keeps its tables while the settings stay the same. */
+static int rzs_taper(rz_ctx *c, float **srcpos, int nshots)
+{
+    const int mode = SWS_TAPER_CIRCULAR_PER_SHOT == 1 || SWS_TAPER_CIRCULAR_PER_SHOT == 2
+                         ? SWS_TAPER_CIRCULAR_PER_SHOT : 0;
+    if (mode == 0 || nshots <= 0) {
+        rzs_check(rz_set_shot_taper(c, 0, 0, 0.f, 0, 0, NULL, NULL, 0), "per-shot taper");
+        return mode;
+    }
+    float *x = rzs_alloc((size_t)nshots * sizeof(float)), *z = rzs_alloc((size_t)nshots * sizeof(float));
+    for (int n = 0; n < nshots; ++n) {
+        x[n] = srcpos[1][n + 1];
+        z[n] = srcpos[2][n + 1];
+    }
+    rzs_check(rz_set_shot_taper(c, 0, mode, SRTRADIUS, FILTSIZE, nshots, x, z, NSHOT1 - 1), "per-shot taper");
+    free(x);
+    free(z);
+    return mode;
+}
+
 void ass_grad(float **grad, float **grad_shot, float **S, float **lam, float **srcpos, int nshots, int **recpos,
               int ntr, int ishot)
 {
-    (void)nshots;
-    (void)ishot;
-    if (SWS_TAPER_CIRCULAR_PER_SHOT != 0)                   /* ass_grad.c:31-37 */
-        err(" rajzel_amd: the per-shot gradient taper (SWS_TAPER_CIRCULAR_PER_SHOT) is not supported");
     rzs_single(srcpos, recpos, ntr);
     rzs_slowness(S);
-    rzs_check(rz_ass_grad_shot(g_rzs.ctx, 0, &lam[1][1], &grad_shot[1][1], &grad[1][1]), "ass_grad");
+    if (rzs_taper(g_rzs.ctx, srcpos, nshots) != 0)          /* ass_grad.c:31-37 */
+        rzs_check(rz_ass_grad_shot_at(g_rzs.ctx, 0, ishot - 1, &lam[1][1], &grad_shot[1][1], &grad[1][1]),
+                  "ass_grad");
+    else
+        rzs_check(rz_ass_grad_shot(g_rzs.ctx, 0, &lam[1][1], &grad_shot[1][1], &grad[1][1]), "ass_grad");
 }
 
 /* The last shot's fields and receiver data, which the reference leaves in
@@ -263,11 +288,10 @@ double grad_obj(float **grad, float **S, float **TT, float **lam, float *Tmod, f
     const size_t ncell = (size_t)NX * NY;
     double L2 = 0.0, L2sum = 0.0;
     SEISMO = 0;                                             /* grad_obj.c:24 */
-    if (SWS_TAPER_CIRCULAR_PER_SHOT != 0)                   /* ass_grad.c:31-37 */
-        err(" rajzel_amd: the per-shot gradient taper (SWS_TAPER_CIRCULAR_PER_SHOT) is not supported");
     if (ns > 0) {
         rz_ctx *c = rzs_context(srcpos, NSHOT1, NSHOT2, recpos, ntr);
         rzs_slowness(S);
+        rzs_taper(c, srcpos, nshots);                       /* ass_grad.c:31-37 */
         if (ntr > 0) {
             /* calc_FA_res.c:28-37: every evaluation reads every shot's pick
              * file with fscanf("%e") into Tobs, in shot order (a value that

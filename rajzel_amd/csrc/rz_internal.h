@@ -16,6 +16,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
 namespace rz {
 
 constexpr int kWave = 64;
@@ -126,14 +129,52 @@ hipError_t launch_residual(const ModelDev *md, int model, int ntr, int s0, int s
                            float *carry, float *tres, double *l2_shot, double *l2_sum, hipStream_t st);
 hipError_t launch_eval_init(float *carry, const float *carry0, int ntr_total, double *l2_sum, int nmodels,
                             float *grad, long long ncells, hipStream_t st);
+// Per-shot gradient taper of ass_grad.c:31-37, applied to each shot's
+// frame-copied lambda*S^3 before it joins the sum.
+//   mode 1  taper_grad_shot.c:43-84: the value is multiplied by the factor of
+//           the shot's source disc, then by the factor of every receiver
+//           disc that holds the cell, in receiver order (float products in
+//           the reference's order; factors of exactly 1 are left out).
+//   mode 2  taper_grad_shot1.c:32-62: cells of any source or receiver window
+//           take 1e-5.
+// Factors are computed on the host with the C library's pow and cos, as the
+// reference computes them.
+struct TaperDev {
+    int mode;                  // 0 none, 1, 2
+    int src0;                  // source index of the launch's first shot
+    const int *box;            // mode 1, per source: k0, k1, c0, c1 of its window (k0 > k1: empty)
+    const long long *off;      // mode 1, per source: offset of its window in fsrc
+    const float *fsrc;         // mode 1: window factors, row-major per window
+    const unsigned *start;     // per row-major cell, nx*ny + 1 entries: [start[i], start[i+1])
+                               // indexes frec (mode 1); non-empty marks the cell (mode 2)
+    const float *frec;         // mode 1: receiver factors of each cell, in receiver order
+};
 hipError_t launch_gradient(const ModelDev *md_host, int model, const ModelDev *md_dev, const int *slot_of,
-                           const int *lsel, int nsh, Slots slots, float *grad, int accumulate, hipStream_t st);
+                           const int *lsel, int nsh, Slots slots, float *grad, int accumulate, const TaperDev &tp,
+                           hipStream_t st);
+// cp_grad_frame.c:10-53 on a row-major NY x NX field on the device (the
+// copy grad_obj.c:70 applies to the shot sum).
+hipError_t launch_grad_frame(int nx, int ny, float *grad, hipStream_t st);
 // Layout-A field -> row-major; with sel != nullptr the field is `alt` where
 // *sel is nonzero (a lambda that ended in LC).
 hipError_t launch_field_to_rowmajor(const ModelDev *md_host, const float *field, const float *alt, const int *sel,
                                     float *out, int strip, hipStream_t st);
 hipError_t launch_rowmajor_to_field(const ModelDev *md_host, const float *in, float *field, int family,
                                     hipStream_t st);
+
+// Host tables of the per-shot taper (rz_taper.cpp), the contents of a
+// TaperDev.  build_taper returns an empty string, or why it cannot build
+// them (more than max_entries factors).
+struct TaperTables {
+    std::vector<int> box;         // mode 1: k0, k1, c0, c1 per source
+    std::vector<long long> off;   // mode 1: per source
+    std::vector<float> fsrc;      // mode 1
+    std::vector<unsigned> start;  // nx*ny + 1
+    std::vector<float> frec;      // mode 1
+};
+std::string build_taper(int mode, int nx, int ny, float dh, float radius, int filtsize, int nsrc,
+                        const float *src_x_m, const float *src_z_m, int ntr, const int *rec_ix, const int *rec_iy,
+                        size_t max_entries, TaperTables &t);
 
 // Message of the last failed call that has no context (rz_last_error(NULL)).
 void set_global_error(const char *msg);

@@ -2094,7 +2094,8 @@ __global__ void k_eval_init(float *carry, const float *carry0, int ntr_total, do
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ void k_gradient(const ModelDev *__restrict__ mds, int model, const int *__restrict__ slot_of,
-                           const int *__restrict__ lsel, int nsh, Slots slots, float *grad, int accumulate)
+                           const int *__restrict__ lsel, int nsh, Slots slots, float *grad, int accumulate,
+                           TaperDev tp)
 {
     constexpr int BR = kWave * R;
     const ModelDev md = mds[model];
@@ -2110,13 +2111,49 @@ __global__ void k_gradient(const ModelDev *__restrict__ mds, int model, const in
         const int pos = (kk == k && cc == c) ? (int)p : addr_a<R>(kk, cc, md.bstride);
         const double s = (double)md.SA[pos];
         const double s3 = s * s * s;
-        float g = accumulate ? grad[(long long)k * md.nx + c] : 0.f;
-        for (int j = 0; j < nsh; ++j) {
-            const float *L = lsel[j] ? slots.LC : slots.LA;   // where the shot's lambda converged
-            const float lam = L[(long long)slot_of[j] * slots.slot_stride + pos];
-            g += (float)((double)lam * s3);
+        const long long cell = (long long)k * md.nx + c;
+        float g = accumulate ? grad[cell] : 0.f;
+        if (tp.mode == 0) {
+            for (int j = 0; j < nsh; ++j) {
+                const float *L = lsel[j] ? slots.LC : slots.LA;   // where the shot's lambda converged
+                const float lam = L[(long long)slot_of[j] * slots.slot_stride + pos];
+                g += (float)((double)lam * s3);
+            }
+        } else {
+            // the taper acts on the frame-copied value at the cell's own position
+            const unsigned r0 = tp.start[cell], r1 = tp.start[cell + 1];
+            for (int j = 0; j < nsh; ++j) {
+                const float *L = lsel[j] ? slots.LC : slots.LA;
+                const float lam = L[(long long)slot_of[j] * slots.slot_stride + pos];
+                float v = (float)((double)lam * s3);
+                if (tp.mode == 2) {
+                    if (r1 > r0) v = 1e-5f;
+                } else {
+                    const int *bx = tp.box + 4 * (tp.src0 + j);
+                    if (k >= bx[0] && k <= bx[1] && c >= bx[2] && c <= bx[3])
+                        v *= tp.fsrc[tp.off[tp.src0 + j] + (long long)(k - bx[0]) * (bx[3] - bx[2] + 1) +
+                                     (c - bx[2])];
+                    for (unsigned i = r0; i < r1; ++i) v *= tp.frec[i];
+                }
+                g += v;
+            }
         }
-        grad[(long long)k * md.nx + c] = g;
+        grad[cell] = g;
+    }
+}
+
+// cp_grad_frame.c:10-53: rows 0-1 and ny-2..ny-1 take row 2 / ny-3, then
+// columns 0-1 and nx-2..nx-1 take column 2 / nx-3; for nx, ny >= 5 every
+// frame cell thus copies the interior cell at its clamped position, which
+// the copy does not change.
+__global__ void k_grad_frame(int nx, int ny, float *grad)
+{
+    const long long n = (long long)nx * ny;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(i / nx), c = (int)(i - (long long)k * nx);
+        const int kk = clampi(k, 2, ny - 3), cc = clampi(c, 2, nx - 3);
+        if (kk != k || cc != c) grad[i] = grad[(long long)kk * nx + cc];
     }
 }
 
@@ -2305,11 +2342,17 @@ hipError_t launch_eval_init(float *carry, const float *carry0, int ntr_total, do
 }
 
 hipError_t launch_gradient(const ModelDev *mh, int model, const ModelDev *md, const int *slot_of, const int *lsel,
-                           int nsh, Slots slots, float *grad, int accumulate, hipStream_t st)
+                           int nsh, Slots slots, float *grad, int accumulate, const TaperDev &tp, hipStream_t st)
 {
     const long long n = (long long)mh->nb * mh->bstride;
     RZ_DISPATCH_R(mh->R, hipLaunchKernelGGL(k_gradient<RR>, dim3(grid_for(n, 256)), dim3(256), 0, st, md, model,
-                                            slot_of, lsel, nsh, slots, grad, accumulate));
+                                            slot_of, lsel, nsh, slots, grad, accumulate, tp));
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_frame(int nx, int ny, float *grad, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_grad_frame, dim3(grid_for((long long)nx * ny, 256)), dim3(256), 0, st, nx, ny, grad);
     return hipGetLastError();
 }
 

@@ -26,7 +26,7 @@ EXPORTS = (
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
     "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb", "rz_set_split", "rz_get_split",
     "rz_read_picks", "rz_picks_parses", "rz_picks_clear", "rz_gauss_filt_noalloc",
-    "rz_slot_bytes", "rz_get_stop_passes",
+    "rz_slot_bytes", "rz_get_stop_passes", "rz_set_shot_taper", "rz_ass_grad_shot_at",
 )
 
 
@@ -90,6 +90,8 @@ def lib():
                 "rz_eikonal_shot": (i, [vp, i, i, i, vp, vp, C.POINTER(i)]),
                 "rz_eikonal_adj_shot": (i, [vp, i, vp, vp, vp, C.POINTER(i)]),
                 "rz_ass_grad_shot": (i, [vp, i, vp, vp, vp]),
+                "rz_ass_grad_shot_at": (i, [vp, i, i, vp, vp, vp]),
+                "rz_set_shot_taper": (i, [vp, i, i, f, i, i, vp, vp, i]),
                 "rz_sync": (i, [vp]),
                 "rz_rows_per_lane": (i, [vp]),
                 "rz_device_bytes": (C.c_size_t, [vp]),
@@ -390,12 +392,31 @@ class Context:
         self._chk(lib().rz_eikonal_adj_shot(self.h, model, _ptr(T), _ptr(tr), _ptr(lam), C.byref(it)))
         return lam, it.value
 
-    def ass_grad_shot(self, model, lam, grad):
+    def ass_grad_shot(self, model, lam, grad, src=None):
+        """ass_grad for one shot: (grad_shot, grad + grad_shot); with `src`
+        (the shot's index in the taper's sources) the per-shot taper set by
+        set_shot_taper applies."""
         lam = np.ascontiguousarray(lam, np.float32)
         g = np.ascontiguousarray(grad, np.float32).copy()
         gs = np.empty_like(lam)
-        self._chk(lib().rz_ass_grad_shot(self.h, model, _ptr(lam), _ptr(gs), _ptr(g)))
+        if src is None:
+            self._chk(lib().rz_ass_grad_shot(self.h, model, _ptr(lam), _ptr(gs), _ptr(g)))
+        else:
+            self._chk(lib().rz_ass_grad_shot_at(self.h, model, int(src), _ptr(lam), _ptr(gs), _ptr(g)))
         return gs, g
+
+    def set_shot_taper(self, model, mode, radius_m=0.0, filtsize=0, src_x_m=(), src_z_m=(), first_shot=0):
+        """Per-shot gradient taper (rz_set_shot_taper): mode 1 radial with
+        radius_m, mode 2 windows of half-size filtsize; source positions of
+        the whole survey in metres; shot s of the model is source
+        first_shot + s."""
+        sx = np.ascontiguousarray(src_x_m, np.float32)
+        sz = np.ascontiguousarray(src_z_m, np.float32)
+        if sx.shape != sz.shape:
+            raise ValueError("src_x_m and src_z_m differ in length")
+        self._chk(lib().rz_set_shot_taper(self.h, model, int(mode), float(radius_m), int(filtsize), sx.size,
+                                          _ptr(sx) if sx.size else None, _ptr(sz) if sz.size else None,
+                                          int(first_shot)))
 
     def set_graph(self, on=True):
         """Replay whole evaluations as HIP graphs (rz_set_graph)."""

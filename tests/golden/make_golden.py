@@ -30,6 +30,13 @@ Cases
   gauss_filt.npz  gauss_filt() of a random 37x53 field and of a gradient-like
                   48x130 field, with odd, even (made odd) and negative sizes
                   (no normalisation), and a size of 0
+  taper.npz       grad_obj with the per-shot gradient tapers of ass_grad.c:
+                  31-37 (SWS_TAPER_CIRCULAR_PER_SHOT 1: SRTRADIUS 35 and 80 m;
+                  2: FILTSIZE 1 and 0) on workloads.taper_case (windows that
+                  run off the grid sideways) with single-shot ass_grad of
+                  shots 0 and 3, and on 3 shots of the 200x600 C2 geometry
+                  (radial taper at shots 57-59; windows of FILTSIZE 0), the
+                  latter as a SHA-256 digest of the gradient and four rows
 """
 import os
 import sys
@@ -159,7 +166,66 @@ def gauss_filt(ref):
     np.savez_compressed(os.path.join(OUT, "gauss_filt.npz"), **out)
 
 
+TAPER_SMALL = [("r35", 1, 35.0, 1), ("r80", 1, 80.0, 1), ("w1", 2, 80.0, 1), ("w0", 2, 80.0, 0)]
+# C2 geometry (sources at row 1, receivers at row 2): FILTSIZE 0 keeps every
+# taper_grad_shot1.c window on the grid
+TAPER_C2 = [("c2_r80", 1, 80.0, 1, 57), ("c2_w0", 2, 80.0, 0, 0)]
+
+
+def _taper_picks(ref, sv, seed):
+    St = W.slowness(sv.vp_true)
+    tm = np.stack([ref.eikonal(St, float(sv.src_x_m[s]), float(sv.src_z_m[s]), sv.rec_ix, sv.rec_iy)[1]
+                   for s in range(sv.nshots)])
+    return (tm + np.random.default_rng(seed).normal(0, sv.noise_s, tm.shape)).astype(np.float32)
+
+
+def tapers(ref):
+    """grad_obj with the per-shot gradient tapers (ass_grad.c:31-37), and
+    single-shot ass_grad with them."""
+    import hashlib
+    out = {}
+    sv = W.taper_case()
+    ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+    tobs = _taper_picks(ref, sv, 6)
+    for name, sws, radius, filt in TAPER_SMALL:
+        ref.setup(sv.nx, sv.ny, sv.dh, sv.time, sv.ttnorm, sv.eps_adj)
+        ref.set_taper(sws, radius, filt)
+        o = ref.grad_obj(sv.S, sv.src_x_m, sv.src_z_m, sv.rec_ix, sv.rec_iy, tobs)
+        out["small_tobs"] = o["tobs"]   # as read back from the pick files
+        out[f"small_{name}_set"] = np.array([sws, radius, filt], np.float64)
+        out[f"small_{name}_grad"] = o["grad"]
+        out[f"small_{name}_L2"] = np.float64(o["L2"])
+        # single-shot ass_grad of shots 0 and 3 on a random lambda and grad
+        rng = np.random.default_rng(90 + sws * 10 + filt)
+        lam = rng.normal(0, 1, (sv.ny, sv.nx)).astype(np.float32)
+        g0 = rng.normal(0, 1, (sv.ny, sv.nx)).astype(np.float32)
+        out[f"small_{name}_lam"], out[f"small_{name}_g0"] = lam, g0
+        for shot in (0, 3):
+            gs, g = ref.ass_grad_at(sv.S, lam, g0, sv.src_x_m, sv.src_z_m, sv.rec_ix, sv.rec_iy, shot)
+            out[f"small_{name}_shot{shot}_gs"], out[f"small_{name}_shot{shot}_g"] = gs, g
+    full = W.c2_gradient()
+    ref.setup(full.nx, full.ny, full.dh, full.time, full.ttnorm, full.eps_adj)
+    for name, sws, radius, filt, first in TAPER_C2:
+        sub = W.c2_gradient()
+        sel = slice(first, first + 3)
+        sub.src_x_m, sub.src_z_m = full.src_x_m[sel].copy(), full.src_z_m[sel].copy()
+        ref.setup(full.nx, full.ny, full.dh, full.time, full.ttnorm, full.eps_adj)
+        tobs = _taper_picks(ref, sub, 7)
+        ref.set_taper(sws, radius, filt)
+        o = ref.grad_obj(sub.S, sub.src_x_m, sub.src_z_m, sub.rec_ix, sub.rec_iy, tobs)
+        out[f"{name}_set"] = np.array([sws, radius, filt, first], np.float64)
+        out[f"{name}_tobs"] = o["tobs"]
+        out[f"{name}_d_grad"] = np.array(hashlib.sha256(np.ascontiguousarray(o["grad"]).tobytes()).hexdigest())
+        out[f"{name}_grad_rows"] = o["grad"][[1, 2, 100, 198]]
+        out[f"{name}_L2"] = np.float64(o["L2"])
+        print(f"  {name}: L2 {o['L2']:.9e}", flush=True)
+    np.savez_compressed(os.path.join(OUT, "taper.npz"), **out)
+
+
 def main():
+    if sys.argv[1:] == ["taper"]:
+        tapers(Reference())
+        return
     ref = Reference()
     sweeps(ref)
     sweeps_adj(ref)
@@ -174,6 +240,7 @@ def main():
                       [(2, 5)], 73)
     c1_forward(ref)
     gauss_filt(ref)
+    tapers(ref)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

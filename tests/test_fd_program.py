@@ -140,6 +140,19 @@ def test_fatt_wolfe_gauss_filter():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kw", [dict(SWS_TAPER_CIRCULAR_PER_SHOT=1, SRTRADIUS=30.0),
+                                dict(SWS_TAPER_CIRCULAR_PER_SHOT=2, FILTSIZE=0)])
+def test_fatt_per_shot_taper(kw):
+    """The per-shot gradient tapers of ass_grad.c:31-37 in the inversion.
+    Sources sit on row 1, so FILTSIZE 0 keeps taper_grad_shot1.c's windows
+    on the grid (a larger one writes through a row pointer above the
+    reference's matrix)."""
+    sv = P.survey()
+    obs = _observed(sv)
+    _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3, **kw))
+
+
+@pytest.mark.gpu
 def test_grid_search_program():
     sv = P.survey(nx=60, ny=30, nshots=3, seed=23)
     obs = _observed(sv)
