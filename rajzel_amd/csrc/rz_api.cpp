@@ -90,6 +90,7 @@ struct rz_ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<int> hist_f, hist_a;    // outer iterations per shot in the last run (-1 unknown)
+    std::vector<int> hist_pf, hist_pa;  // stopping-test passes per shot in the last run (ord_f/ord_a words)
     ShotDev *plan_d = nullptr;          // [4][total_shots]: forward split / rest, adjoint split / rest
     int plan_n[4] = {0, 0, 0, 0};
     std::vector<int> plan_ids;          // shot order of the plan now on the device
@@ -423,9 +424,13 @@ int check_convergence(rz_ctx *c, bool adj)
     if (c->total_shots == 0) return RZ_OK;
     int serr = 0;
     HIPCHK(c, hipMemcpyAsync(&serr, c->ss.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    std::vector<int> pf(c->total_shots), pa(c->total_shots);
     HIPCHK(c, hipMemcpyAsync(itf.data(), c->it_f_d, itf.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    if (adj)
+    HIPCHK(c, hipMemcpyAsync(pf.data(), c->ord_f_d, pf.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (adj) {
         HIPCHK(c, hipMemcpyAsync(ita.data(), c->it_a_d, ita.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(pa.data(), c->ord_a_d, pa.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (serr != 0) {
         c->split = false;
@@ -435,8 +440,14 @@ int check_convergence(rz_ctx *c, bool adj)
         return kSplitRetry;
     }
     c->hist_f = itf;
-    if (adj) c->hist_a = ita;
-    else if ((int)c->hist_a.size() != c->total_shots) c->hist_a.assign(c->total_shots, -1);
+    c->hist_pf = pf;
+    if (adj) {
+        c->hist_a = ita;
+        c->hist_pa = pa;
+    } else if ((int)c->hist_a.size() != c->total_shots) {
+        c->hist_a.assign(c->total_shots, -1);
+        c->hist_pa.assign(c->total_shots, 0);
+    }
     for (int i = 0; i < c->total_shots; ++i)
         if (itf[i] < 0 || (adj && ita[i] < 0))
             return fail(c, RZ_ERR_NOT_CONVERGED, "shot %d did not converge within max_iter", i);
@@ -512,10 +523,19 @@ int plan_mixed(rz_ctx *c, int per)
     const int ns = c->ncu - T;
     std::vector<int> ids;
     ids.reserve(4 * (size_t)T);
-    for (const std::vector<int> *h : {&c->hist_f, &c->hist_a}) {
+    for (int ph = 0; ph < 2; ++ph) {
+        const std::vector<int> &h = ph ? c->hist_a : c->hist_f, &hp = ph ? c->hist_pa : c->hist_pf;
+        // a shot's expected time in sixteenths of an outer iteration: an
+        // ordered-sum pass of the stopping test costs about half an
+        // iteration, a count pass about a sixteenth
+        std::vector<int> cost(T);
+        for (int i = 0; i < T; ++i) {
+            const int w = (int)hp.size() == T ? hp[i] : 0;
+            cost[i] = 16 * h[i] + 8 * (w >> 16) + (w & 0xFFFF);
+        }
         std::vector<int> ord(T);
         for (int i = 0; i < T; ++i) ord[i] = i;
-        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return (*h)[a] > (*h)[b]; });
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
         // Workgroup w of a launch runs on XCD w mod 8 (a split shot's two at
         // list position j run on XCD j mod 8).  The slowest shot of the phase
         // goes first, to XCD 0, and the one-workgroup list puts its fastest
