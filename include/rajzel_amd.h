@@ -249,6 +249,42 @@ int rz_set_graph(rz_ctx *ctx, int on);
  * reported by rz_last_error(NULL). */
 int rz_gauss_filt(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy, int on_device);
 
+/* Gradient tapers of precond.c:43-53, taper_grad.c:11-299, on an NY*NX
+ * row-major field in place, bitwise.  `sws` is taper_grad's argument:
+ *   1  vertical taper (SWS_TAPER_GRAD_VERT): columns GRADT1 < i < GRADT4
+ *      (1-based) take the Gaussian window of taper_grad.c:62-75, the others 0;
+ *   2  horizontal taper (SWS_TAPER_GRAD_HOR): rows below GRADT1 0, rows
+ *      GRADT1..GRADT2 the window of taper_grad.c:137-146, deeper rows
+ *      pow(j*DH, EXP_TAPER_GRAD_HOR);
+ *   3  radial taper at every source and every receiver
+ *      (SWS_TAPER_GRAD_SOURCES), as taper_grad.c:199-258, with SRTRADIUS;
+ *   4..6  the coefficients of taper.bin (SWS_TAPER_FILE), given here as its
+ *      NX*NY floats in file order (x outer, y inner).
+ * Window values, powers and disc factors are computed on the host with the C
+ * library's exp, pow and cos, as the reference computes them; the device
+ * multiplies each cell by its factors in the reference's order.  coeff (host,
+ * NY*NX row-major, may be NULL) receives taper_coeff as the reference writes
+ * it to taper_coeff_vert.bin, _hor.bin and _file.bin (sws 1, 2, 4..6; left
+ * unchanged for sws 3, whose coefficients the reference does not write).
+ * field is a host (on_device = 0) or device pointer on HIP device `device`;
+ * hip_stream orders the work, and the call returns when it is done.  The
+ * sws 3 factors (at most 2^28) are kept between calls with the same geometry.
+ * Errors are reported by rz_last_error(NULL). */
+typedef struct rz_grad_taper {
+    int sws;                          /* 1, 2, 3, or 4..6 (file)                   */
+    int gradt1, gradt2, gradt3, gradt4;   /* GRADT1..GRADT4 (sws 1, 2)             */
+    float dh;                         /* DH (sws 2, 3)                             */
+    float exp_taper_grad_hor;         /* EXP_TAPER_GRAD_HOR (sws 2)                */
+    float srtradius;                  /* SRTRADIUS in metres (sws 3)               */
+    int nsrc;                         /* sws 3: all sources, srcpos in metres      */
+    const float *src_x_m, *src_z_m;
+    int ntr;                          /* sws 3: all receivers, 0-based cells       */
+    const int *rec_ix, *rec_iy;
+    const float *file;                /* sws 4..6: NX*NY floats, x outer (host)    */
+} rz_grad_taper;
+int rz_taper_grad(int device, void *hip_stream, float *field, int nx, int ny, const rz_grad_taper *taper, float *coeff,
+                  int on_device);
+
 /* rz_gauss_filt on a device field without any device allocation.  No
  * counterpart in the reference: it serves callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops, harnesses

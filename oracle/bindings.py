@@ -260,6 +260,9 @@ class Reference:
         L.rzref_set_taper.restype = None
         L.rzref_ass_grad_at.argtypes = [_f32, _f32, C.c_int, _f32, _f32, C.c_int, _i32, _i32, C.c_int, _f32, _f32]
         L.rzref_ass_grad_at.restype = None
+        L.rzref_taper_grad.argtypes = [_f32, C.c_int, _f32, _f32, C.c_int, _i32, _i32, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_float, C.c_float]
+        L.rzref_taper_grad.restype = None
 
     def setup(self, nx, ny, dh, time=6.0, ttnorm=1e-10, eps_adj=1e-9):
         self.nx, self.ny = nx, ny
@@ -322,6 +325,31 @@ class Reference:
         self.lib.rzref_ass_grad_at(self._f(S), self._f(lam), len(sx), sx, sz, len(rec_ix), self._i1(rec_ix),
                                    self._i1(rec_iy), int(ishot) + 1, gs, g)
         return gs, g
+
+    def taper_grad(self, A, sws, gradt=(0, 0, 0, 0), exp_hor=0.0, srtradius=80.0, src_x_m=(), src_z_m=(),
+                   rec_ix=(), rec_iy=(), taper_file=None):
+        """The reference's taper_grad() (NX, NY, DH from setup) on a copy of A:
+        returns (tapered field, taper_coeff as written to taper_coeff_*.bin
+        or None).  It runs in a scratch directory: taper_file (NX*NY floats,
+        x outer) is written there as taper.bin for sws 4..6."""
+        import tempfile
+        A = self._f(A).copy()
+        sx, sz = self._f(np.asarray(src_x_m, np.float32)), self._f(np.asarray(src_z_m, np.float32))
+        name = {1: "taper_coeff_vert.bin", 2: "taper_coeff_hor.bin"}.get(sws, "taper_coeff_file.bin")
+        cwd = os.getcwd()
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            os.chdir(d)
+            try:
+                if taper_file is not None:
+                    np.ascontiguousarray(taper_file, np.float32).tofile("taper.bin")
+                self.lib.rzref_taper_grad(A, len(sx), sx, sz, len(rec_ix), self._i1(rec_ix), self._i1(rec_iy), sws,
+                                          *[int(g) for g in gradt], exp_hor, srtradius)
+                coeff = None
+                if sws != 3 and os.path.exists(name):   # x outer, y inner
+                    coeff = np.fromfile(name, np.float32).reshape(self.nx, self.ny).T.copy()
+            finally:
+                os.chdir(cwd)
+        return A, coeff
 
     def gauss_filt(self, A, fsx, fsy):
         """The reference's gauss_filt() (NX, NY from setup) on a copy of A."""

@@ -164,6 +164,30 @@ void rzref_ass_grad_at(const float *S, const float *lam, int nshots, const float
     free_imatrix(rec, 1, 3, 1, ntr > 0 ? ntr : 1);
 }
 
+/* taper_grad() (precond.c:43-53) on a field (in/out): sws 1..6 with the
+ * globals GRADT1..4, EXP_TAPER_GRAD_HOR, SRTRADIUS; nsrc sources (metres)
+ * and ntr receivers (1-based cells) for sws 3.  The reference reads
+ * taper.bin (sws 4..6) and writes taper_coeff_*.bin in the working
+ * directory; its log lines go to FP (/dev/null here unless set). */
+void rzref_taper_grad(float *field_io, int nsrc, const float *srcx, const float *srcy, int ntr, const int *rx1,
+                      const int *ry1, int sws, int g1, int g2, int g3, int g4, float exp_hor, float srtradius)
+{
+    if (!FP) FP = fopen("/dev/null", "w");
+    IDX = 1; IDY = 1;
+    GRADT1 = g1; GRADT2 = g2; GRADT3 = g3; GRADT4 = g4;
+    EXP_TAPER_GRAD_HOR = exp_hor;
+    SRTRADIUS = srtradius;
+    float **g = mat_from(field_io);
+    const int ns = nsrc > 0 ? nsrc : 1;
+    float **srcpos = matrix(1, 8, 1, ns);
+    for (int i = 1; i <= nsrc; ++i) { srcpos[1][i] = srcx[i - 1]; srcpos[2][i] = srcy[i - 1]; }
+    int **rec = recpos_from(ntr, rx1, ry1);
+    taper_grad(g, srcpos, nsrc, rec, ntr, sws);
+    mat_to(g, field_io);
+    free_matrix(g, 1, NY, 1, NX); free_matrix(srcpos, 1, 8, 1, ns);
+    free_imatrix(rec, 1, 3, 1, ntr > 0 ? ntr : 1);
+}
+
 static void write_picks_file(const char *stem, int shot1, int ntr, const float *v)
 {
     char fn[STRING_SIZE2];

@@ -4,7 +4,7 @@
  *
  * Linked into the reference program in place of eikonal.c, sweep.c,
  * eikonal_adj.c, sweep_adj.c, ass_grad.c, grad_obj.c, forward.c,
- * gauss_filt.c, grid_search.c, LBFGS.c, readmod.c and writemod.c, it keeps
+ * gauss_filt.c, taper_grad.c, grid_search.c and LBFGS.c, it keeps
  * their fd.h signatures, their globals and their side effects, so that
  * fatt.c, PCG.c, wolfels.c, parabolicls.c, step_length_est.c and rajzel.c
  * call it unchanged:
@@ -27,9 +27,11 @@
  *               gradient (precond.c:35-37, SPATFILTER = 4) and of the
  *               search direction (fatt.c:179), on the GPU; linked in
  *               place of gauss_filt.c
+ *   taper_grad  (fd.h:149, taper_grad.c:11-299)  the gradient tapers of
+ *               precond.c:43-53, multiplied on the GPU; log lines, taper.bin
+ *               and taper_coeff_*.bin files as the reference's
  *   LBFGS       (fd.h:88, LBFGS.c:12-226)  the reference's arithmetic,
  *               with each history file read or written in one call
- *   readmod, writemod (fd.h:127,170)  binary models in one call
  *
  * sweep() and sweep_adj() (fd.h:145,147) have no callers outside
  * eikonal.c and eikonal_adj.c; their work happens inside the kernels.
@@ -50,14 +52,17 @@
 /* globals of the reference (globvar.h:12-29,38,69,96) */
 extern float DH, TIME, TTNORM, EPS_ADJ;
 extern int NX, NY, SEISMO, SNAP, MYID, INFO, NSHOT1, NSHOT2, SWS_TAPER_CIRCULAR_PER_SHOT;
-extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1, IDX, IDY, FILTSIZE;
-extern float SRTRADIUS;
+extern int FILT_SIZE_GRAD, FILT_SIZE_GRAD1, IDX, IDY, FILTSIZE, GRADT1, GRADT2, GRADT3, GRADT4;
+extern float SRTRADIUS, EXP_TAPER_GRAD_HOR;
 extern char JACOBIAN[], MFILE[];
 extern FILE *FP;
 extern float VP0_1, VP0_2, DVP0, GRAD0_1, GRAD0_2, DGRAD0;
 extern char SNAP_FILE[], DATA_DIR[];
 
-/* routines of the reference that stay in use (fd.h:52,119,143,166,170,180) */
+/* routines of the reference that stay in use (fd.h:52,119,143,166,170,180;
+ * util.c's matrix helpers) */
+float **matrix(int nrl, int nrh, int ncl, int nch);
+void free_matrix(float **m, int nrl, int nrh, int ncl, int nch);
 void cp_grad_frame(float **A);
 void precond(float **grad, int nsrc, float **srcpos, int **recpos, int ntr, int iter);
 void sum_grad_MPI(float **grad);
@@ -331,6 +336,68 @@ double grad_obj(float **grad, float **S, float **TT, float **lam, float *Tmod, f
     cp_grad_frame(grad);
     precond(grad, nshots, srcpos, recpos, ntr, iter);
     return L2;
+}
+
+/* taper_grad.c:11-299 (precond.c:43-53) with the multiplication on the GPU:
+ * the reference's log lines, its read of taper.bin (sws 4..6) and its
+ * taper_coeff_vert/hor/file.bin are kept; sws 3 writes no file, as there. */
+void taper_grad(float **waveconv, float **srcpos, int nshots, int **recpos, int ntr, int sws)
+{
+    const int msg = sws == 1 || sws == 2 || (sws >= 4 && sws <= 6);
+    if (!(sws >= 1 && sws <= 6)) return;
+    if (sws == 2 && MYID == 0) {                            /* taper_grad.c:124-127 */
+        printf("%d \t %d \t %d \t %d \n", GRADT1, GRADT2, GRADT3, GRADT4);
+        printf("%d \t %d \t %d \n", GRADT2 - GRADT1, GRADT4 - GRADT3, GRADT2 - GRADT1 + 1);
+    }
+    if (msg && MYID == 0) {
+        fprintf(FP, "\n **Message from taper_grid (printed by PE %d):\n", MYID);
+        fprintf(FP, " Coefficients for gradient taper are now calculated.\n");
+    }
+    const size_t n = (size_t)NX * NY;
+    rz_grad_taper t;
+    memset(&t, 0, sizeof t);
+    t.sws = sws;
+    t.gradt1 = GRADT1; t.gradt2 = GRADT2; t.gradt3 = GRADT3; t.gradt4 = GRADT4;
+    t.dh = DH;
+    t.exp_taper_grad_hor = EXP_TAPER_GRAD_HOR;
+    t.srtradius = SRTRADIUS;
+    float *x = NULL, *z = NULL, *file = NULL;
+    int *rx = NULL, *ry = NULL;
+    if (sws == 3) {
+        t.nsrc = nshots > 0 ? nshots : 0;
+        t.ntr = ntr > 0 ? ntr : 0;
+        x = rzs_alloc((size_t)(t.nsrc + 1) * sizeof(float));
+        z = rzs_alloc((size_t)(t.nsrc + 1) * sizeof(float));
+        rx = rzs_alloc((size_t)(t.ntr + 1) * sizeof(int));
+        ry = rzs_alloc((size_t)(t.ntr + 1) * sizeof(int));
+        for (int i = 0; i < t.nsrc; ++i) { x[i] = srcpos[1][i + 1]; z[i] = srcpos[2][i + 1]; }
+        for (int i = 0; i < t.ntr; ++i) { rx[i] = recpos[1][i + 1] - 1; ry[i] = recpos[2][i + 1] - 1; }
+        t.src_x_m = x; t.src_z_m = z; t.rec_ix = rx; t.rec_iy = ry;
+    }
+    if (sws >= 4) {                                         /* taper_grad.c:272-282 */
+        FILE *fp = fopen("taper.bin", "r");
+        if (!fp) err(" rajzel_amd taper_grad: taper.bin could not be opened");
+        file = rzs_alloc(n * sizeof(float));
+        const size_t got = fread(file, sizeof(float), n, fp);
+        for (size_t i = got; i < n; ++i) file[i] = got ? file[got - 1] : 0.f;   /* a failed fread keeps the value */
+        fclose(fp);
+        t.file = file;
+    }
+    float **coeff = sws == 3 ? NULL : matrix(1, NY, 1, NX);
+    const char *dev = getenv("RAJZEL_AMD_DEVICE");
+    if (rz_taper_grad(dev ? atoi(dev) : 0, NULL, &waveconv[1][1], NX, NY, &t, coeff ? &coeff[1][1] : NULL, 0) !=
+        RZ_OK) {
+        snprintf(g_msg, sizeof g_msg, " rajzel_amd taper_grad: %s", rz_last_error(NULL));
+        err(g_msg);
+    }
+    if (coeff) {
+        char name[64];
+        snprintf(name, sizeof name, "%s", sws == 1 ? "taper_coeff_vert.bin"
+                                          : (sws == 2 ? "taper_coeff_hor.bin" : "taper_coeff_file.bin"));
+        writemod(name, coeff, 3);
+        free_matrix(coeff, 1, NY, 1, NX);
+    }
+    free(x); free(z); free(rx); free(ry); free(file);
 }
 
 /* gauss_filt.c:10-139 on the GPU.  The sizes are adjusted in the globals as
@@ -653,60 +720,4 @@ void LBFGS(float **Hgrad, float **grad, float **gradm, int iter, float *y_LBFGS,
     }
     MPI_Barrier(MPI_COMM_WORLD);                             /* LBFGS.c:221-223 */
     exchange_grad_MPI(Hgrad);
-}
-
-/* ---------------------------------------------------------------------------
- * Model files: writemod.c:10-33 (fd.h:170) and readmod.c:10-49 (fd.h:127)
- * with the same log lines and the same bytes, binary fields (format 3) in
- * one fwrite or fread instead of one call per float (about 80 ms per 12 MB
- * model; the program writes a model per iteration).  Other formats go
- * through the reference's writedsk() value by value, as there.
- * --------------------------------------------------------------------------- */
-void writemod(char modfile[], float **array, int format)
-{
-    fprintf(FP, "\n\n PE %d is writing model to \n", MYID);
-    fprintf(FP, "\t%s\n\n", modfile);
-    FILE *fp = fopen(modfile, "w");
-    if (fp == NULL) {
-        snprintf(g_msg, sizeof g_msg, " writemod: %s could not be opened", modfile);
-        err(g_msg);
-    }
-    if (format == 3) {
-        const size_t n = rzs_strided();
-        float *buf = rzs_alloc(n * sizeof(float));
-        rzs_gather(array, buf);
-        const size_t put = fwrite(buf, sizeof(float), n, fp);
-        free(buf);
-        if (put != n) {
-            snprintf(g_msg, sizeof g_msg, " writemod: %s could not be written", modfile);
-            err(g_msg);
-        }
-    } else {
-        for (int i = 1; i <= NX; i += IDX)
-            for (int j = 1; j <= NY; j += IDY) writedsk(fp, array[j][i], format);
-    }
-    fclose(fp);
-}
-
-void readmod(float **Vp)
-{
-    char filename[RZS_PATH + 16];
-    fprintf(FP, "\n... reading P-wave velocity models from file...\n");
-    fprintf(FP, "\t Vp:\n\t %s.vp\n\n", MFILE);
-    snprintf(filename, sizeof filename, "%s.vp", MFILE);
-    FILE *fp = fopen(filename, "r");
-    if (fp == NULL) err(" Could not open model file for Vp ! ");
-    /* every cell, x slow and y fast; past the end of a short file the
-     * reference's fread leaves the last value read */
-    const size_t n = (size_t)NX * NY;
-    float *buf = rzs_alloc(n * sizeof(float));
-    const size_t got = fread(buf, sizeof(float), n, fp);
-    fclose(fp);
-    for (size_t t = got; t < n; ++t) buf[t] = got ? buf[got - 1] : 0.f;
-    size_t t = 0;
-    for (int i = 1; i <= NX; ++i)
-        for (int j = 1; j <= NY; ++j) Vp[j][i] = buf[t++];
-    free(buf);
-    snprintf(filename, sizeof filename, "%s.rajzel.vp", MFILE);   /* readmod.c:45-46 */
-    writemod(filename, Vp, 3);
 }

@@ -175,6 +175,16 @@ struct TaperTables {
 std::string build_taper(int mode, int nx, int ny, float dh, float radius, int filtsize, int nsrc,
                         const float *src_x_m, const float *src_z_m, int ntr, const int *rec_ix, const int *rec_iy,
                         size_t max_entries, TaperTables &t);
+// The product of the radial tapers of taper_grad_shot.c:100-110 around n
+// centres (cx, cy) in metres, in centre order: per row-major cell the
+// factors other than 1, [start[i], start[i+1]) of fac.  An empty string, or
+// why not (more than max_entries factors).
+std::string disc_csr(int nx, int ny, float dh, float R, int n, const float *cx, const float *cy, size_t max_entries,
+                     std::vector<unsigned> &start, std::vector<float> &fac);
+// taper_grad.c's coefficients: per column of the vertical taper (sws 1) and
+// per row of the horizontal one (sws 2).
+std::vector<float> grad_taper_columns(int nx, int g1, int g2, int g3, int g4);
+std::vector<float> grad_taper_rows(int ny, int g1, int g2, float dh, float expo);
 
 // Message of the last failed call that has no context (rz_last_error(NULL)).
 void set_global_error(const char *msg);

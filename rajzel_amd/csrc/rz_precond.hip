@@ -1,6 +1,7 @@
 // Gaussian smoothing on the device (gauss_filt.c:10-139): the SPATFILTER = 4
 // branch of precond.c:35-37 on the gradient, and fatt.c:179 on the search
-// direction.
+// direction.  Below it, the gradient tapers of precond.c:43-53
+// (taper_grad.c): elementwise products with factors computed on the host.
 //
 // The weights are computed on the host exactly as the reference computes
 // them (float arguments, libm exp in double, float sum and division).  Each
@@ -12,11 +13,13 @@
 // y offset.  The weights are wave-uniform (scalar loads).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/rajzel_amd.h"
@@ -152,8 +155,171 @@ int scratch_on(int device, float **out)
     return RZ_OK;
 }
 
+// taper_grad.c's multiplications (waveconv[j][i] *= taper_coeff[j][i]):
+// by a per-column (sws 1), per-row (sws 2) or per-cell (sws 4..6) factor.
+__global__ void k_taper_scale(float *f, int nx, int ny, const float *col, const float *row, const float *cell)
+{
+    const long long n = (long long)nx * ny;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(i / nx), c = (int)(i - (long long)k * nx);
+        f[i] *= col ? col[c] : (row ? row[k] : cell[i]);
+    }
+}
+
+// sws 3: each cell times its disc factors, in source-then-receiver order.
+__global__ void k_taper_discs(float *f, long long n, const unsigned *start, const float *fac)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const unsigned a = start[i], b = start[i + 1];
+        if (a == b) continue;
+        float v = f[i];
+        for (unsigned q = a; q < b; ++q) v *= fac[q];
+        f[i] = v;
+    }
+}
+
+int grid_blocks(long long n)
+{
+    return (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), 65535);
+}
+
+// The disc factors of the last sws-3 geometry, kept on its device.
+struct DiscCache {
+    int device = -1, nx = 0, ny = 0;
+    float dh = 0.f, R = 0.f;
+    std::vector<float> cx, cy;
+    unsigned *start = nullptr;
+    float *fac = nullptr;
+};
+std::mutex g_dc_mu;
+DiscCache g_dc;
+constexpr size_t kDiscMaxFactors = (size_t)1 << 28;
+
+// Tables of the sws-3 taper for this geometry on `device` (current), built
+// and uploaded unless the cache holds them.  Call with g_dc_mu held.
+int disc_tables(int device, int nx, int ny, const rz_grad_taper &t)
+{
+    std::vector<float> cx(t.nsrc + t.ntr), cy(t.nsrc + t.ntr);
+    for (int n = 0; n < t.nsrc; ++n) {
+        cx[n] = t.src_x_m[n];
+        cy[n] = t.src_z_m[n];
+    }
+    for (int n = 0; n < t.ntr; ++n) {   // recpos * DH, a float product (taper_grad.c:239)
+        cx[t.nsrc + n] = (float)(t.rec_ix[n] + 1) * t.dh;
+        cy[t.nsrc + n] = (float)(t.rec_iy[n] + 1) * t.dh;
+    }
+    if (g_dc.start && g_dc.device == device && g_dc.nx == nx && g_dc.ny == ny && g_dc.dh == t.dh &&
+        g_dc.R == t.srtradius && g_dc.cx == cx && g_dc.cy == cy)
+        return RZ_OK;
+    std::vector<unsigned> start;
+    std::vector<float> fac;
+    const std::string why = disc_csr(nx, ny, t.dh, t.srtradius, (int)cx.size(), cx.data(), cy.data(),
+                                     kDiscMaxFactors, start, fac);
+    if (!why.empty()) return fail_global(RZ_ERR_INVALID, "taper_grad: %s", why.c_str());
+    if (g_dc.start) {
+        (void)hipSetDevice(g_dc.device);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_dc.start);
+        (void)hipFree(g_dc.fac);
+        g_dc = DiscCache{};
+        (void)hipSetDevice(device);
+    }
+    unsigned *sd = nullptr;
+    float *fd = nullptr;
+    if (hipMalloc(&sd, start.size() * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&fd, std::max<size_t>(fac.size(), 1) * sizeof(float)) != hipSuccess) {
+        if (sd) (void)hipFree(sd);
+        return fail_global(RZ_ERR_NOMEM, "taper_grad: device allocation failed");
+    }
+    if (hipMemcpy(sd, start.data(), start.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess ||
+        (!fac.empty() && hipMemcpy(fd, fac.data(), fac.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)) {
+        (void)hipFree(sd);
+        (void)hipFree(fd);
+        return fail_global(RZ_ERR_HIP, "taper_grad: upload failed");
+    }
+    g_dc.device = device; g_dc.nx = nx; g_dc.ny = ny; g_dc.dh = t.dh; g_dc.R = t.srtradius;
+    g_dc.cx = std::move(cx); g_dc.cy = std::move(cy);
+    g_dc.start = sd; g_dc.fac = fd;
+    return RZ_OK;
+}
+
 }  // namespace
 }  // namespace rz
+
+extern "C" int rz_taper_grad(int device, void *hip_stream, float *field, int nx, int ny, const rz_grad_taper *t,
+                             float *coeff, int on_device)
+{
+    using namespace rz;
+    if (!field || !t) return fail_global(RZ_ERR_INVALID, "taper_grad: null pointer");
+    if (nx < 1 || ny < 1 || (long long)nx * ny > (1LL << 31))
+        return fail_global(RZ_ERR_INVALID, "taper_grad: NX, NY must be >= 1 with NX*NY <= 2^31");
+    const int sws = t->sws;
+    if (sws < 1 || sws > 6) return fail_global(RZ_ERR_INVALID, "taper_grad: sws %d (1..6)", sws);
+    if (sws == 3) {
+        if (!(t->srtradius > 0.f) || !std::isfinite(t->srtradius) || !(t->dh > 0.f))
+            return fail_global(RZ_ERR_INVALID, "taper_grad: SRTRADIUS and DH must be finite and > 0");
+        if (t->nsrc < 0 || t->ntr < 0 || (t->nsrc > 0 && (!t->src_x_m || !t->src_z_m)) ||
+            (t->ntr > 0 && (!t->rec_ix || !t->rec_iy)))
+            return fail_global(RZ_ERR_INVALID, "taper_grad: source or receiver positions missing");
+    }
+    if (sws >= 4 && !t->file) return fail_global(RZ_ERR_INVALID, "taper_grad: no taper file contents");
+    const size_t n = (size_t)nx * ny;
+    // host factors (sws 1, 2, 4..6)
+    std::vector<float> col, row, cell;
+    if (sws == 1) col = grad_taper_columns(nx, t->gradt1, t->gradt2, t->gradt3, t->gradt4);
+    if (sws == 2) row = grad_taper_rows(ny, t->gradt1, t->gradt2, t->dh, t->exp_taper_grad_hor);
+    if (sws >= 4) {   // taper.bin is read x outer, y inner (taper_grad.c:275-282)
+        cell.resize(n);
+        for (int c = 0; c < nx; ++c)
+            for (int k = 0; k < ny; ++k) cell[(size_t)k * nx + c] = t->file[(size_t)c * ny + k];
+    }
+    if (coeff && sws != 3)
+        for (int k = 0; k < ny; ++k)
+            for (int c = 0; c < nx; ++c)
+                coeff[(size_t)k * nx + c] = sws == 1 ? col[c] : (sws == 2 ? row[k] : cell[(size_t)k * nx + c]);
+
+    hipStream_t st = (hipStream_t)hip_stream;
+    if (hipSetDevice(device) != hipSuccess) return fail_global(RZ_ERR_HIP, "taper_grad: no device %d", device);
+    std::unique_lock<std::mutex> lock(g_dc_mu, std::defer_lock);
+    if (sws == 3) {
+        lock.lock();
+        if (const int rc = disc_tables(device, nx, ny, *t)) return rc;
+    }
+    float *fd = nullptr, *cd = nullptr;
+    auto cleanup = [&](int rc) {
+        if (st) (void)hipStreamSynchronize(st);
+        else (void)hipDeviceSynchronize();
+        if (fd && !on_device) (void)hipFree(fd);
+        if (cd) (void)hipFree(cd);
+        return rc;
+    };
+    const size_t ncoef = sws == 1 ? (size_t)nx : (sws == 2 ? (size_t)ny : (sws == 3 ? 0 : n));
+    if ((!on_device && hipMalloc(&fd, n * sizeof(float)) != hipSuccess) ||
+        (ncoef && hipMalloc(&cd, ncoef * sizeof(float)) != hipSuccess))
+        return cleanup(fail_global(RZ_ERR_NOMEM, "taper_grad: device allocation failed"));
+    if (on_device) fd = field;
+    hipError_t e = on_device ? hipSuccess
+                             : hipMemcpyAsync(fd, field, n * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && ncoef) {
+        const float *src = sws == 1 ? col.data() : (sws == 2 ? row.data() : cell.data());
+        e = hipMemcpyAsync(cd, src, ncoef * sizeof(float), hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess) {
+        if (sws == 3)
+            hipLaunchKernelGGL(k_taper_discs, dim3(grid_blocks((long long)n)), dim3(256), 0, st, fd, (long long)n,
+                               g_dc.start, g_dc.fac);
+        else
+            hipLaunchKernelGGL(k_taper_scale, dim3(grid_blocks((long long)n)), dim3(256), 0, st, fd, nx, ny,
+                               sws == 1 ? cd : nullptr, sws == 2 ? cd : nullptr, sws >= 4 ? cd : nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && !on_device) e = hipMemcpyAsync(field, fd, n * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = st ? hipStreamSynchronize(st) : hipDeviceSynchronize();
+    if (e != hipSuccess) return cleanup(fail_global(RZ_ERR_HIP, "taper_grad: %s", hipGetErrorString(e)));
+    return cleanup(RZ_OK);
+}
 
 extern "C" int rz_gauss_filt(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy,
                              int on_device)

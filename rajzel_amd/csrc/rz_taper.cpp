@@ -1,5 +1,6 @@
-// Host tables of the per-shot gradient tapers of ass_grad.c:31-37
-// (SWS_TAPER_CIRCULAR_PER_SHOT), which k_gradient applies on the device.
+// Host tables of the gradient tapers: the per-shot ones of ass_grad.c:31-37
+// (SWS_TAPER_CIRCULAR_PER_SHOT), which k_gradient applies on the device, and
+// those of taper_grad.c (precond.c:43-53), which rz_taper_grad applies.
 //
 // Mode 1 (taper_grad_shot.c:43-84) multiplies each shot's gradient by a
 // radial cosine taper around its source and then around every receiver in
@@ -63,6 +64,91 @@ bool disc_box(int nx, int ny, float dh, float cx, float cy, float R, int &i0, in
 
 }  // namespace
 
+float taper_exponent()
+{
+    const float frh = 0.1f, f0 = 1e-3f;
+    // taper_grad_shot.c:38, taper_grad.c:204: from float frh - f0, in double, stored as float
+    return (float)(-(std::log(frh - f0) - std::log(1.0 - f0)) / std::log(2.0));
+}
+
+std::string disc_csr(int nx, int ny, float dh, float R, int n, const float *cx, const float *cy, size_t max_entries,
+                     std::vector<unsigned> &start, std::vector<float> &fac_out)
+{
+    const float a = taper_exponent();
+    const size_t ncell = (size_t)nx * ny;
+    start.assign(ncell + 1, 0u);
+    std::vector<unsigned> cell;
+    std::vector<float> fac;
+    for (int q = 0; q < n; ++q) {
+        int i0, i1, j0, j1;
+        if (!disc_box(nx, ny, dh, cx[q], cy[q], R, i0, i1, j0, j1)) continue;
+        for (int iy = j0; iy <= j1; ++iy)
+            for (int ix = i0; ix <= i1; ++ix) {
+                const float f = disc_factor(ix, iy, dh, cx[q], cy[q], R, a);
+                if (f == 1.f) continue;
+                if (cell.size() >= max_entries)
+                    return "the discs cover too many cells (more than " + std::to_string(max_entries) +
+                           " factors); reduce the radius";
+                cell.push_back((unsigned)((size_t)(iy - 1) * nx + (ix - 1)));
+                fac.push_back(f);
+            }
+    }
+    // a stable bucket by cell keeps each cell's factors in disc order
+    for (unsigned q : cell) ++start[q + 1];
+    for (size_t i = 0; i < ncell; ++i) start[i + 1] += start[i];
+    fac_out.resize(fac.size());
+    std::vector<unsigned> fill(start.begin(), start.end() - 1);
+    for (size_t e = 0; e < cell.size(); ++e) fac_out[fill[cell[e]]++] = fac[e];
+    return std::string();
+}
+
+// taper_grad.c:62-75 and 137-146: a value of the Gaussian window, a = 3
+// (float), exp(-(1/2) * A * A) in double with A = (a * x) / (L/2.0) (a float
+// product divided in double), stored as float; x is i - L, or
+// i - (ifw - L2) with L = L2 for the far side of the vertical taper.
+static float gauss_window(int x, int L)
+{
+    const float a = 3.f;
+    const double A = (double)(a * (float)x) / (L / 2.0);
+    return (float)std::exp(-(1.0 / 2.0) * A * A);
+}
+
+std::vector<float> grad_taper_columns(int nx, int g1, int g2, int g3, int g4)
+{
+    // taper_grad.c:44-102: window[1..ifw], then column i (1-based) in
+    // (GRADT1, GRADT4) takes window[h], h counting those columns from 1
+    const int L = g2 - g1, L2 = g4 - g3, ifw = g4 - g1;
+    std::vector<float> col(nx, 0.f);
+    for (int i = 1; i <= nx; ++i) {
+        if (!(i > g1 && i < g4)) continue;
+        const int h = i - std::max(g1, 0);
+        float w = 1.0f;
+        if (h <= L) w = gauss_window(h - L, L);
+        if (h >= ifw - L2) w = gauss_window(h - (ifw - L2), L2);
+        col[i - 1] = w;
+    }
+    return col;
+}
+
+std::vector<float> grad_taper_rows(int ny, int g1, int g2, float dh, float expo)
+{
+    // taper_grad.c:118-187: rows GRADT1..GRADT2 take window[h] (h counting
+    // rows from GRADT1, or from 1), deeper rows pow((float)(j*DH), EXP) in
+    // double stored as float, shallower rows 0
+    const int L = g2 - g1;
+    std::vector<float> row(ny, 0.f);
+    for (int j = 1; j <= ny; ++j) {
+        float g = 0.f;
+        if (j >= g1 && j <= g2) {
+            const int h = 1 + j - std::max(g1, 1);
+            g = h <= L ? gauss_window(h - L, L) : 1.0f;
+        }
+        if (j > g2) g = (float)std::pow((double)(float)((float)j * dh), (double)expo);
+        row[j - 1] = g;
+    }
+    return row;
+}
+
 std::string build_taper(int mode, int nx, int ny, float dh, float radius, int filtsize, int nsrc,
                         const float *src_x_m, const float *src_z_m, int ntr, const int *rec_ix, const int *rec_iy,
                         size_t max_entries, TaperTables &t)
@@ -71,9 +157,7 @@ std::string build_taper(int mode, int nx, int ny, float dh, float radius, int fi
     const size_t ncell = (size_t)nx * ny;
     t.start.assign(ncell + 1, 0u);
     if (mode == 1) {
-        const float frh = 0.1f, f0 = 1e-3f;
-        // taper_grad_shot.c:38: the exponent from float frh - f0, in double, stored as float
-        const float a = (float)(-(std::log(frh - f0) - std::log(1.0 - f0)) / std::log(2.0));
+        const float a = taper_exponent();
         const float R = radius;
         // source windows
         t.box.assign(4 * (size_t)nsrc, 0);
@@ -105,30 +189,14 @@ std::string build_taper(int mode, int nx, int ny, float dh, float radius, int fi
             for (int k = ki0; k <= ki1; ++k)
                 for (int c = ci0; c <= ci1; ++c) t.fsrc.push_back(win[(size_t)(k + 1 - j0) * w + (c + 1 - i0)]);
         }
-        // receiver discs: (cell, factor) in receiver order, then a stable
-        // bucket by cell
-        std::vector<unsigned> cell;
-        std::vector<float> fac;
+        // receiver discs, in receiver order
+        std::vector<float> cx(ntr), cy(ntr);
         for (int n = 0; n < ntr; ++n) {
-            const float cx = (float)(rec_ix[n] + 1) * dh, cy = (float)(rec_iy[n] + 1) * dh;
-            int i0, i1, j0, j1;
-            if (!disc_box(nx, ny, dh, cx, cy, R, i0, i1, j0, j1)) continue;
-            for (int iy = j0; iy <= j1; ++iy)
-                for (int ix = i0; ix <= i1; ++ix) {
-                    const float f = disc_factor(ix, iy, dh, cx, cy, R, a);
-                    if (f == 1.f) continue;
-                    if (cell.size() >= max_entries)
-                        return "the receiver discs cover too many cells (more than " + std::to_string(max_entries) +
-                               " factors); reduce the radius";
-                    cell.push_back((unsigned)((size_t)(iy - 1) * nx + (ix - 1)));
-                    fac.push_back(f);
-                }
+            cx[n] = (float)(rec_ix[n] + 1) * dh;   // recpos * DH, a float product
+            cy[n] = (float)(rec_iy[n] + 1) * dh;
         }
-        for (unsigned q : cell) ++t.start[q + 1];
-        for (size_t i = 0; i < ncell; ++i) t.start[i + 1] += t.start[i];
-        t.frec.resize(fac.size());
-        std::vector<unsigned> fill(t.start.begin(), t.start.end() - 1);
-        for (size_t e = 0; e < cell.size(); ++e) t.frec[fill[cell[e]]++] = fac[e];
+        const std::string why = disc_csr(nx, ny, dh, R, ntr, cx.data(), cy.data(), max_entries, t.start, t.frec);
+        if (!why.empty()) return why;
     } else if (mode == 2) {
         // windows as linear ranges of the contiguous matrix, one per row:
         // a difference array over the cells

@@ -26,7 +26,7 @@ EXPORTS = (
     "rz_get_profile", "rz_set_residuals", "rz_adjoint", "rz_get_stream", "rz_set_graph",
     "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb", "rz_set_split", "rz_get_split",
     "rz_read_picks", "rz_picks_parses", "rz_picks_clear", "rz_gauss_filt_noalloc",
-    "rz_slot_bytes", "rz_get_stop_passes", "rz_set_shot_taper", "rz_ass_grad_shot_at",
+    "rz_slot_bytes", "rz_get_stop_passes", "rz_set_shot_taper", "rz_ass_grad_shot_at", "rz_taper_grad",
 )
 
 
@@ -107,6 +107,7 @@ def lib():
                 "rz_autotune_rows": (i, [i, i, C.POINTER(ModelDesc), C.c_size_t, C.POINTER(i), C.POINTER(d)]),
                 "rz_gauss_filt": (i, [i, vp, vp, i, i, C.POINTER(i), C.POINTER(i), i]),
                 "rz_gauss_filt_noalloc": (i, [i, vp, vp, i, i, C.POINTER(i), C.POINTER(i)]),
+                "rz_taper_grad": (i, [i, vp, vp, i, i, vp, vp, i]),
                 "rz_get_tnb": (i, [vp, i, vp, i]),
                 "rz_set_split": (i, [vp, i]),
                 "rz_get_split": (i, [vp, C.POINTER(i), C.POINTER(i)]),
@@ -187,6 +188,49 @@ def gauss_filt(field, fsx, fsy, device=0, stream=None):
     if rc != RZ_OK:
         raise RzError(rc, lib().rz_last_error(None).decode())
     return out, x.value, y.value
+
+
+class GradTaper(C.Structure):
+    _fields_ = [
+        ("sws", C.c_int), ("gradt1", C.c_int), ("gradt2", C.c_int), ("gradt3", C.c_int), ("gradt4", C.c_int),
+        ("dh", C.c_float), ("exp_taper_grad_hor", C.c_float), ("srtradius", C.c_float),
+        ("nsrc", C.c_int), ("src_x_m", C.POINTER(C.c_float)), ("src_z_m", C.POINTER(C.c_float)),
+        ("ntr", C.c_int), ("rec_ix", C.POINTER(C.c_int)), ("rec_iy", C.POINTER(C.c_int)),
+        ("file", C.POINTER(C.c_float)),
+    ]
+
+
+def taper_grad(field, sws, gradt=(0, 0, 0, 0), dh=0.0, exp_hor=0.0, srtradius=0.0, src_x_m=(), src_z_m=(),
+               rec_ix=(), rec_iy=(), taper_file=None, device=0, stream=None, want_coeff=False):
+    """taper_grad.c (precond.c:43-53) through rz_taper_grad on a numpy
+    (NY, NX) float32 array (on a copy) or a torch CUDA tensor (in place):
+    sws 1 vertical, 2 horizontal, 3 at all sources and receivers, 4..6
+    from taper_file (NX*NY floats, x outer).  Returns the field, and with
+    want_coeff also taper_coeff as the reference writes it (None for sws 3)."""
+    keep = []
+
+    def arr(a, dt, ct):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(ct)) if a.size else None
+    sx, sz = arr(src_x_m, np.float32, C.c_float), arr(src_z_m, np.float32, C.c_float)
+    rx, ry = arr(rec_ix, np.int32, C.c_int), arr(rec_iy, np.int32, C.c_int)
+    t = GradTaper(int(sws), *[int(g) for g in gradt], float(dh), float(exp_hor), float(srtradius),
+                  len(np.atleast_1d(src_x_m)) if np.size(src_x_m) else 0, sx, sz,
+                  len(np.atleast_1d(rec_ix)) if np.size(rec_ix) else 0, rx, ry,
+                  arr(taper_file, np.float32, C.c_float) if taper_file is not None else None)
+    if hasattr(field, "data_ptr"):
+        ny, nx = field.shape
+        out, ptr, on_dev = field, C.c_void_p(field.data_ptr()), 1
+    else:
+        out = np.ascontiguousarray(field, np.float32).copy()
+        ny, nx = out.shape
+        ptr, on_dev = out.ctypes.data_as(C.c_void_p), 0
+    coeff = np.zeros((ny, nx), np.float32) if want_coeff and sws != 3 else None
+    rc = lib().rz_taper_grad(device, stream, ptr, nx, ny, C.byref(t), _ptr(coeff), on_dev)
+    if rc != RZ_OK:
+        raise RzError(rc, lib().rz_last_error(None).decode())
+    return (out, coeff) if want_coeff else out
 
 
 def read_picks(path, tobs):

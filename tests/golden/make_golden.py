@@ -37,6 +37,13 @@ Cases
                   shots 0 and 3, and on 3 shots of the 200x600 C2 geometry
                   (radial taper at shots 57-59; windows of FILTSIZE 0), the
                   latter as a SHA-256 digest of the gradient and four rows
+  grad_taper.npz  taper_grad() of precond.c:43-53 on a seeded 34x60 field:
+                  vertical (5 GRADT sets) and horizontal (5 GRADT/EXP sets)
+                  tapers with the taper_coeff files they write, the
+                  source/receiver taper (SRTRADIUS 35, 80 m) and the file
+                  taper (sws 4, 6); the stock horizontal taper on 1001x3001
+                  and the source/receiver taper on the C2 geometry as
+                  digests and rows
 """
 import os
 import sys
@@ -222,9 +229,61 @@ def tapers(ref):
     np.savez_compressed(os.path.join(OUT, "taper.npz"), **out)
 
 
+GT_COLS = [(22, 25, 40, 50), (0, 5, 50, 59), (3, 3, 10, 10), (-2, 4, 55, 70), (5, 12, 20, 30)]
+GT_ROWS = [((22, 25, 0, 0), 0.0), ((2, 9, 0, 0), 1.0), ((0, 5, 0, 0), 0.5), ((3, 3, 0, 0), 2.0), ((-1, 6, 0, 0), 0.25)]
+
+
+def grad_tapers(ref):
+    """taper_grad() (precond.c:43-53) on seeded random fields: every sws
+    with several geometries on 34x60, the stock horizontal taper on the
+    1001x3001 grid and the source/receiver taper on the 200x600 C2
+    geometry (the last two as digests and rows)."""
+    import hashlib
+    dig = lambda a: np.array(hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest())  # noqa: E731
+    out = {}
+    rng = np.random.default_rng(505)
+    nx, ny = 60, 34
+    A = rng.normal(0, 1, (ny, nx)).astype(np.float32)
+    out["small_in"] = A
+    ref.setup(nx, ny, 10.0)
+    for i, g in enumerate(GT_COLS):
+        B, c = ref.taper_grad(A, 1, gradt=g)
+        out[f"v{i}_gradt"], out[f"v{i}_out"], out[f"v{i}_coeff"] = np.array(g, np.int32), B, c
+    for i, (g, e) in enumerate(GT_ROWS):
+        B, c = ref.taper_grad(A, 2, gradt=g, exp_hor=e)
+        out[f"h{i}_gradt"], out[f"h{i}_exp"], out[f"h{i}_out"], out[f"h{i}_coeff"] = \
+            np.array(g, np.int32), np.float32(e), B, c
+    sv = W.taper_case()
+    for i, R in enumerate((35.0, 80.0)):
+        B, _ = ref.taper_grad(A, 3, srtradius=R, src_x_m=sv.src_x_m, src_z_m=sv.src_z_m, rec_ix=sv.rec_ix,
+                              rec_iy=sv.rec_iy)
+        out[f"r{i}_radius"], out[f"r{i}_out"] = np.float32(R), B
+    tf = rng.uniform(0, 2, nx * ny).astype(np.float32)
+    for sws in (4, 6):
+        B, c = ref.taper_grad(A, sws, taper_file=tf)
+        out[f"f{sws}_out"], out[f"f{sws}_coeff"] = B, c
+    out["taper_file"] = tf
+    # the stock horizontal taper (GRADT 22, 25, EXP 0) on the 1001x3001 grid
+    big = np.random.default_rng(506).normal(0, 1e-3, (1001, 3001)).astype(np.float32)
+    ref.setup(3001, 1001, 5.0)
+    B, c = ref.taper_grad(big, 2, gradt=(22, 25, 490, 500), exp_hor=0.0)
+    out["c4hor_d_out"], out["c4hor_rows"], out["c4hor_d_coeff"] = dig(B), B[[0, 21, 22, 24, 25, 1000]], dig(c)
+    # sources and receivers of the C2 geometry, SRTRADIUS 80 m
+    c2 = W.c2_gradient()
+    mid = np.random.default_rng(507).normal(0, 1e-3, (c2.ny, c2.nx)).astype(np.float32)
+    ref.setup(c2.nx, c2.ny, c2.dh)
+    B, _ = ref.taper_grad(mid, 3, srtradius=80.0, src_x_m=c2.src_x_m, src_z_m=c2.src_z_m, rec_ix=c2.rec_ix,
+                          rec_iy=c2.rec_iy)
+    out["c2disc_d_out"], out["c2disc_rows"] = dig(B), B[[0, 1, 2, 5, 100, 199]]
+    np.savez_compressed(os.path.join(OUT, "grad_taper.npz"), **out)
+
+
 def main():
     if sys.argv[1:] == ["taper"]:
         tapers(Reference())
+        return
+    if sys.argv[1:] == ["grad_taper"]:
+        grad_tapers(Reference())
         return
     ref = Reference()
     sweeps(ref)
@@ -241,6 +300,7 @@ def main():
     c1_forward(ref)
     gauss_filt(ref)
     tapers(ref)
+    grad_tapers(ref)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

@@ -1,6 +1,7 @@
 """The reference program driving the MI355X path through the fd.h entry
-points (rajzel_amd/csrc/rz_fd_surface.c): rajzel.c, fatt.c, LBFGS.c,
-PCG.c, wolfels.c, parabolicls.c, gauss_filt.c and grid_search.c run unchanged, and every
+points (rajzel_amd/csrc/rz_fd_surface.c, which also replaces gauss_filt.c,
+taper_grad.c, grid_search.c and LBFGS.c): rajzel.c, fatt.c, PCG.c,
+wolfels.c, parabolicls.c, precond.c, readmod.c and writemod.c run unchanged, and every
 file they write must equal, byte for byte, what the unchanged reference
 program writes from the same inputs.
 
@@ -150,6 +151,33 @@ def test_fatt_per_shot_taper(kw):
     sv = P.survey()
     obs = _observed(sv)
     _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3, **kw))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kw", [dict(SWS_TAPER_GRAD_VERT=1, GRADT1=5, GRADT2=12, GRADT3=55, GRADT4=68),
+                                dict(SWS_TAPER_GRAD_HOR=1, GRADT1=3, GRADT2=6, EXP_TAPER_GRAD_HOR=0.5),
+                                dict(SWS_TAPER_GRAD_SOURCES=1, SRTRADIUS=30.0),
+                                dict(SWS_TAPER_FILE=1),
+                                dict(SWS_TAPER_GRAD_VERT=1, SWS_TAPER_GRAD_HOR=1, SWS_TAPER_GRAD_SOURCES=1,
+                                     SWS_TAPER_FILE=1, GRADT1=3, GRADT2=6, GRADT3=60, GRADT4=70, SRTRADIUS=25.0)],
+                         ids=["vertical", "horizontal", "sources", "file", "all"])
+def test_fatt_gradient_tapers(kw):
+    """The gradient tapers of precond.c:43-53 (taper_grad.c) in the
+    inversion, with the Gaussian filter's companion taper files: models,
+    misfit log, L-BFGS files and taper_coeff_*.bin equal the reference's."""
+    sv = P.survey()
+    obs = _observed(sv)
+
+    def make():
+        ws = P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3, **kw)
+        if kw.get("SWS_TAPER_FILE"):   # read by taper_grad.c:272 from the working directory
+            tb = np.random.default_rng(9).uniform(0.5, 1.5, sv.nx * sv.ny).astype(np.float32)
+            tb.tofile(os.path.join(ws, "taper.bin"))
+        return ws
+    out = _compare(make)
+    for flag, name in (("SWS_TAPER_GRAD_VERT", "taper_coeff_vert.bin"), ("SWS_TAPER_GRAD_HOR", "taper_coeff_hor.bin"),
+                       ("SWS_TAPER_FILE", "taper_coeff_file.bin")):
+        assert (name in out) == bool(kw.get(flag)), name
 
 
 @pytest.mark.gpu
