@@ -116,6 +116,48 @@ def gauss_filter_timing(nx, ny, fsx=81, fsy=41, reps=5, cpu_rows=84):
     return res
 
 
+def lbfgs_timing(nx, ny, iterate, nlbfgs=20, reps=3):
+    """One LBFGS() call (LBFGS.c:12-226) with a full history of nlbfgs slots
+    (the stock NLBFGS = 20) on an ny x nx grid: the GPU two-loop
+    (rz_lbfgs_step: fields in, direction and the three history vectors out,
+    all included) and the reference's own LBFGS() on one host core (its
+    history files included, one float per fread/fwrite call), over the same
+    iterates, with the directions compared bit for bit.  iterate(k) gives
+    the gradient and model of iteration k (row-major float32 fields)."""
+    import tempfile
+    from rajzel_amd import native as N
+    ptr = lambda it: 1 if it <= 1 else (it - 2) % nlbfgs + 1     # fatt.c:108-119
+    its = [iterate(k) for k in range(1, nlbfgs + 2 + reps)]
+    dev = N.LBFGS(nx, ny, nlbfgs)
+    times, dirs = [], []
+    for it, (g, vp) in enumerate(its, start=1):
+        t0 = time.perf_counter()
+        out = dev.step(it, ptr(it), g, -g, vp)
+        if it > nlbfgs + 1:                                     # every slot filled
+            times.append(time.perf_counter() - t0)
+            dirs.append(out["hgrad"])
+    dev.close()
+    res = {"grid": f"{ny}x{nx}", "nlbfgs": nlbfgs, "dot_products_per_call": 3 * nlbfgs + 2,
+           "calls_timed": len(times), "gpu_ms": float(np.median(times)) * 1e3}
+    try:
+        from oracle.bindings import Reference
+        ref = Reference()
+        ref.setup(nx, ny, 5.0)
+        st = ref.lbfgs_state(nlbfgs)
+        with tempfile.TemporaryDirectory(prefix="rzlb", dir="/tmp") as d:
+            jac = os.path.join(d, "j")
+            for it, (g, vp) in enumerate(its[:nlbfgs + 2], start=1):
+                t0 = time.perf_counter()
+                H = ref.lbfgs(st, jac, it, ptr(it), g, -g, vp)
+                dt = time.perf_counter() - t0
+        res["cpu_reference_ms"] = dt * 1e3
+        res["cpu_cores"] = 1
+        res["bitwise_vs_reference"] = bool(np.array_equal(H.view(np.uint32), dirs[0].view(np.uint32)))
+    except OSError as e:
+        log(f"reference CPU build unavailable for LBFGS ({e})")
+    return res
+
+
 def concurrency_scan(sv, rows_per_lane, counts=(1, 8, 64, 128, 200)):
     """Per-outer-iteration time of the forward and adjoint kernels with n
     shots in flight (n = 1 ... 200, one workgroup per shot): the difference
@@ -273,6 +315,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-precond", action="store_true", help="skip the Gaussian gradient filter timing")
+    ap.add_argument("--no-lbfgs", action="store_true", help="skip the l-BFGS two-loop timing")
     ap.add_argument("--no-split", action="store_true",
                     help="one workgroup per shot in every launch (RAJZEL_AMD_SPLIT=0)")
     ap.add_argument("--concurrency-scan", action="store_true",
@@ -412,10 +455,26 @@ def main():
             "iterations_equal": all(rows[s][2] == itf[s] and rows[s][3] == ita[s] for s in shots),
         }
         log(f"[bench] parity of the CPU sample: {out['parity_sample']}")
+    rows_used = ctx.rows_per_lane
     ctx.close()
     if not args.no_precond:
         out["precond_gauss_filt"] = gauss_filter_timing(sv.nx, sv.ny)
         log(f"[bench] gauss filter: {out['precond_gauss_filt']}")
+    if not args.no_lbfgs:
+        # l-BFGS iterates: gradients of this workload at models moved along a
+        # smooth perturbation (untimed evaluations)
+        X, Z = np.meshgrid(np.arange(sv.nx), np.arange(sv.ny))
+        bump = (np.sin(X / 97.0) * np.cos(Z / 61.0)).astype(np.float32)
+        with N.Context([model_dict(sv)], rows_per_lane=rows_used) as cl:
+            cl.set_picks(0, sv.tobs)
+
+            def iterate(k):
+                vp = (sv.vp + np.float32(3.0 * k) * bump).astype(np.float32)
+                cl.set_slowness(0, W.slowness(vp))
+                cl.gradient()
+                return cl.grad(0), vp
+            out["lbfgs"] = lbfgs_timing(sv.nx, sv.ny, iterate)
+        log(f"[bench] lbfgs: {out['lbfgs']}")
     if args.copy_probe:
         copy_probe(args.copy_probe)
     print(json.dumps(out), flush=True)

@@ -331,6 +331,43 @@ int rz_read_picks(const char *name, float *Tobs, int ntr);
 long rz_picks_parses(void);
 void rz_picks_clear(void);
 
+/* dotp.c:12-41 (fd.h:80) with sw = 0: the float sum tmp += a[i]*b[i] for
+ * i = 0 .. n-1 in that order, computed on device `device` by one workgroup,
+ * exactly (every partial sum is the serial one; a NaN result is a NaN, its
+ * payload bits may differ).  a and b are host arrays, or device arrays with
+ * on_device = 1.  No context. */
+int rz_dotp_ordered(int device, const float *a, const float *b, long long n, int on_device, float *out);
+
+/* The l-BFGS two-loop recursion of LBFGS.c:12-226 (fd.h:88) on the
+ * device, for the unstrided case (IDX = IDY = 1, NLBFGS_vec = NX*NY): the
+ * NLBFGS history slots y, s stay in device memory, every dot product is
+ * dotp.c's serial float sum (as rz_dotp_ordered), every update the
+ * reference's float operations, so the direction is the reference's bit
+ * for bit.  Vectors in "h order" are the reference's vector and file order
+ * (x slow, y fast: h = x*NY + y, LBFGS.c:45-58); fields are row-major
+ * NY x NX (&A[1][1]).  One handle per NX x NY x NLBFGS; single-threaded. */
+typedef struct rz_lbfgs rz_lbfgs;
+int rz_lbfgs_create(rz_lbfgs **out, int device, int nx, int ny, int nlbfgs);
+void rz_lbfgs_destroy(rz_lbfgs *lb);
+const char *rz_lbfgs_error(const rz_lbfgs *lb);
+/* zero_LBFGS.c / zero_LBFGS1.c (fd.h:172,175): every slot's y and s to 0 */
+int rz_lbfgs_zero(rz_lbfgs *lb);
+/* The previous model and gradient (the files <JACOBIAN>_p_vp.old and
+ * _p.old, h order) are kept from the last rz_lbfgs_step; after a restart
+ * they can be set from the files.  has_previous: 1 if there are some. */
+int rz_lbfgs_has_previous(const rz_lbfgs *lb);
+int rz_lbfgs_set_previous(rz_lbfgs *lb, const float *grad_h, const float *vp_h);
+/* One LBFGS() call.  iter > 1: slot `pointer` (LBFGS_pointer, 1-based)
+ * takes y = grad - previous gradient and s = vp - previous model, and hgrad
+ * = H^-1 gradm; gamma, rho[0..NLBFGS-1] and alpha[0..NLBFGS-1] get the
+ * reference's gamma_LBFGS, rho_LBFGS[1..] and alpha_LBFGS[1..].  iter = 1:
+ * hgrad = gradm.  grad_h, vp_h, hgrad_h (any may be NULL) get grad, vp and
+ * hgrad in h order, the contents of _p.old, _p_vp.old and _c.old
+ * (LBFGS.c:178-216).  grad, gradm, vp, hgrad: row-major fields (host). */
+int rz_lbfgs_step(rz_lbfgs *lb, int iter, int pointer, const float *grad, const float *gradm, const float *vp,
+                  float *hgrad, float *grad_h, float *vp_h, float *hgrad_h, float *gamma, float *rho,
+                  float *alpha);
+
 /* Block until the context's stream is idle. */
 int rz_sync(rz_ctx *ctx);
 

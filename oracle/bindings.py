@@ -263,6 +263,29 @@ class Reference:
         L.rzref_taper_grad.argtypes = [_f32, C.c_int, _f32, _f32, C.c_int, _i32, _i32, C.c_int, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_float, C.c_float]
         L.rzref_taper_grad.restype = None
+        L.rzref_lbfgs.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f32, _f32, _f32,
+                                  _f32, _f32, _f32, _f32, _f32, _f32, _f32, _f32]
+        L.rzref_lbfgs.restype = None
+
+    def lbfgs_state(self, nlbfgs):
+        """Zeroed history and work vectors for lbfgs(), 1-based as fatt.c
+        allocates them (fatt.c:74-82)."""
+        n = self.nx * self.ny
+        return {"y": np.zeros(nlbfgs * n + 1, np.float32), "s": np.zeros(nlbfgs * n + 1, np.float32),
+                "q": np.zeros(n + 1, np.float32), "r": np.zeros(n + 1, np.float32),
+                "rho": np.zeros(nlbfgs + 1, np.float32), "alpha": np.zeros(nlbfgs + 1, np.float32),
+                "beta": np.zeros(nlbfgs + 1, np.float32), "nlbfgs": nlbfgs}
+
+    def lbfgs(self, state, jac, it, pointer, grad, gradm, vp, hgrad=None, idx=1, idy=1):
+        """The reference's LBFGS() on row-major (NY, NX) fields; state from
+        lbfgs_state() is updated in place.  Returns Hgrad (cells it does not
+        write keep hgrad's values, zeros by default)."""
+        H = np.zeros((self.ny, self.nx), np.float32) if hgrad is None else self._f(hgrad).copy()
+        st = state
+        self.lib.rzref_lbfgs(os.fsencode(jac), idx, idy, it, pointer, st["nlbfgs"], st["y"], st["s"], st["rho"],
+                             st["alpha"], st["beta"], st["q"], st["r"], self._f(grad), self._f(gradm),
+                             self._f(vp), H)
+        return H
 
     def setup(self, nx, ny, dh, time=6.0, ttnorm=1e-10, eps_adj=1e-9):
         self.nx, self.ny = nx, ny

@@ -279,3 +279,38 @@ void rzref_gauss_filt(float *field, int *fsx, int *fsy)
     mat_to(m, field);
     free_matrix(m, 1, NY, 1, NX);
 }
+
+/* The reference's LBFGS() (LBFGS.c:12-226) on flat row-major fields, with
+ * the caller's 1-based history and work vectors (y, s: nlbfgs*NX*NY + 1
+ * floats; q, r: NX*NY + 1; rho, alpha, beta: nlbfgs + 1) kept between calls
+ * by the caller, IDX = idx, IDY = idy, and its files <jac>_p.old,
+ * _p_vp.old and _c.old.  One rank: LBFGS.c's MPI_Barrier and
+ * exchange_grad_MPI (a broadcast from rank 0) leave everything as it is,
+ * so both are wrapped at link time (-Wl,--wrap) to no-ops. */
+int __wrap_MPI_Barrier(MPI_Comm comm)
+{
+    (void)comm;
+    return MPI_SUCCESS;
+}
+
+void __wrap_exchange_grad_MPI(float **grad)
+{
+    (void)grad;
+}
+
+void rzref_lbfgs(const char *jac, int idx, int idy, int iter, int pointer, int nlbfgs, float *y, float *s,
+                 float *rho, float *alpha, float *beta, float *q, float *r, const float *grad, const float *gradm,
+                 const float *vp, float *hgrad)
+{
+    float **G = mat_from(grad), **Gm = mat_from(gradm), **V = mat_from(vp), **H = mat_from(hgrad);
+    IDX = idx;
+    IDY = idy;
+    MYID = 0;
+    snprintf(JACOBIAN, STRING_SIZE, "%s", jac);
+    LBFGS(H, G, Gm, iter, y, s, rho, alpha, V, q, r, beta, pointer, nlbfgs, NX * NY);
+    mat_to(H, hgrad);
+    free_matrix(G, 1, NY, 1, NX);
+    free_matrix(Gm, 1, NY, 1, NX);
+    free_matrix(V, 1, NY, 1, NX);
+    free_matrix(H, 1, NY, 1, NX);
+}

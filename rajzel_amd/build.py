@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "librajzel_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RZ_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["rz_kernels.hip", "rz_api.cpp", "rz_precond.hip", "rz_picks.cpp", "rz_taper.cpp"]
+SOURCES = ["rz_kernels.hip", "rz_api.cpp", "rz_precond.hip", "rz_picks.cpp", "rz_taper.cpp", "rz_lbfgs.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
@@ -32,14 +32,19 @@ def build(verbose: bool = False) -> str:
     """Compile every source to an object and link the shared library."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd[1:1] = ["-x", "hip"]
-        _run(cmd, verbose)
+        cmds.append(cmd)
         objs.append(obj)
+    # the sources compile independently: one process each, at most 8 at once
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)

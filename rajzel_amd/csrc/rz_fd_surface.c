@@ -4,7 +4,8 @@
  *
  * Linked into the reference program in place of eikonal.c, sweep.c,
  * eikonal_adj.c, sweep_adj.c, ass_grad.c, grad_obj.c, forward.c,
- * gauss_filt.c, taper_grad.c, grid_search.c and LBFGS.c, it keeps
+ * gauss_filt.c, taper_grad.c, grid_search.c, LBFGS.c, zero_LBFGS.c and
+ * zero_LBFGS1.c, it keeps
  * their fd.h signatures, their globals and their side effects, so that
  * fatt.c, PCG.c, wolfels.c, parabolicls.c, step_length_est.c and rajzel.c
  * call it unchanged:
@@ -30,8 +31,10 @@
  *   taper_grad  (fd.h:149, taper_grad.c:11-299)  the gradient tapers of
  *               precond.c:43-53, multiplied on the GPU; log lines, taper.bin
  *               and taper_coeff_*.bin files as the reference's
- *   LBFGS       (fd.h:88, LBFGS.c:12-226)  the reference's arithmetic,
- *               with each history file read or written in one call
+ *   LBFGS       (fd.h:88, LBFGS.c:12-226)  the two-loop recursion on the
+ *               GPU with the reference's arithmetic (unstrided; strided
+ *               on the host), each history file read or written in one
+ *               call; zero_LBFGS and zero_LBFGS1 (fd.h:172,175) with it
  *
  * sweep() and sweep_adj() (fd.h:145,147) have no callers outside
  * eikonal.c and eikonal_adj.c; their work happens inside the kernels.
@@ -107,6 +110,13 @@ static void *rzs_alloc(size_t n)
     void *p = malloc(n ? n : 1);
     if (!p) err(" rajzel_amd: host allocation failed");
     return p;
+}
+
+/* an environment switch: unset -> dflt, else its integer value != 0 */
+static int getenv_flag(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return (v && *v) ? atoi(v) != 0 : dflt;
 }
 
 static void rzs_release(void)
@@ -603,15 +613,24 @@ void grid_search(float **Vp, float **S, float **TT, float *Tmod, float *Tobs, fl
 }
 
 /* ---------------------------------------------------------------------------
- * LBFGS.c:12-226 (fd.h:88): the two-loop recursion of fatt.c:160 with the
- * reference's arithmetic (its dotp() for every dot product, the same float
- * operations in the same order on the caller's history arrays), and the
+ * LBFGS.c:12-226 (fd.h:88): the two-loop recursion of fatt.c:160, and the
  * previous model and gradient exchanged through the same files
- * <JACOBIAN>_p_vp.old, _p.old and _c.old.  The reference reads and writes
- * those files one float per fread/fwrite call, about 80 ms per 12 MB file;
- * here each file is one fread or fwrite of a buffer in the same order (x
- * slow, y fast, strides IDX and IDY).  A file shorter than the grid leaves
- * the rest as the reference's fread leaves it: the last value read.
+ * <JACOBIAN>_p_vp.old, _p.old and _c.old (each read or written in one call;
+ * the reference makes one fread/fwrite call per float, about 80 ms per 12 MB
+ * file; a file shorter than the grid leaves the rest as the reference's
+ * fread leaves it: the last value read).
+ *
+ * Unstrided (IDX = IDY = 1, the vectors NX*NY long), the recursion runs on
+ * the GPU (rz_lbfgs_step): the history slots stay in device memory, and the
+ * direction, gamma, rho and alpha are the reference's bit for bit, every
+ * dot product being dotp()'s serial float sum.  The device keeps the
+ * previous model and gradient itself (the files are still written); it
+ * reads them from the files only when it has none.  Its history is reset by
+ * zero_LBFGS and zero_LBFGS1 below, the only writers of the history besides
+ * LBFGS (check_descent.c:39, fatt.c:232), so the caller's y_LBFGS, s_LBFGS,
+ * q_LBFGS and r_LBFGS arrays are left as they are.  Strided, the reference's
+ * arithmetic runs on the host on the caller's arrays (its dotp() for every
+ * dot product, the same float operations in the same order).
  * --------------------------------------------------------------------------- */
 static size_t rzs_strided(void)
 {
@@ -659,12 +678,83 @@ static void rzs_write_old(const char *suffix, const float *buf, size_t n)
     fclose(fp);
 }
 
+static rz_lbfgs *g_lb;              /* the device history (rank 0) */
+
+static void rzs_lbfgs_release(void)
+{
+    rz_lbfgs_destroy(g_lb);
+    g_lb = NULL;
+}
+
+static void rzs_lbfgs_check(int rc, const char *what)
+{
+    if (rc != RZ_OK) {
+        snprintf(g_msg, sizeof g_msg, " rajzel_amd LBFGS %s: %s", what, rz_lbfgs_error(g_lb));
+        err(g_msg);
+    }
+}
+
+/* LBFGS on the GPU (unstrided); the log lines and files of LBFGS.c:75-216 */
+static void rzs_lbfgs_device(float **Hgrad, float **grad, float **gradm, int iter, float *rho_LBFGS,
+                             float *alpha_LBFGS, float **Vp, int LBFGS_pointer, int NLBFGS)
+{
+    const size_t n = (size_t)NX * NY;
+    if (g_lb == NULL) {
+        const char *dev = getenv("RAJZEL_AMD_DEVICE");
+        if (rz_lbfgs_create(&g_lb, dev ? atoi(dev) : 0, NX, NY, NLBFGS) != RZ_OK) {
+            snprintf(g_msg, sizeof g_msg, " rajzel_amd LBFGS: %s", rz_last_error(NULL));
+            err(g_msg);
+        }
+        atexit(rzs_lbfgs_release);
+    }
+    float *buf = rzs_alloc(3 * n * sizeof(float));
+    float *gh = buf, *vh = buf + n, *hh = buf + 2 * n;
+    if (iter > 1 && !rz_lbfgs_has_previous(g_lb)) {
+        rzs_read_old("_p.old", gh, n);
+        rzs_read_old("_p_vp.old", vh, n);
+        rzs_lbfgs_check(rz_lbfgs_set_previous(g_lb, gh, vh), "previous model");
+    }
+    float gamma = 0.f;
+    rzs_lbfgs_check(rz_lbfgs_step(g_lb, iter, LBFGS_pointer, &grad[1][1], &gradm[1][1], &Vp[1][1], &Hgrad[1][1], gh,
+                                  vh, hh, &gamma, rho_LBFGS + 1, alpha_LBFGS + 1),
+                    "step");
+    if (iter > 1) {
+        printf("MYID = %d \n", MYID);
+        printf("gamma_LBFGS = %e \n", gamma);
+        for (int k = 1; k <= NLBFGS; ++k) printf("rho_LBFGS = %e of k = %d \n", rho_LBFGS[k], k);
+    }
+    rzs_write_old("_p_vp.old", vh, n);
+    rzs_write_old("_p.old", gh, n);
+    rzs_write_old("_c.old", hh, n);
+    free(buf);
+}
+
+/* zero_LBFGS.c:12-31 and zero_LBFGS1.c:12-21 (fd.h:172,175), and the device
+ * history with them */
+void zero_LBFGS(int NLBFGS, int NLBFGS_vec, float *y_LBFGS, float *s_LBFGS, float *q_LBFGS, float *r_LBFGS,
+                float *alpha_LBFGS, float *beta_LBFGS, float *rho_LBFGS)
+{
+    for (long i = 1; i <= (long)NLBFGS_vec * NLBFGS; i++) y_LBFGS[i] = s_LBFGS[i] = 0.0f;
+    for (long i = 1; i <= NLBFGS_vec; i++) q_LBFGS[i] = r_LBFGS[i] = 0.0f;
+    for (int i = 1; i <= NLBFGS; i++) alpha_LBFGS[i] = beta_LBFGS[i] = rho_LBFGS[i] = 0.0f;
+    if (g_lb) rzs_lbfgs_check(rz_lbfgs_zero(g_lb), "zero");
+}
+
+void zero_LBFGS1(int NLBFGS, int NLBFGS_vec, float *y_LBFGS, float *s_LBFGS)
+{
+    for (long i = 1; i <= (long)NLBFGS_vec * NLBFGS; i++) y_LBFGS[i] = s_LBFGS[i] = 0.0f;
+    if (g_lb) rzs_lbfgs_check(rz_lbfgs_zero(g_lb), "zero");
+}
+
 void LBFGS(float **Hgrad, float **grad, float **gradm, int iter, float *y_LBFGS, float *s_LBFGS, float *rho_LBFGS,
            float *alpha_LBFGS, float **Vp, float *q_LBFGS, float *r_LBFGS, float *beta_LBFGS, int LBFGS_pointer,
            int NLBFGS, int NLBFGS_vec)
 {
     (void)beta_LBFGS;   /* the reference keeps beta in a local */
-    if (MYID == 0) {
+    if (MYID == 0 && IDX == 1 && IDY == 1 && NLBFGS_vec == NX * NY && iter >= 1 &&
+        getenv_flag("RAJZEL_AMD_LBFGS", 1)) {
+        rzs_lbfgs_device(Hgrad, grad, gradm, iter, rho_LBFGS, alpha_LBFGS, Vp, LBFGS_pointer, NLBFGS);
+    } else if (MYID == 0) {
         const size_t n = rzs_strided();
         float *buf = rzs_alloc(2 * n * sizeof(float));
         float *old = buf + n;

@@ -27,6 +27,8 @@ EXPORTS = (
     "rz_autotune_rows", "rz_gauss_filt", "rz_get_tnb", "rz_set_split", "rz_get_split",
     "rz_read_picks", "rz_picks_parses", "rz_picks_clear", "rz_gauss_filt_noalloc",
     "rz_slot_bytes", "rz_get_stop_passes", "rz_set_shot_taper", "rz_ass_grad_shot_at", "rz_taper_grad",
+    "rz_dotp_ordered", "rz_lbfgs_create", "rz_lbfgs_destroy", "rz_lbfgs_error", "rz_lbfgs_zero",
+    "rz_lbfgs_has_previous", "rz_lbfgs_set_previous", "rz_lbfgs_step",
 )
 
 
@@ -114,6 +116,14 @@ def lib():
                 "rz_read_picks": (i, [C.c_char_p, vp, i]),
                 "rz_picks_parses": (C.c_long, []),
                 "rz_picks_clear": (None, []),
+                "rz_dotp_ordered": (i, [i, vp, vp, C.c_longlong, i, C.POINTER(f)]),
+                "rz_lbfgs_create": (i, [C.POINTER(vp), i, i, i, i]),
+                "rz_lbfgs_destroy": (None, [vp]),
+                "rz_lbfgs_error": (C.c_char_p, [vp]),
+                "rz_lbfgs_zero": (i, [vp]),
+                "rz_lbfgs_has_previous": (i, [vp]),
+                "rz_lbfgs_set_previous": (i, [vp, vp, vp]),
+                "rz_lbfgs_step": (i, [vp, i, i, vp, vp, vp, vp, vp, vp, vp, C.POINTER(f), vp, vp]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -231,6 +241,78 @@ def taper_grad(field, sws, gradt=(0, 0, 0, 0), dh=0.0, exp_hor=0.0, srtradius=0.
     if rc != RZ_OK:
         raise RzError(rc, lib().rz_last_error(None).decode())
     return (out, coeff) if want_coeff else out
+
+
+def dotp_ordered(a, b, device=0):
+    """dotp.c's serial float dot product sum(a[i]*b[i]), i ascending, of two
+    float32 vectors, on the device (rz_dotp_ordered)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    if a.shape != b.shape or a.ndim != 1:
+        raise ValueError("a and b must be 1-D float32 vectors of one length")
+    out = C.c_float()
+    rc = lib().rz_dotp_ordered(device, _ptr(a), _ptr(b), a.size, 0, C.byref(out))
+    if rc != RZ_OK:
+        raise RzError(rc, lib().rz_last_error(None).decode())
+    return np.float32(out.value)
+
+
+class LBFGS:
+    """LBFGS.c's two-loop recursion on the device (rz_lbfgs_*), for an
+    NY x NX grid and nlbfgs history slots.  step() takes the row-major fields
+    grad, gradm, vp and returns the direction hgrad and the reference's
+    scalars and file contents."""
+
+    def __init__(self, nx, ny, nlbfgs, device=0):
+        self.nx, self.ny, self.K = nx, ny, nlbfgs
+        h = C.c_void_p()
+        rc = lib().rz_lbfgs_create(C.byref(h), device, nx, ny, nlbfgs)
+        if rc != RZ_OK:
+            raise RzError(rc, lib().rz_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != RZ_OK:
+            raise RzError(rc, lib().rz_lbfgs_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().rz_lbfgs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def zero(self):
+        self._chk(lib().rz_lbfgs_zero(self.h))
+
+    def has_previous(self):
+        return bool(lib().rz_lbfgs_has_previous(self.h))
+
+    def set_previous(self, grad_h, vp_h):
+        g = np.ascontiguousarray(grad_h, np.float32)
+        v = np.ascontiguousarray(vp_h, np.float32)
+        self._chk(lib().rz_lbfgs_set_previous(self.h, _ptr(g), _ptr(v)))
+
+    def step(self, it, pointer, grad, gradm, vp):
+        """One LBFGS() call; returns a dict with hgrad (NY, NX), grad_h,
+        vp_h, hgrad_h (h order), gamma, rho, alpha (iter > 1)."""
+        shp = (self.ny, self.nx)
+        f = [np.ascontiguousarray(x, np.float32).reshape(shp) for x in (grad, gradm, vp)]
+        out = {"hgrad": np.empty(shp, np.float32)}
+        for k in ("grad_h", "vp_h", "hgrad_h"):
+            out[k] = np.empty(self.nx * self.ny, np.float32)
+        gamma = C.c_float(np.nan)
+        rho = np.full(self.K, np.nan, np.float32)
+        alpha = np.full(self.K, np.nan, np.float32)
+        self._chk(lib().rz_lbfgs_step(self.h, it, pointer, _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), _ptr(out["hgrad"]),
+                                      _ptr(out["grad_h"]), _ptr(out["vp_h"]), _ptr(out["hgrad_h"]),
+                                      C.byref(gamma), _ptr(rho), _ptr(alpha)))
+        out.update(gamma=np.float32(gamma.value), rho=rho, alpha=alpha)
+        return out
 
 
 def read_picks(path, tobs):

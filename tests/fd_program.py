@@ -45,16 +45,18 @@ def par_values(nx, ny, dh, invmat, **kw):
     return v
 
 
-def make_workspace(sv, vp_model, invmat, picks=None, spatfilter=0, **kw):
+def make_workspace(sv, vp_model, invmat, picks=None, spatfilter=0, stages=None, **kw):
     """A fresh directory with parameter, workflow, acquisition and model
-    files (and pick files <DATA_DIR>_<shot>.dat when ``picks`` is given)."""
+    files (and pick files <DATA_DIR>_<shot>.dat when ``picks`` is given).
+    ``stages``: workflow rows (dicts of read_par_inv.c's six columns), by
+    default one stage with PRO = 0 and the given SPATFILTER."""
     ws = tempfile.mkdtemp(prefix="rzp", dir="/tmp")
     for d in ("model", "picks", "jac", "log", "tt"):
         os.makedirs(os.path.join(ws, d))
     F.write_par(os.path.join(ws, "rz.inp"), par_values(sv.nx, sv.ny, sv.dh, invmat, **kw))
     F.write_workflow(os.path.join(ws, "wf.inp"),
-                     [dict(PRO=0.0, SPATFILTER=spatfilter, WD_DAMP=1e-3, WD_DAMP1=1e-3, FILT_SIZE_GRAD=8,
-                           FILT_SIZE_GRAD1=6)])
+                     stages or [dict(PRO=0.0, SPATFILTER=spatfilter, WD_DAMP=1e-3, WD_DAMP1=1e-3,
+                                     FILT_SIZE_GRAD=8, FILT_SIZE_GRAD1=6)])
     F.write_sources(os.path.join(ws, "src.dat"), sv.src_x_m, sv.src_z_m)
     F.write_receivers(os.path.join(ws, "rec.dat"), sv.rec_x_m, sv.rec_z_m)
     F.write_model(os.path.join(ws, "model", "start.vp"), vp_model)

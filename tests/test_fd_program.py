@@ -1,6 +1,7 @@
 """The reference program driving the MI355X path through the fd.h entry
 points (rajzel_amd/csrc/rz_fd_surface.c, which also replaces gauss_filt.c,
-taper_grad.c, grid_search.c and LBFGS.c): rajzel.c, fatt.c, PCG.c,
+taper_grad.c, grid_search.c, LBFGS.c, zero_LBFGS.c and zero_LBFGS1.c):
+rajzel.c, fatt.c, PCG.c,
 wolfels.c, parabolicls.c, precond.c, readmod.c and writemod.c run unchanged, and every
 file they write must equal, byte for byte, what the unchanged reference
 program writes from the same inputs.
@@ -110,9 +111,9 @@ def test_fatt_lbfgs_wolfe():
 @pytest.mark.gpu
 @pytest.mark.parametrize("kw", [dict(ITERMAX=5, NLBFGS=2), dict(ITERMAX=3, IDX=2, IDY=3)])
 def test_fatt_lbfgs_history(kw):
-    """The shim's LBFGS (LBFGS.c with bulk history files): slots that wrap
-    around (NLBFGS = 2 over 5 iterations) and strided history (IDX, IDY);
-    the history files, models and misfit log equal the reference's."""
+    """The shim's LBFGS: on the GPU with slots that wrap around (NLBFGS = 2
+    over 5 iterations), on the host with strided history (IDX, IDY); the
+    history files, models and misfit log equal the reference's."""
     sv = P.survey()
     obs = _observed(sv)
     out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, **kw))
@@ -120,6 +121,21 @@ def test_fatt_lbfgs_history(kw):
         assert f in out
     n = ((sv.nx + kw.get("IDX", 1) - 1) // kw.get("IDX", 1)) * ((sv.ny + kw.get("IDY", 1) - 1) // kw.get("IDY", 1))
     assert len(out["jac/grad_c.old"]) == 4 * n
+
+
+@pytest.mark.gpu
+def test_fatt_lbfgs_stages():
+    """Two workflow stages: the first ends after three iterations (PRO = 1
+    accepts any misfit change, fatt.c:219-251), which zeroes the l-BFGS
+    history (zero_LBFGS, fatt.c:232) and starts the second at iteration 1
+    with the pointer at slot 1; the GPU history must follow."""
+    sv = P.survey()
+    obs = _observed(sv)
+    row = dict(SPATFILTER=0, WD_DAMP=1e-3, WD_DAMP1=1e-3, FILT_SIZE_GRAD=8, FILT_SIZE_GRAD1=6)
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=4, NLBFGS=3,
+                                            stages=[dict(row, PRO=1.0), dict(row, PRO=0.0)]))
+    assert "model/inv_vp_stage_1.bin" in out
+    assert len([m for m in out["misfit.dat"].decode().split("\n") if m.strip()]) > 3   # the second stage ran
 
 
 @pytest.mark.gpu
