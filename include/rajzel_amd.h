@@ -373,6 +373,8 @@ int rz_sync(rz_ctx *ctx);
 
 /* Rows per lane in use and bytes of device memory held. */
 int rz_rows_per_lane(const rz_ctx *ctx);
+/* Compute units of the context's device (one shot per CU fills it). */
+int rz_compute_units(const rz_ctx *ctx);
 size_t rz_device_bytes(const rz_ctx *ctx);
 /* Shots per launch of rz_gradient (all shots, or fewer under a budget). */
 int rz_batch_shots(const rz_ctx *ctx);

@@ -1461,6 +1461,7 @@ int rz_get_profile(const rz_ctx *c, double *ms, long long *launches)
 }
 
 int rz_rows_per_lane(const rz_ctx *c) { return c ? c->R : 0; }
+int rz_compute_units(const rz_ctx *c) { return c ? c->ncu : 0; }
 
 int rz_set_split(rz_ctx *c, int on)
 {

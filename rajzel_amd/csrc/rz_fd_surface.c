@@ -4,14 +4,17 @@
  *
  * Linked into the reference program in place of eikonal.c, sweep.c,
  * eikonal_adj.c, sweep_adj.c, ass_grad.c, grad_obj.c, forward.c,
- * gauss_filt.c, taper_grad.c, grid_search.c, LBFGS.c, zero_LBFGS.c and
- * zero_LBFGS1.c, it keeps
+ * gauss_filt.c, taper_grad.c, grid_search.c, LBFGS.c, zero_LBFGS.c,
+ * zero_LBFGS1.c and calc_mat_change_wolfe.c, it keeps
  * their fd.h signatures, their globals and their side effects, so that
  * fatt.c, PCG.c, wolfels.c, parabolicls.c, step_length_est.c and rajzel.c
  * call it unchanged:
  *
  *   grad_obj    (fd.h:72, grad_obj.c:10-79)   every shot of this rank in one
- *               batched evaluation on the GPU; picks read from the
+ *               batched evaluation on the GPU, with the line search's
+ *               predicted next trial in the same launch where CUs are idle
+ *               (calc_mat_change_wolfe, fd.h:44, reports the trials); picks
+ *               read from the
  *               <DATA_DIR>_<shot>.dat files as calc_FA_res.c:28-37 does
  *               (each file parsed again only when it changed, rz_read_picks);
  *               MPI sum, frame copy and precond() as grad_obj.c:57-73
@@ -61,6 +64,9 @@ extern char JACOBIAN[], MFILE[];
 extern FILE *FP;
 extern float VP0_1, VP0_2, DVP0, GRAD0_1, GRAD0_2, DGRAD0;
 extern char SNAP_FILE[], DATA_DIR[];
+extern int LINESEARCH;
+extern float SCALEFAC, VPUPPERLIM, VPLOWERLIM;
+extern char INV_MODELFILE[];
 
 /* routines of the reference that stay in use (fd.h:52,119,143,166,170,180;
  * util.c's matrix helpers) */
@@ -94,6 +100,19 @@ static struct {
 
 static char g_msg[512];
 
+/* The next line-search trial, evaluated ahead (see grad_obj below): a
+ * two-model context of the same geometry (model 0 = the trial asked for,
+ * model 1 = the predicted next trial), the predicted model's slowness and
+ * the picks both used. */
+static struct {
+    rz_ctx *ctx;
+    int nmodels;                    /* 2 or 3: the trial and up to two predicted next ones */
+    float *S[2], *picks;            /* slowness of models 1 and 2 (host) */
+    size_t npicks;
+    int iter, nvalid;               /* models 1 .. nvalid hold results */
+    long calls, served, ahead;
+} g_spec;
+
 static void rzs_fail(const char *what)
 {
     snprintf(g_msg, sizeof g_msg, " rajzel_amd %s: %s", what, rz_last_error(g_rzs.ctx));
@@ -119,8 +138,20 @@ static int getenv_flag(const char *name, int dflt)
     return (v && *v) ? atoi(v) != 0 : dflt;
 }
 
+static void rzs_spec_release(void)
+{
+    rz_destroy(g_spec.ctx);
+    free(g_spec.S[0]);
+    free(g_spec.S[1]);
+    free(g_spec.picks);
+    g_spec.ctx = NULL;
+    g_spec.S[0] = g_spec.S[1] = g_spec.picks = NULL;
+    g_spec.nvalid = 0;
+}
+
 static void rzs_release(void)
 {
+    rzs_spec_release();
     rz_destroy(g_rzs.ctx);
     free(g_rzs.src_ix); free(g_rzs.src_iy); free(g_rzs.rec_ix); free(g_rzs.rec_iy); free(g_rzs.S_host);
     memset(&g_rzs, 0, sizeof g_rzs);
@@ -230,12 +261,12 @@ void eikonal_adj(float **TT, float *Tres, float **lam, int **recpos, int ntr, in
  * (taper_grad_shot.c, SRTRADIUS) or 2 (taper_grad_shot1.c, FILTSIZE) over the
  * nshots sources of srcpos; other values apply none, as in ass_grad.c.  The
  * library keeps its tables while the settings stay the same. */
-static int rzs_taper(rz_ctx *c, float **srcpos, int nshots)
+static int rzs_taper_at(rz_ctx *c, int model, float **srcpos, int nshots)
 {
     const int mode = SWS_TAPER_CIRCULAR_PER_SHOT == 1 || SWS_TAPER_CIRCULAR_PER_SHOT == 2
                          ? SWS_TAPER_CIRCULAR_PER_SHOT : 0;
     if (mode == 0 || nshots <= 0) {
-        rzs_check(rz_set_shot_taper(c, 0, 0, 0.f, 0, 0, NULL, NULL, 0), "per-shot taper");
+        rzs_check(rz_set_shot_taper(c, model, 0, 0.f, 0, 0, NULL, NULL, 0), "per-shot taper");
         return mode;
     }
     float *x = rzs_alloc((size_t)nshots * sizeof(float)), *z = rzs_alloc((size_t)nshots * sizeof(float));
@@ -243,10 +274,15 @@ static int rzs_taper(rz_ctx *c, float **srcpos, int nshots)
         x[n] = srcpos[1][n + 1];
         z[n] = srcpos[2][n + 1];
     }
-    rzs_check(rz_set_shot_taper(c, 0, mode, SRTRADIUS, FILTSIZE, nshots, x, z, NSHOT1 - 1), "per-shot taper");
+    rzs_check(rz_set_shot_taper(c, model, mode, SRTRADIUS, FILTSIZE, nshots, x, z, NSHOT1 - 1), "per-shot taper");
     free(x);
     free(z);
     return mode;
+}
+
+static int rzs_taper(rz_ctx *c, float **srcpos, int nshots)
+{
+    return rzs_taper_at(c, 0, srcpos, nshots);
 }
 
 void ass_grad(float **grad, float **grad_shot, float **S, float **lam, float **srcpos, int nshots, int **recpos,
@@ -263,12 +299,18 @@ void ass_grad(float **grad, float **grad_shot, float **S, float **lam, float **s
 
 /* The last shot's fields and receiver data, which the reference leaves in
  * the caller's arrays. */
-static void rzs_last_shot(int ns, int ntr, float **TT, float **lam, float *Tmod, float *tm_all)
+static void rzs_last_shot_of(rz_ctx *c, int model, int ns, int ntr, float **TT, float **lam, float *Tmod,
+                             float *tm_all)
 {
     const int last = ns - 1;
-    if (TT) rzs_check(rz_get_field(g_rzs.ctx, 0, last, 0, &TT[1][1], 0), "traveltime field");
-    if (lam) rzs_check(rz_get_field(g_rzs.ctx, 0, last, 1, &lam[1][1], 0), "adjoint field");
+    if (TT) rzs_check(rz_get_field(c, model, last, 0, &TT[1][1], 0), "traveltime field");
+    if (lam) rzs_check(rz_get_field(c, model, last, 1, &lam[1][1], 0), "adjoint field");
     if (Tmod && ntr > 0) memcpy(&Tmod[1], tm_all + (size_t)last * ntr, ntr * sizeof(float));
+}
+
+static void rzs_last_shot(int ns, int ntr, float **TT, float **lam, float *Tmod, float *tm_all)
+{
+    rzs_last_shot_of(g_rzs.ctx, 0, ns, ntr, TT, lam, Tmod, tm_all);
 }
 
 void forward(float **S, float **TT, float *Tmod, float **srcpos, int nshots, int **recpos, int ntr)
@@ -296,6 +338,151 @@ void forward(float **S, float **TT, float *Tmod, float **srcpos, int nshots, int
     free(tm);
 }
 
+/* ---------------------------------------------------------------------------
+ * Line-search trials evaluated ahead: the batched line search.
+ *
+ * wolfels.c:55-101 and parabolicls.c:56-61 evaluate trial models one after
+ * another, each grad_obj waiting for the decision on the previous one.  The
+ * next trial is one of two, both known before the decision: the parabolic
+ * search (LINESEARCH = 2, the stock one) continues with alpha +
+ * alpha/SCALEFAC when the trial lowered the misfit or has not yet bracketed
+ * the minimum, and with alpha/SCALEFAC when it did not lower it
+ * (parabolicls.c:64-96); the Wolfe search (LINESEARCH = 1) sets mu or nu to
+ * the trial's alpha and continues with SCALEFAC*alpha or (nu + mu)/SCALEFAC
+ * (wolfels.c:55-58,90-97), whose mu and nu the shim follows from the
+ * trials it sees.  When a trial's shots leave CUs idle even with each shot
+ * split over two CUs, grad_obj evaluates the trial and the predicted next
+ * trials that fit so (the likelier first) in one launch; when the search then asks for exactly one of those models
+ * (the same slowness bits, picks and iteration), grad_obj returns its
+ * results without evaluating again.  A wrong prediction only costs the idle
+ * CUs (with fewer free CUs a second model costs about what it saves, and
+ * the launch is left alone).  All models start from the caller's residual
+ * carry, so trials are
+ * evaluated ahead only when no pick is muted (the carry then does not enter
+ * any result).
+ *
+ * The trials come from calc_mat_change_wolfe (calc_mat_change_wolfe.c:10-58,
+ * fd.h:44), which the shim provides with the reference's operations, so
+ * that it knows the direction, the model and the step of each trial.
+ * RAJZEL_AMD_SPECULATE=0 turns this off, =2 evaluates ahead whatever the
+ * CU count; RAJZEL_AMD_SPECULATE_STATS=1 prints the count at exit.
+ * --------------------------------------------------------------------------- */
+static struct {
+    float **Hgrad, **Vp, **Vpnp1;     /* of the last trial (itest = 1) */
+    float alpha;
+    int fresh;                        /* a trial not yet evaluated */
+    int open, known;                  /* a search is open; mu and nu follow it */
+    float mu, nu, cand[2];            /* wolfels.c's mu, nu; the last trial's two successors */
+} g_trial;
+
+/* calc_mat_change_wolfe.c:29-47 on contiguous blocks: vp + eps*hg, kept
+ * where it leaves [VPLOWERLIM, VPUPPERLIM] or is negative */
+static void rzs_step_model(const float *vp, const float *hg, float eps, float *out, size_t n)
+{
+    for (size_t k = 0; k < n; ++k) {
+        float v = vp[k] + eps * hg[k];
+        if (v < VPLOWERLIM) v = vp[k];
+        if (v > VPUPPERLIM) v = vp[k];
+        if (v < 0.0) v = vp[k];
+        out[k] = v;
+    }
+}
+
+void calc_mat_change_wolfe(float **Hgrad, float **Vp, float **Vpnp1, float eps_scale, int itest)
+{
+    const size_t n = (size_t)NX * NY;
+    rzs_step_model(&Vp[1][1], &Hgrad[1][1], eps_scale, &Vpnp1[1][1], n);
+    if (itest == 0) {                                       /* the accepted step: fatt.c:210 */
+        char modfile[RZS_PATH];
+        memcpy(&Vp[1][1], &Vpnp1[1][1], n * sizeof(float));
+        snprintf(modfile, sizeof modfile, "%s_vp.bin", INV_MODELFILE);
+        writemod(modfile, Vpnp1, 3);
+        g_trial.fresh = 0;
+        g_trial.open = 0;
+        return;
+    }
+    if (!g_trial.open) {                                    /* a search's first trial: wolfels.c:44-45 */
+        g_trial.open = 1;
+        g_trial.known = 1;
+        g_trial.mu = 0.0f;
+        g_trial.nu = 1e30f;
+    } else if (g_trial.known) {                             /* which successor the search took */
+        if (!memcmp(&eps_scale, &g_trial.cand[0], sizeof(float))) g_trial.mu = g_trial.alpha;
+        else if (!memcmp(&eps_scale, &g_trial.cand[1], sizeof(float))) g_trial.nu = g_trial.alpha;
+        else g_trial.known = 0;
+    }
+    /* wolfels.c:55-58 after mu = alpha (too short) and after nu = alpha (too long) */
+    const float mu = g_trial.mu, nu = g_trial.nu;
+    g_trial.cand[0] = nu < 1e30 ? (nu + eps_scale) / SCALEFAC : SCALEFAC * eps_scale;
+    g_trial.cand[1] = (eps_scale + mu) / SCALEFAC;
+    g_trial.Hgrad = Hgrad;
+    g_trial.Vp = Vp;
+    g_trial.Vpnp1 = Vpnp1;
+    g_trial.alpha = eps_scale;
+    g_trial.fresh = 1;
+}
+
+static void rzs_spec_stats(void)
+{
+    fprintf(stderr, "rajzel_amd line search: %ld grad_obj calls, %ld evaluated with the next trial ahead, "
+                    "%ld served from it\n", g_spec.calls, g_spec.ahead, g_spec.served);
+}
+
+/* The current search's possible next trials, the likelier first; returns
+ * how many (0 if the search is not one the shim can follow). */
+static int rzs_next_alphas(float alpha[2])
+{
+    if (LINESEARCH == 2) {                                  /* parabolicls.c:76-95 */
+        alpha[0] = g_trial.alpha + (g_trial.alpha / SCALEFAC);
+        alpha[1] = g_trial.alpha / SCALEFAC;
+        return 2;
+    }
+    if (LINESEARCH == 1 && g_trial.known) {                 /* wolfels.c:55-58 */
+        alpha[0] = g_trial.cand[0];
+        alpha[1] = g_trial.cand[1];
+        return 2;
+    }
+    return 0;
+}
+
+/* S equals calc_S.c's 1.0/Vpnp1 of the last trial, bit for bit */
+static int rzs_is_trial(float **S)
+{
+    const size_t n = (size_t)NX * NY;
+    const float *v = &g_trial.Vpnp1[1][1], *s = &S[1][1];
+    for (size_t k = 0; k < n; ++k) {
+        const float q = (float)(1.0 / v[k]);
+        if (memcmp(&q, s + k, sizeof q) != 0) return 0;
+    }
+    return 1;
+}
+
+static int rzs_spec_mode(void)
+{
+    const char *v = getenv("RAJZEL_AMD_SPECULATE");
+    return (v && *v) ? atoi(v) : 1;
+}
+
+/* The results of (c, model) into the caller's arrays: gradient, this rank's
+ * misfit sum, the residual carry and the last shot's fields. */
+static double rzs_collect(rz_ctx *c, int model, int ns, int ntr, float **grad, float **TT, float **lam,
+                          float *Tmod, float *Tres)
+{
+    double half = 0.0;
+    rzs_check(rz_get_grad(c, model, &grad[1][1], 0), "gradient");
+    rzs_check(rz_get_misfit(c, model, &half, NULL), "misfit");
+    if (ntr > 0) {
+        float *tm = rzs_alloc((size_t)ns * ntr * sizeof(float));
+        rzs_check(rz_get_tmod(c, model, tm, 0), "receiver times");
+        rzs_check(rz_get_tres_carry(c, model, &Tres[1]), "residual carry");
+        rzs_last_shot_of(c, model, ns, ntr, TT, lam, Tmod, tm);
+        free(tm);
+    } else {
+        rzs_last_shot_of(c, model, ns, 0, TT, lam, NULL, NULL);
+    }
+    return 2.0 * half;                                      /* this rank's sum, grad_obj.c:42 */
+}
+
 double grad_obj(float **grad, float **S, float **TT, float **lam, float *Tmod, float *Tobs, float *Tres,
                 float **srcpos, int nshots, int **recpos, int ntr, int iter)
 {
@@ -303,39 +490,102 @@ double grad_obj(float **grad, float **S, float **TT, float **lam, float *Tmod, f
     const size_t ncell = (size_t)NX * NY;
     double L2 = 0.0, L2sum = 0.0;
     SEISMO = 0;                                             /* grad_obj.c:24 */
+    const int trial = g_trial.fresh;
+    g_trial.fresh = 0;
     if (ns > 0) {
         rz_ctx *c = rzs_context(srcpos, NSHOT1, NSHOT2, recpos, ntr);
-        rzs_slowness(S);
-        rzs_taper(c, srcpos, nshots);                       /* ass_grad.c:31-37 */
+        const size_t np = (size_t)ns * (ntr > 0 ? ntr : 0);
+        float *tobs = NULL;
+        int muted = 0;
         if (ntr > 0) {
             /* calc_FA_res.c:28-37: every evaluation reads every shot's pick
              * file with fscanf("%e") into Tobs, in shot order (a value that
              * does not parse keeps what Tobs held) */
-            float *tobs = rzs_alloc((size_t)ns * ntr * sizeof(float));
+            tobs = rzs_alloc(np * sizeof(float));
             char name[RZS_PATH];
             for (int s = 0; s < ns; ++s) {
                 snprintf(name, sizeof name, "%s_%i.dat", DATA_DIR, NSHOT1 + s);
                 if (rz_read_picks(name, Tobs, ntr) != 0) err(" picks_?.dat could not be opened !");
                 memcpy(tobs + (size_t)s * ntr, &Tobs[1], ntr * sizeof(float));
             }
-            rzs_check(rz_set_picks(c, 0, tobs, 0), "picks");
-            free(tobs);
-            rzs_check(rz_set_tres_carry(c, 0, &Tres[1]), "residual carry");
+            for (size_t k = 0; k < np && !muted; ++k) muted = !(tobs[k] >= 0.f);
         }
-        rzs_check(rz_gradient(c), "gradient");
-        rzs_check(rz_get_grad(c, 0, &grad[1][1], 0), "gradient");
-        double half = 0.0;
-        rzs_check(rz_get_misfit(c, 0, &half, NULL), "misfit");
-        L2 = 2.0 * half;                                    /* this rank's sum, grad_obj.c:42 */
-        if (ntr > 0) {
-            float *tm = rzs_alloc((size_t)ns * ntr * sizeof(float));
-            rzs_check(rz_get_tmod(c, 0, tm, 0), "receiver times");
-            rzs_check(rz_get_tres_carry(c, 0, &Tres[1]), "residual carry");
-            rzs_last_shot(ns, ntr, TT, lam, Tmod, tm);
-            free(tm);
+        const int mode = rzs_spec_mode();
+        const int cus = rz_compute_units(c);
+        /* predicted trials that fit beside this one with every shot still split over two CUs (a
+         * lone shot already uses a second CU, so only CUs beyond that are free) */
+        int fit = mode == 2 ? 2 : mode == 1 ? cus / (2 * ns) - 1 : 0;
+        fit = fit < 0 ? 0 : fit > 2 ? 2 : fit;
+        float alphas[2];
+        int served = -1;
+        if (g_spec.calls++ == 0 && MYID == 0 && getenv_flag("RAJZEL_AMD_SPECULATE_STATS", 0)) atexit(rzs_spec_stats);
+        if (mode && np > 0 && g_spec.nvalid > 0 && g_spec.iter == iter && g_spec.npicks == np &&
+            !memcmp(g_spec.picks, tobs, np * sizeof(float))) {
+            for (int m = 0; m < g_spec.nvalid && served < 0; ++m)
+                if (!memcmp(g_spec.S[m], &S[1][1], ncell * sizeof(float))) served = m + 1;
+        }
+        int na = 0;
+        if (served > 0) {
+            /* a trial evaluated ahead with the previous one */
+            g_spec.nvalid = 0;
+            ++g_spec.served;
+            L2 = rzs_collect(g_spec.ctx, served, ns, ntr, grad, TT, lam, Tmod, Tres);
+        } else if (mode && fit > 0 && trial && ntr > 0 && !muted && (na = rzs_next_alphas(alphas)) > 0 &&
+                   rzs_is_trial(S)) {
+            /* this trial and its predicted successors in one launch */
+            if (na > fit) na = fit;
+            if (g_spec.ctx == NULL) {
+                rz_model_desc d3[3] = {g_rzs.d, g_rzs.d, g_rzs.d};
+                const char *dev = getenv("RAJZEL_AMD_DEVICE");
+                g_spec.nmodels = 1 + fit;
+                if (rz_create(&g_spec.ctx, dev ? atoi(dev) : 0, g_spec.nmodels, d3, 0, 0) != RZ_OK) {
+                    snprintf(g_msg, sizeof g_msg, " rajzel_amd context: %s", rz_last_error(NULL));
+                    err(g_msg);
+                }
+                g_spec.S[0] = rzs_alloc(ncell * sizeof(float));
+                g_spec.S[1] = rzs_alloc(ncell * sizeof(float));
+            }
+            if (na > g_spec.nmodels - 1) na = g_spec.nmodels - 1;
+            rz_ctx *sc = g_spec.ctx;
+            float *v1 = rzs_alloc(ncell * sizeof(float));
+            for (int m = 0; m < g_spec.nmodels - 1; ++m) {
+                /* the successors' models (calc_mat_change_wolfe.c, calc_S.c:20-22); a
+                 * model without a successor repeats the trial's */
+                if (m < na) {
+                    rzs_step_model(&g_trial.Vp[1][1], &g_trial.Hgrad[1][1], alphas[m], v1, ncell);
+                    for (size_t k = 0; k < ncell; ++k) g_spec.S[m][k] = (float)(1.0 / v1[k]);
+                } else {
+                    memcpy(g_spec.S[m], &S[1][1], ncell * sizeof(float));
+                }
+            }
+            free(v1);
+            rzs_check(rz_set_slowness(sc, 0, &S[1][1], 0), "slowness");
+            for (int m = 1; m < g_spec.nmodels; ++m) rzs_check(rz_set_slowness(sc, m, g_spec.S[m - 1], 0), "slowness");
+            for (int m = 0; m < g_spec.nmodels; ++m) {
+                rzs_taper_at(sc, m, srcpos, nshots);        /* ass_grad.c:31-37 */
+                rzs_check(rz_set_picks(sc, m, tobs, 0), "picks");
+                rzs_check(rz_set_tres_carry(sc, m, &Tres[1]), "residual carry");
+            }
+            rzs_check(rz_gradient(sc), "gradient");
+            free(g_spec.picks);
+            g_spec.picks = rzs_alloc(np * sizeof(float));
+            memcpy(g_spec.picks, tobs, np * sizeof(float));
+            g_spec.npicks = np;
+            g_spec.iter = iter;
+            g_spec.nvalid = na;
+            ++g_spec.ahead;
+            L2 = rzs_collect(sc, 0, ns, ntr, grad, TT, lam, Tmod, Tres);
         } else {
-            rzs_last_shot(ns, 0, TT, lam, NULL, NULL);
+            rzs_slowness(S);
+            rzs_taper(c, srcpos, nshots);                   /* ass_grad.c:31-37 */
+            if (ntr > 0) {
+                rzs_check(rz_set_picks(c, 0, tobs, 0), "picks");
+                rzs_check(rz_set_tres_carry(c, 0, &Tres[1]), "residual carry");
+            }
+            rzs_check(rz_gradient(c), "gradient");
+            L2 = rzs_collect(c, 0, ns, ntr, grad, TT, lam, Tmod, Tres);
         }
+        free(tobs);
     } else {
         memset(&grad[1][1], 0, ncell * sizeof(float));     /* init_grad.c */
     }

@@ -28,7 +28,7 @@ EXPORTS = (
     "rz_read_picks", "rz_picks_parses", "rz_picks_clear", "rz_gauss_filt_noalloc",
     "rz_slot_bytes", "rz_get_stop_passes", "rz_set_shot_taper", "rz_ass_grad_shot_at", "rz_taper_grad",
     "rz_dotp_ordered", "rz_lbfgs_create", "rz_lbfgs_destroy", "rz_lbfgs_error", "rz_lbfgs_zero",
-    "rz_lbfgs_has_previous", "rz_lbfgs_set_previous", "rz_lbfgs_step",
+    "rz_lbfgs_has_previous", "rz_lbfgs_set_previous", "rz_lbfgs_step", "rz_compute_units",
 )
 
 
@@ -96,6 +96,7 @@ def lib():
                 "rz_set_shot_taper": (i, [vp, i, i, f, i, i, vp, vp, i]),
                 "rz_sync": (i, [vp]),
                 "rz_rows_per_lane": (i, [vp]),
+                "rz_compute_units": (i, [vp]),
                 "rz_device_bytes": (C.c_size_t, [vp]),
                 "rz_batch_shots": (i, [vp]),
                 "rz_slot_bytes": (C.c_size_t, [vp]),

@@ -184,6 +184,53 @@ def time_fatt_picks(itermax=2):
     return res
 
 
+def time_fatt_line_search(itermax=5, linesearch=2, workload="c2"):
+    """Wall time of the MI355X program's FATT inversion (INVMAT=1, L-BFGS)
+    on the 200 x 600 workload of 120 shots and 600 receivers ("c2"), or the
+    1001 x 3001 one with n of its shots and 1500 receivers ("c4_<n>", a
+    shard that leaves CUs idle) (its truth plus noise as pick files, no
+    pick muted) with the stock parabolic line
+    search (LINESEARCH = 2; 1 = Wolfe), once evaluating each line-search
+    trial alone (RAJZEL_AMD_SPECULATE=0) and once with its predicted
+    successors in the same launch (default).  Returns both times, the
+    program's grad_obj calls and how many were served from a trial
+    evaluated ahead, and whether every output file is byte-identical."""
+    import time
+    from rajzel_amd import native as N
+    sv = W.c2_gradient() if workload == "c2" else W.c4_gradient(nshots=int(workload.split("_")[1]))
+    with N.Context([dict(nx=sv.nx, ny=sv.ny, dh=sv.dh, src_ix=sv.src_ix, src_iy=sv.src_iy, rec_ix=sv.rec_ix,
+                         rec_iy=sv.rec_iy)]) as ctx:
+        def fwd(S, _sv):
+            ctx.set_slowness(0, S)
+            ctx.forward()
+            return ctx.tmod(0)
+        W.synthesize_picks(sv, fwd)
+    obs = F.roundtrip_picks(sv.tobs)
+    res = {"grid": f"{sv.ny}x{sv.nx}", "shots": sv.nshots, "receivers": sv.ntr, "itermax": itermax,
+           "linesearch": "parabolic" if linesearch == 2 else "wolfe"}
+    spaces, outs = [], []
+    try:
+        for key, flag in (("one_trial_per_launch_s", "0"), ("trials_ahead_s", "1")):
+            ws = make_workspace(sv, sv.vp, invmat=1, picks=obs, ITERMAX=itermax, SEISMO=0, LINESEARCH=linesearch,
+                                NLBFGS=20)
+            spaces.append(ws)
+            t0 = time.perf_counter()
+            r = run(ws, GPU_LIB, timeout=900, env={"RAJZEL_AMD_SPECULATE": flag, "RAJZEL_AMD_SPECULATE_STATS": "1"})
+            res[key] = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            line = [t for t in r.stderr.splitlines() if t.startswith("rajzel_amd line search:")][-1]
+            nums = [int(w) for w in line.replace(",", " ").split() if w.isdigit()]
+            res[key.replace("_s", "_calls")] = {"grad_obj": nums[0], "evaluated_ahead": nums[1], "served": nums[2]}
+            outs.append(outputs(ws))
+        res["outputs_identical"] = outs[0] == outs[1]
+        res["speedup"] = res["one_trial_per_launch_s"] / res["trials_ahead_s"]
+    finally:
+        for ws in spaces:
+            remove(ws)
+    return res
+
+
 if __name__ == "__main__":
     import json
     import sys
@@ -191,5 +238,11 @@ if __name__ == "__main__":
         print(json.dumps(time_grid_search()), flush=True)
     elif sys.argv[1:] == ["--time-fatt-picks"]:
         print(json.dumps(time_fatt_picks()), flush=True)
+    elif sys.argv[1:2] == ["--time-line-search"]:
+        ls = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+        wl = sys.argv[3] if len(sys.argv) > 3 else "c2"
+        print(json.dumps(time_fatt_line_search(itermax=5 if wl == "c2" else 6, linesearch=ls, workload=wl)),
+              flush=True)
     else:
-        sys.exit("usage: python -m tests.fd_program --time-grid-search | --time-fatt-picks")
+        sys.exit("usage: python -m tests.fd_program --time-grid-search | --time-fatt-picks | "
+                 "--time-line-search [2|1] [c2|c4_<shots>]")

@@ -1,7 +1,7 @@
 """The reference program driving the MI355X path through the fd.h entry
 points (rajzel_amd/csrc/rz_fd_surface.c, which also replaces gauss_filt.c,
-taper_grad.c, grid_search.c, LBFGS.c, zero_LBFGS.c and zero_LBFGS1.c):
-rajzel.c, fatt.c, PCG.c,
+taper_grad.c, grid_search.c, LBFGS.c, zero_LBFGS.c, zero_LBFGS1.c and
+calc_mat_change_wolfe.c): rajzel.c, fatt.c, PCG.c,
 wolfels.c, parabolicls.c, precond.c, readmod.c and writemod.c run unchanged, and every
 file they write must equal, byte for byte, what the unchanged reference
 program writes from the same inputs.
@@ -60,27 +60,32 @@ def test_reference_program_forward_equals_restatement(ora):
         P.remove(ws)
 
 
-def _observed(sv):
+def _observed(sv, muted=True):
     """Picks of the reference forward in the true model, N(0, 1 ms) noise,
-    one muted pick; as read back from pick files."""
+    one muted pick (unless muted=False); as read back from pick files."""
     from oracle.bindings import Restatement
     S = (1.0 / sv.vp_true.astype(np.float64)).astype(np.float32)
     tm, _, _ = Restatement().forward(S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.dh)
     obs = F.roundtrip_picks(tm + np.random.default_rng(3).normal(0, 1e-3, tm.shape))
-    obs[1, 4] = -1.0
+    if muted:
+        obs[1, 4] = -1.0
     return obs
 
 
-def _compare(make):
-    """Run the unchanged and the MI355X program on identical workspaces."""
+def _compare(make, env=None, runs=None):
+    """Run the unchanged and the MI355X program on identical workspaces
+    (env: extra environment of the MI355X run; runs: a list that receives
+    its CompletedProcess)."""
     _need(gpu=True)
     ws_c, ws_g = make(), make()
     try:
         print(f"[case] reference program in {ws_c}, MI355X program in {ws_g}", flush=True)
         rc = P.run(ws_c, P.CPU_LIB)
         assert rc.returncode == 0, rc.stdout[-2000:] + rc.stderr[-2000:]
-        rg = P.run(ws_g, P.GPU_LIB)
+        rg = P.run(ws_g, P.GPU_LIB, env=env)
         assert rg.returncode == 0, rg.stdout[-2000:] + rg.stderr[-2000:]
+        if runs is not None:
+            runs.append(rg)
         oc, og = P.outputs(ws_c), P.outputs(ws_g)
         assert sorted(oc) == sorted(og)
         differ = [k for k in oc if oc[k] != og[k]]
@@ -136,6 +141,28 @@ def test_fatt_lbfgs_stages():
                                             stages=[dict(row, PRO=1.0), dict(row, PRO=0.0)]))
     assert "model/inv_vp_stage_1.bin" in out
     assert len([m for m in out["misfit.dat"].decode().split("\n") if m.strip()]) > 3   # the second stage ran
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ls,method,eps", [(2, 2, 0.01), (1, 2, 1e-5), (2, 1, 0.01)],
+                         ids=["parabolic-lbfgs", "wolfe-lbfgs", "parabolic-pcg"])
+def test_fatt_line_search_trials_ahead(ls, method, eps):
+    """The batched line search: with no muted pick and idle CUs, grad_obj
+    evaluates each line-search trial together with the predicted next one
+    and serves the next call from it when the search asks for that model.
+    Every file equals the reference program's, and calls are served (the
+    Wolfe case starts from a step too short, so that its search expands)."""
+    sv = P.survey()
+    obs = _observed(sv, muted=False)
+    runs = []
+    _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=4, LINESEARCH=ls,
+                                      GRAD_METHOD=method, EPS_SCALE=eps),
+             env={"RAJZEL_AMD_SPECULATE_STATS": "1"}, runs=runs)
+    line = [t for t in runs[0].stderr.splitlines() if t.startswith("rajzel_amd line search:")]
+    assert line, runs[0].stderr[-2000:]
+    served = int(line[-1].split(",")[-1].split()[0])
+    print(f"[result] {line[-1]}", flush=True)
+    assert served > 0
 
 
 @pytest.mark.gpu
