@@ -372,8 +372,11 @@ def main():
     # outer-iteration counts that schedule the split shots of later ones (in
     # an inversion every evaluation after the first has them).
     ctx.gradient()
+    # shots per CU: how many of the slowest shots run split in the mixed
+    # launches, timed on this evaluation (every candidate gives the same bits)
+    split_shots, tune_ms = (-1, None) if args.no_split else ctx.tune_split()
     log(f"[bench] setup done, R={ctx.rows_per_lane}, {ctx.device_bytes / 2**30:.2f} GiB on device, "
-        f"split {ctx.split_state()}")
+        f"split {ctx.split_state()}, split shots {split_shots} ({tune_ms} ms)")
 
     for _ in range(args.warmup):
         ctx.gradient()
@@ -414,7 +417,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload,
                    "nx": sv.nx, "ny": sv.ny, "shots": sv.nshots, "receivers": sv.ntr,
-                   "rows_per_lane": ctx.rows_per_lane},
+                   "rows_per_lane": ctx.rows_per_lane, "split_shots": split_shots},
         "roofline": roofline,
         "evaluation_hbm_frac": eval_gbs / HBM_PEAK_GBS,
         "phase_ms": phase_ms,

@@ -316,6 +316,20 @@ int rz_gauss_filt_noalloc(int device, void *hip_stream, float *field, int nx, in
 int rz_set_split(rz_ctx *ctx, int on);
 int rz_get_split(const rz_ctx *ctx, int *active, int *failure);
 
+/* Shots per CU.  With more shots than half the CUs, a mixed launch runs the
+ * n shots that took the most outer iterations last time split (two CUs
+ * each) and the others with one workgroup: n = -1 (default) uses every CU
+ * the others leave, 0 splits none; with at most one shot per two CUs, 0
+ * runs no shot split and any other value all of them.  rz_tune_split times
+ * the context's own evaluation (slowness and picks set) with each candidate
+ * (n in {0, F/4, F/2, 3F/4, -1}, F = CUs - shots; or split or not) and
+ * keeps the fastest, returning it and its time in ms.  Results are the same
+ * bits for every setting.  No counterpart in the reference (its shots run
+ * one after another per rank). */
+int rz_set_split_shots(rz_ctx *ctx, int n);
+int rz_get_split_shots(const rz_ctx *ctx);
+int rz_tune_split(rz_ctx *ctx, int *split_shots, double *ms);
+
 /* Pick file of one shot into Tobs[1..ntr] (1-based, as the reference's
  * vectors): calc_FA_res.c:28-37, one fscanf("%e") per receiver, where a
  * value that does not parse leaves Tobs[i] as it was.  The reference reads

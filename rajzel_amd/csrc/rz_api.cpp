@@ -82,6 +82,7 @@ struct rz_ctx {
     bool split = true;
     int ncu = 0;
     int split_fail = 0;                 // 1: workgroups of a shot on different XCDs, 2: a wait timed out
+    int split_shots = -1;               // split shots per mixed launch: -1 all the CUs allow, 0 none
     SplitSync ss{nullptr, nullptr, nullptr};
     // Mixed launches: with more shots than half the CUs (and fewer than
     // the CUs), the shots that took the most outer iterations in the
@@ -516,11 +517,11 @@ int plan_mixed(rz_ctx *c, int per)
 {
     const int T = c->total_shots;
     c->plan_on = false;
-    if (!c->split || per < T || T <= 0 || 2 * T <= c->ncu || T >= c->ncu) return RZ_OK;
+    if (!c->split || c->split_shots == 0 || per < T || T <= 0 || 2 * T <= c->ncu || T >= c->ncu) return RZ_OK;
     if ((int)c->hist_f.size() != T || (int)c->hist_a.size() != T) return RZ_OK;
     for (int i = 0; i < T; ++i)
         if (c->hist_f[i] < 0 || c->hist_a[i] < 0) return RZ_OK;
-    const int ns = c->ncu - T;
+    const int ns = c->split_shots < 0 ? c->ncu - T : std::min(c->split_shots, c->ncu - T);
     std::vector<int> ids;
     ids.reserve(4 * (size_t)T);
     for (int ph = 0; ph < 2; ++ph) {
@@ -581,7 +582,10 @@ int plan_mixed(rz_ctx *c, int per)
 // its own: at most one shot per two CUs.  With more shots the workgroups
 // would share CUs (two fit one), which measured slower than one workgroup
 // per shot.
-bool use_split(const rz_ctx *c, int nshots) { return c->split && nshots > 0 && 2 * nshots <= c->ncu; }
+bool use_split(const rz_ctx *c, int nshots)
+{
+    return c->split && c->split_shots != 0 && nshots > 0 && 2 * nshots <= c->ncu;
+}
 
 // Enqueue the launches of one batch of shots [b0, b1) on the work stream:
 // forward (unless its fields are reused), and with grad the residuals, the
@@ -1482,6 +1486,62 @@ int rz_get_split(const rz_ctx *c, int *active, int *failure)
     if (!c) return RZ_ERR_INVALID;
     if (active) *active = c->split ? 1 : 0;
     if (failure) *failure = c->split_fail;
+    return RZ_OK;
+}
+
+int rz_set_split_shots(rz_ctx *c, int n)
+{
+    if (!c || n < -1) return RZ_ERR_INVALID;
+    if (n != c->split_shots) drop_graphs(c);
+    c->split_shots = n;
+    return RZ_OK;
+}
+
+int rz_get_split_shots(const rz_ctx *c) { return c ? c->split_shots : 0; }
+
+int rz_tune_split(rz_ctx *c, int *split_shots, double *ms)
+{
+    if (!c) return RZ_ERR_INVALID;
+    if (const int rc = ready(c, true)) return rc;
+    const int T = c->total_shots, F = c->ncu - T;
+    std::vector<int> cand;
+    if (T > 0 && 2 * T <= c->ncu) cand = {-1, 0};                          // every shot split, or none
+    else if (T > 0 && F > 0) cand = {0, F / 4, F / 2, (3 * F) / 4, -1};   // how many of the slowest run split
+    const int keep = c->split_shots;
+    int best = keep;
+    double best_ms = -1.0;
+    HIPCHK(c, hipSetDevice(c->device));
+    auto run = [&](double *el) {
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = rz_gradient(c);
+        if (rc == RZ_OK || rc == RZ_ERR_NOT_CONVERGED) rc = rz_sync(c);
+        *el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return rc;
+    };
+    double el = 0.0;
+    if (cand.empty()) {                          // nothing to choose: the time of the setting in use
+        const int rc = run(&el);
+        if (rc != RZ_OK && rc != RZ_ERR_NOT_CONVERGED) return rc;
+        best_ms = el;
+    }
+    for (int n : cand) {
+        rz_set_split_shots(c, n);
+        // the first run plans from the previous run's iteration counts; time the second
+        for (int rep = 0; rep < 2; ++rep) {
+            const int rc = run(&el);
+            if (rc != RZ_OK && rc != RZ_ERR_NOT_CONVERGED) {
+                rz_set_split_shots(c, keep);
+                return rc;
+            }
+        }
+        if (best_ms < 0.0 || el < best_ms) {
+            best = n;
+            best_ms = el;
+        }
+    }
+    rz_set_split_shots(c, best);
+    if (split_shots) *split_shots = best;
+    if (ms) *ms = best_ms;
     return RZ_OK;
 }
 

@@ -29,6 +29,7 @@ EXPORTS = (
     "rz_slot_bytes", "rz_get_stop_passes", "rz_set_shot_taper", "rz_ass_grad_shot_at", "rz_taper_grad",
     "rz_dotp_ordered", "rz_lbfgs_create", "rz_lbfgs_destroy", "rz_lbfgs_error", "rz_lbfgs_zero",
     "rz_lbfgs_has_previous", "rz_lbfgs_set_previous", "rz_lbfgs_step", "rz_compute_units",
+    "rz_set_split_shots", "rz_get_split_shots", "rz_tune_split",
 )
 
 
@@ -97,6 +98,9 @@ def lib():
                 "rz_sync": (i, [vp]),
                 "rz_rows_per_lane": (i, [vp]),
                 "rz_compute_units": (i, [vp]),
+                "rz_set_split_shots": (i, [vp, i]),
+                "rz_get_split_shots": (i, [vp]),
+                "rz_tune_split": (i, [vp, C.POINTER(i), C.POINTER(d)]),
                 "rz_device_bytes": (C.c_size_t, [vp]),
                 "rz_batch_shots": (i, [vp]),
                 "rz_slot_bytes": (C.c_size_t, [vp]),
@@ -562,6 +566,23 @@ class Context:
     def set_split(self, on=True):
         """Two workgroups per shot (rz_set_split; on=2 forces the fallback)."""
         self._chk(lib().rz_set_split(self.h, int(on)))
+
+    def set_split_shots(self, n):
+        """Split shots per mixed launch (-1 = every CU the others leave, 0 = none)."""
+        self._chk(lib().rz_set_split_shots(self.h, n))
+
+    def split_shots(self):
+        return lib().rz_get_split_shots(self.h)
+
+    def tune_split(self):
+        """Time this context's evaluation with each split-shot candidate and
+        keep the fastest; returns (split_shots, ms)."""
+        n, ms = C.c_int(), C.c_double()
+        self._chk(lib().rz_tune_split(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def compute_units(self):
+        return lib().rz_compute_units(self.h)
 
     def split_state(self):
         """(active, failure) of rz_get_split."""

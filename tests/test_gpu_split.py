@@ -114,3 +114,32 @@ def test_mixed_launch_replayed_as_graph():
             ctx.sync()
             assert bits_equal(ctx.grad(0), want), f"replay {rep}: {ndiff(ctx.grad(0), want)} differ"
         assert ctx.split_state() == (1, 0)
+
+
+def test_split_shot_counts_and_tuner_keep_results():
+    """Shots per CU (rz_set_split_shots, rz_tune_split): with more shots than
+    half the CUs, mixed launches running 0, some or all free CUs' worth of
+    shots split give the same bits, and the tuner keeps one of its
+    candidates with the results unchanged."""
+    from rajzel_amd import native as N
+    sv = W.small_case(nx=70, ny=60, nshots=200, seed=17, rec_row=2)
+    sv.tobs = np.full((sv.nshots, sv.ntr), 0.4, np.float32)
+    print("[case] split shot counts 60x70, 200 shots", flush=True)
+    outs = {}
+    with N.Context([model_dict(sv, max_iter=200)], rows_per_lane=1) as ctx:
+        ctx.set_slowness(0, sv.S)
+        ctx.set_picks(0, sv.tobs)
+        F = ctx.compute_units() - sv.nshots
+        ctx.gradient()                     # iteration counts that plan the mixed launches
+        for n in (-1, 0, max(F // 2, 1)):
+            ctx.set_split_shots(n)
+            ctx.gradient()
+            outs[n] = (ctx.grad(0), ctx.tmod(0), ctx.misfit(0)[0])
+        n, ms = ctx.tune_split()
+        assert n in (-1, 0, F // 4, F // 2, (3 * F) // 4) and ms > 0
+        assert ctx.split_shots() == n
+        ctx.gradient()
+        outs["tuned"] = (ctx.grad(0), ctx.tmod(0), ctx.misfit(0)[0])
+    ref = outs[-1]
+    for k, v in outs.items():
+        assert bits_equal(v[0], ref[0]) and bits_equal(v[1], ref[1]) and v[2] == ref[2], k
