@@ -231,6 +231,50 @@ def time_fatt_line_search(itermax=5, linesearch=2, workload="c2"):
     return res
 
 
+def time_fatt_c4(itermax=20):
+    """The 1001 x 3001 configuration's inversion: 200 shots, 1500 receivers,
+    l-BFGS (NLBFGS = 20) with the Wolfe line search for `itermax`
+    iterations, on the MI355X program; once with the l-BFGS recursion on the
+    GPU (default) and once on the host (RAJZEL_AMD_LBFGS=0).  Picks: the
+    forward in the true model plus N(0, 1 ms).  Returns the wall times, the
+    grad_obj calls, the first and last misfit of the log and whether the two
+    runs wrote the same files."""
+    import time
+    from rajzel_amd import native as N
+    sv = W.c4_gradient()
+    with N.Context([dict(nx=sv.nx, ny=sv.ny, dh=sv.dh, src_ix=sv.src_ix, src_iy=sv.src_iy, rec_ix=sv.rec_ix,
+                         rec_iy=sv.rec_iy)]) as ctx:
+        ctx.set_slowness(0, W.slowness(sv.vp_true))
+        ctx.forward()
+        tm = ctx.tmod(0)
+    obs = F.roundtrip_picks(tm + np.random.default_rng(sv.meta["seed"] + 1000).normal(0.0, sv.noise_s, tm.shape))
+    res = {"grid": f"{sv.ny}x{sv.nx}", "shots": sv.nshots, "receivers": sv.ntr, "itermax": itermax,
+           "method": "l-BFGS, NLBFGS = 20, Wolfe line search"}
+    spaces, outs = [], []
+    try:
+        for key, flag in (("gpu_lbfgs_s", "1"), ("host_lbfgs_s", "0")):
+            ws = make_workspace(sv, sv.vp, invmat=1, picks=obs, ITERMAX=itermax, SEISMO=0, LINESEARCH=1,
+                                NLBFGS=20, GRAD_METHOD=2)
+            spaces.append(ws)
+            t0 = time.perf_counter()
+            r = run(ws, GPU_LIB, timeout=1100, env={"RAJZEL_AMD_LBFGS": flag, "RAJZEL_AMD_SPECULATE_STATS": "1"})
+            res[key] = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            line = [t for t in r.stderr.splitlines() if t.startswith("rajzel_amd line search:")][-1]
+            res.setdefault("grad_obj_calls", int(line.split()[3]))
+            outs.append(outputs(ws))
+            print(f"[step] {key} {res[key]:.1f} s", flush=True)
+        misfit = [l.split() for l in outs[0]["misfit.dat"].decode().splitlines() if l.strip()]
+        res["misfit_first"], res["misfit_last"] = float(misfit[0][1]), float(misfit[-1][1])
+        res["iterations_logged"] = len(misfit)
+        res["outputs_identical"] = outs[0] == outs[1]
+    finally:
+        for ws in spaces:
+            remove(ws)
+    return res
+
+
 if __name__ == "__main__":
     import json
     import sys
@@ -238,6 +282,8 @@ if __name__ == "__main__":
         print(json.dumps(time_grid_search()), flush=True)
     elif sys.argv[1:] == ["--time-fatt-picks"]:
         print(json.dumps(time_fatt_picks()), flush=True)
+    elif sys.argv[1:] == ["--time-fatt-c4"]:
+        print(json.dumps(time_fatt_c4()), flush=True)
     elif sys.argv[1:2] == ["--time-line-search"]:
         ls = int(sys.argv[2]) if len(sys.argv) > 2 else 2
         wl = sys.argv[3] if len(sys.argv) > 3 else "c2"
@@ -245,4 +291,4 @@ if __name__ == "__main__":
               flush=True)
     else:
         sys.exit("usage: python -m tests.fd_program --time-grid-search | --time-fatt-picks | "
-                 "--time-line-search [2|1] [c2|c4_<shots>]")
+                 "--time-line-search [2|1] [c2|c4_<shots>] | --time-fatt-c4")
