@@ -66,12 +66,18 @@ def make_workspace(sv, vp_model, invmat, picks=None, spatfilter=0, stages=None, 
     return ws
 
 
-def run(ws, lib, timeout=300, env=None):
-    """Run the program in ``ws`` (``env``: extra environment variables);
-    returns the CompletedProcess."""
+MPIEXEC = next((p for p in ("/opt/conda/bin/mpiexec", "/usr/bin/mpiexec") if os.path.exists(p)), None)
+
+
+def run(ws, lib, timeout=300, env=None, ranks=1):
+    """Run the program in ``ws`` (``env``: extra environment variables) as
+    one process, or as ``ranks`` MPI ranks under mpiexec; returns the
+    CompletedProcess."""
     env = dict(os.environ, **(env or {}))
-    return subprocess.run([LAUNCHER, lib, "rz.inp", "wf.inp"], cwd=ws, env=env, capture_output=True,
-                          text=True, timeout=timeout)
+    cmd = [LAUNCHER, lib, "rz.inp", "wf.inp"]
+    if ranks > 1:
+        cmd = [MPIEXEC, "-n", str(ranks)] + cmd
+    return subprocess.run(cmd, cwd=ws, env=env, capture_output=True, text=True, timeout=timeout)
 
 
 def outputs(ws):

@@ -72,17 +72,17 @@ def _observed(sv, muted=True):
     return obs
 
 
-def _compare(make, env=None, runs=None):
+def _compare(make, env=None, runs=None, ranks=1):
     """Run the unchanged and the MI355X program on identical workspaces
     (env: extra environment of the MI355X run; runs: a list that receives
-    its CompletedProcess)."""
+    its CompletedProcess; ranks: MPI ranks of both runs)."""
     _need(gpu=True)
     ws_c, ws_g = make(), make()
     try:
-        print(f"[case] reference program in {ws_c}, MI355X program in {ws_g}", flush=True)
-        rc = P.run(ws_c, P.CPU_LIB)
+        print(f"[case] reference program in {ws_c}, MI355X program in {ws_g}, {ranks} rank(s)", flush=True)
+        rc = P.run(ws_c, P.CPU_LIB, ranks=ranks)
         assert rc.returncode == 0, rc.stdout[-2000:] + rc.stderr[-2000:]
-        rg = P.run(ws_g, P.GPU_LIB, env=env)
+        rg = P.run(ws_g, P.GPU_LIB, env=env, ranks=ranks)
         assert rg.returncode == 0, rg.stdout[-2000:] + rg.stderr[-2000:]
         if runs is not None:
             runs.append(rg)
@@ -163,6 +163,26 @@ def test_fatt_line_search_trials_ahead(ls, method, eps):
     served = int(line[-1].split(",")[-1].split()[0])
     print(f"[result] {line[-1]}", flush=True)
     assert served > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ranks", [2, 3])
+def test_fatt_mpi_ranks(ranks):
+    """The program as 2 and 3 MPI ranks on the one GPU, each evaluating its
+    shot range [NSHOT1, NSHOT2) of init_MPIshot.c (5 shots: uneven ranges
+    with 3 ranks) and summing gradients and misfits through MPI
+    (sum_grad_MPI.c, grad_obj.c:58-61), l-BFGS on rank 0 and its direction
+    broadcast (exchange_grad_MPI.c): every file equals the reference
+    program's with the same ranks.  A forward run writes each rank's picks."""
+    if P.MPIEXEC is None:
+        pytest.skip("no mpiexec")
+    sv = P.survey()
+    obs = _observed(sv)
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3),
+                   ranks=ranks)
+    assert len([m for m in out["misfit.dat"].decode().split("\n") if m.strip()]) == 3
+    out = _compare(lambda: P.make_workspace(sv, sv.vp_true, invmat=0), ranks=ranks)
+    assert sum(k.startswith("picks/mod_") for k in out) == sv.nshots
 
 
 @pytest.mark.gpu
