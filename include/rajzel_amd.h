@@ -303,6 +303,24 @@ int rz_taper_grad(int device, void *hip_stream, float *field, int nx, int ny, co
 #define RZ_GAUSS_NOALLOC_WEIGHTS (16u << 10)
 int rz_gauss_filt_noalloc(int device, void *hip_stream, float *field, int nx, int ny, int *fsx, int *fsy);
 
+/* smooth_grad.c:10-99 (fd.h:131; precond.c:30-33, SPATFILTER = 3): each
+ * cell becomes the median (median2d.c:12-40) of the *fsize x *fsize window
+ * around it, the grid's edges replicated, bit for bit.  *fsize is
+ * FILT_SIZE_GRAD: 0 leaves the field, an even size is moved one away from 0
+ * as the reference does (the new value is returned); |*fsize| <= 129.
+ * field: NY x NX row-major, on the host or (on_device = 1) on the device.
+ * No context. */
+int rz_median_filt(int device, void *hip_stream, float *field, int nx, int ny, int *fsize, int on_device);
+
+/* smooth2.c:9-201 (fd.h:133; precond.c:25-28, SPATFILTER = 2): the
+ * tridiagonal smoothing with WD_DAMP (along z) and WD_DAMP1 (along x), bit
+ * for bit, on the gradient in the order smooth2.c writes and reads it
+ * (x slow, z fast: in[(i-1)*n1 + (j-1)] = v[i][j], i <= n2 = NX,
+ * j <= n1 = NY); out receives v[1..n2][1..n1] in the same order (the
+ * contents of <JACOBIAN>_smooth2.old).  Host arrays; no context. */
+int rz_smooth2(int device, void *hip_stream, const float *in, float *out, int n2, int n1, float wd_damp,
+               float wd_damp1);
+
 /* Split shots (on by default; RAJZEL_AMD_SPLIT=0 in the environment turns
  * the default off): a launch with at most one shot per two CUs runs each shot's
  * two paired sweeps in two workgroups on two CUs of one XCD, coupled

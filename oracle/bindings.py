@@ -266,6 +266,24 @@ class Reference:
         L.rzref_lbfgs.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f32, _f32, _f32,
                                   _f32, _f32, _f32, _f32, _f32, _f32, _f32, _f32]
         L.rzref_lbfgs.restype = None
+        L.rzref_smooth_grad.argtypes = [_f32, C.POINTER(C.c_int)]
+        L.rzref_smooth_grad.restype = None
+        L.rzref_smooth2.argtypes = [_f32, C.c_float, C.c_float, C.c_char_p]
+        L.rzref_smooth2.restype = None
+
+    def smooth_grad(self, A, fsize):
+        """The reference's median filter (SPATFILTER = 3); returns (field,
+        FILT_SIZE_GRAD as it leaves it)."""
+        a = self._f(A).copy()
+        fs = C.c_int(int(fsize))
+        self.lib.rzref_smooth_grad(a, C.byref(fs))
+        return a, fs.value
+
+    def smooth2(self, A, wd_damp, wd_damp1, jac):
+        """The reference's smooth2 (SPATFILTER = 2) with its files under jac."""
+        a = self._f(A).copy()
+        self.lib.rzref_smooth2(a, wd_damp, wd_damp1, os.fsencode(jac))
+        return a
 
     def lbfgs_state(self, nlbfgs):
         """Zeroed history and work vectors for lbfgs(), 1-based as fatt.c

@@ -314,3 +314,34 @@ void rzref_lbfgs(const char *jac, int idx, int idy, int iter, int pointer, int n
     free_matrix(V, 1, NY, 1, NX);
     free_matrix(H, 1, NY, 1, NX);
 }
+
+/* The reference's smooth_grad() (SPATFILTER = 3) on a flat NY x NX field;
+ * the size goes in through FILT_SIZE_GRAD and comes back as it leaves it. */
+void rzref_smooth_grad(float *field, int *fsize)
+{
+    float **m = mat_from(field);
+    FILT_SIZE_GRAD = *fsize;
+    smooth_grad(m);
+    *fsize = FILT_SIZE_GRAD;
+    mat_to(m, field);
+    free_matrix(m, 1, NY, 1, NX);
+}
+
+/* The reference's smooth2() (SPATFILTER = 2) on a flat NY x NX field, one
+ * rank (NXG = NX, NYG = NY, POS = 0, IDX = IDY = 1), its two files under
+ * JACOBIAN = jac. */
+void rzref_smooth2(float *field, float wd_damp, float wd_damp1, const char *jac)
+{
+    float **m = mat_from(field);
+    NXG = NX;
+    NYG = NY;
+    POS[1] = POS[2] = 0;
+    IDX = IDY = 1;
+    MYID = 0;
+    WD_DAMP = wd_damp;
+    WD_DAMP1 = wd_damp1;
+    snprintf(JACOBIAN, STRING_SIZE, "%s", jac);
+    smooth2(m);
+    mat_to(m, field);
+    free_matrix(m, 1, NY, 1, NX);
+}

@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "librajzel_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RZ_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["rz_kernels.hip", "rz_api.cpp", "rz_precond.hip", "rz_picks.cpp", "rz_taper.cpp", "rz_lbfgs.hip"]
+SOURCES = ["rz_kernels.hip", "rz_api.cpp", "rz_precond.hip", "rz_picks.cpp", "rz_taper.cpp", "rz_lbfgs.hip",
+           "rz_smooth.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",

@@ -4,8 +4,8 @@
  *
  * Linked into the reference program in place of eikonal.c, sweep.c,
  * eikonal_adj.c, sweep_adj.c, ass_grad.c, grad_obj.c, forward.c,
- * gauss_filt.c, taper_grad.c, grid_search.c, LBFGS.c, zero_LBFGS.c,
- * zero_LBFGS1.c and calc_mat_change_wolfe.c, it keeps
+ * gauss_filt.c, smooth2.c, smooth_grad.c, taper_grad.c, grid_search.c,
+ * LBFGS.c, zero_LBFGS.c, zero_LBFGS1.c and calc_mat_change_wolfe.c, it keeps
  * their fd.h signatures, their globals and their side effects, so that
  * fatt.c, PCG.c, wolfels.c, parabolicls.c, step_length_est.c and rajzel.c
  * call it unchanged:
@@ -31,6 +31,8 @@
  *               gradient (precond.c:35-37, SPATFILTER = 4) and of the
  *               search direction (fatt.c:179), on the GPU; linked in
  *               place of gauss_filt.c
+ *   smooth2, smooth_grad (fd.h:133,131)  precond.c's SPATFILTER 2 and 3
+ *               smoothings on the GPU, same files and messages
  *   taper_grad  (fd.h:149, taper_grad.c:11-299)  the gradient tapers of
  *               precond.c:43-53, multiplied on the GPU; log lines, taper.bin
  *               and taper_coeff_*.bin files as the reference's
@@ -596,6 +598,80 @@ double grad_obj(float **grad, float **S, float **TT, float **lam, float *Tmod, f
     cp_grad_frame(grad);
     precond(grad, nshots, srcpos, recpos, ntr, iter);
     return L2;
+}
+
+static size_t rzs_strided(void);
+static void rzs_gather(float **A, float *buf);
+
+/* smooth_grad.c:10-99 (fd.h:131; precond.c:30-33, SPATFILTER = 3): the
+ * median filter over FILT_SIZE_GRAD x FILT_SIZE_GRAD windows on the GPU
+ * (rz_median_filt), FILT_SIZE_GRAD made odd and its half printed as there. */
+void smooth_grad(float **waveconv)
+{
+    if (FILT_SIZE_GRAD == 0) return;
+    if (!(FILT_SIZE_GRAD % 2)) FILT_SIZE_GRAD += FILT_SIZE_GRAD > 0 ? 1 : -1;
+    printf("\n hfs: %d \n", abs(FILT_SIZE_GRAD) / 2);
+    int fs = FILT_SIZE_GRAD;
+    const char *dev = getenv("RAJZEL_AMD_DEVICE");
+    if (rz_median_filt(dev ? atoi(dev) : 0, NULL, &waveconv[1][1], NX, NY, &fs, 0) != RZ_OK) {
+        snprintf(g_msg, sizeof g_msg, " rajzel_amd smooth_grad: %s", rz_last_error(NULL));
+        err(g_msg);
+    }
+}
+
+/* smooth2.c:9-201 (fd.h:133; precond.c:25-28, SPATFILTER = 2): the same
+ * files and messages (<JACOBIAN>_wavenumber.old written by every rank with
+ * strides IDX/IDY, read back and smoothed on rank 0, <JACOBIAN>_smooth2.old
+ * written there and read back by every rank), each file in one call, the
+ * tridiagonal smoothing on the GPU (rz_smooth2). */
+void smooth2(float **grad)
+{
+    extern int NXG, NYG, POS[];
+    extern float WD_DAMP, WD_DAMP1;
+    char jac[RZS_PATH + 32];
+    const size_t ns = rzs_strided(), n = (size_t)NXG * NYG;
+    float *buf = rzs_alloc((ns > n ? ns : n) * sizeof(float));
+    rzs_gather(grad, buf);                                  /* smooth2.c:55-65 */
+    snprintf(jac, sizeof jac, "%s_wavenumber.old", JACOBIAN);
+    FILE *fp = fopen(jac, "wb");
+    if (fp) {
+        fwrite(buf, sizeof(float), ns, fp);
+        fclose(fp);
+    }
+    if (MYID == 0) {
+        printf("\n Smooth2 is applied to gradient (written by PE %d)\n", MYID);
+        fp = fopen(jac, "rb");
+        if (fp == NULL) err(" Could not open gradient file ! ");
+        float *in = rzs_alloc(n * sizeof(float)), *out = rzs_alloc(n * sizeof(float));
+        const size_t got = fread(in, sizeof(float), n, fp);   /* a short file: fread leaves the last value */
+        fclose(fp);
+        for (size_t t = got; t < n; ++t) in[t] = got ? in[got - 1] : 0.f;
+        const char *dev = getenv("RAJZEL_AMD_DEVICE");
+        if (rz_smooth2(dev ? atoi(dev) : 0, NULL, in, out, NXG, NYG, WD_DAMP, WD_DAMP1) != RZ_OK) {
+            snprintf(g_msg, sizeof g_msg, " rajzel_amd smooth2: %s", rz_last_error(NULL));
+            err(g_msg);
+        }
+        snprintf(jac, sizeof jac, "%s_smooth2.old", JACOBIAN);
+        fp = fopen(jac, "wb");
+        if (fp) {
+            fwrite(out, sizeof(float), n, fp);
+            fclose(fp);
+        }
+        free(in);
+        free(out);
+    }
+    MPI_Barrier(MPI_COMM_WORLD);
+    snprintf(jac, sizeof jac, "%s_smooth2.old", JACOBIAN);  /* smooth2.c:176-199 */
+    fp = fopen(jac, "rb");
+    const size_t got = fp ? fread(buf, sizeof(float), n, fp) : 0;
+    if (fp) fclose(fp);
+    size_t t = 0;
+    for (int i = 1; i <= NXG; i++)
+        for (int j = 1; j <= NYG; j++, t++) {
+            const float g = t < got ? buf[t] : (got ? buf[got - 1] : 0.f);
+            if (POS[1] == (i - 1) / NX && POS[2] == (j - 1) / NY) grad[j - POS[2] * NY][i - POS[1] * NX] = g;
+        }
+    free(buf);
 }
 
 /* taper_grad.c:11-299 (precond.c:43-53) with the multiplication on the GPU:

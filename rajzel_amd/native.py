@@ -29,7 +29,7 @@ EXPORTS = (
     "rz_slot_bytes", "rz_get_stop_passes", "rz_set_shot_taper", "rz_ass_grad_shot_at", "rz_taper_grad",
     "rz_dotp_ordered", "rz_lbfgs_create", "rz_lbfgs_destroy", "rz_lbfgs_error", "rz_lbfgs_zero",
     "rz_lbfgs_has_previous", "rz_lbfgs_set_previous", "rz_lbfgs_step", "rz_compute_units",
-    "rz_set_split_shots", "rz_get_split_shots", "rz_tune_split",
+    "rz_set_split_shots", "rz_get_split_shots", "rz_tune_split", "rz_median_filt", "rz_smooth2",
 )
 
 
@@ -101,6 +101,8 @@ def lib():
                 "rz_set_split_shots": (i, [vp, i]),
                 "rz_get_split_shots": (i, [vp]),
                 "rz_tune_split": (i, [vp, C.POINTER(i), C.POINTER(d)]),
+                "rz_median_filt": (i, [i, vp, vp, i, i, C.POINTER(i), i]),
+                "rz_smooth2": (i, [i, vp, vp, vp, i, i, f, f]),
                 "rz_device_bytes": (C.c_size_t, [vp]),
                 "rz_batch_shots": (i, [vp]),
                 "rz_slot_bytes": (C.c_size_t, [vp]),
@@ -246,6 +248,32 @@ def taper_grad(field, sws, gradt=(0, 0, 0, 0), dh=0.0, exp_hor=0.0, srtradius=0.
     if rc != RZ_OK:
         raise RzError(rc, lib().rz_last_error(None).decode())
     return (out, coeff) if want_coeff else out
+
+
+def median_filt(field, fsize, device=0, stream=None):
+    """smooth_grad.c's median filter on a numpy (NY, NX) float32 array (a
+    copy).  Returns (field, fsize made odd)."""
+    out = np.ascontiguousarray(field, np.float32).copy()
+    ny, nx = out.shape
+    fs = C.c_int(int(fsize))
+    rc = lib().rz_median_filt(device, stream, _ptr(out), nx, ny, C.byref(fs), 0)
+    if rc != RZ_OK:
+        raise RzError(rc, lib().rz_last_error(None).decode())
+    return out, fs.value
+
+
+def smooth2(field, wd_damp, wd_damp1, device=0, stream=None):
+    """smooth2.c's tridiagonal smoothing of a numpy (NY, NX) float32 field
+    (unstrided); returns the smoothed field (NY, NX) as smooth2 leaves it in
+    grad."""
+    A = np.ascontiguousarray(field, np.float32)
+    ny, nx = A.shape
+    xin = np.ascontiguousarray(A.T)                       # x slow, z fast
+    out = np.empty_like(xin)
+    rc = lib().rz_smooth2(device, stream, _ptr(xin), _ptr(out), nx, ny, float(wd_damp), float(wd_damp1))
+    if rc != RZ_OK:
+        raise RzError(rc, lib().rz_last_error(None).decode())
+    return np.ascontiguousarray(out.T)
 
 
 def dotp_ordered(a, b, device=0):

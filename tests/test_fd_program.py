@@ -186,6 +186,21 @@ def test_fatt_mpi_ranks(ranks):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("spatfilter", [2, 3])
+def test_fatt_smoothing_filters(spatfilter):
+    """precond.c's SPATFILTER = 2 (smooth2.c, WD_DAMP/WD_DAMP1) and 3
+    (smooth_grad.c, the median over FILT_SIZE_GRAD) through the shim's GPU
+    versions: every file (models, history, smooth2's two files) equals the
+    reference program's."""
+    sv = P.survey()
+    obs = _observed(sv)
+    row = dict(PRO=0.0, SPATFILTER=spatfilter, WD_DAMP=2.0, WD_DAMP1=3.0, FILT_SIZE_GRAD=4, FILT_SIZE_GRAD1=6)
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3, stages=[row]))
+    if spatfilter == 2:
+        assert "jac/grad_smooth2.old" in out and "jac/grad_wavenumber.old" in out
+
+
+@pytest.mark.gpu
 def test_fatt_pcg_parabolic_line_search():
     """PCG.c directions with the parabolic line search (parabolicls.c)."""
     sv = P.survey()
