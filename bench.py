@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-precond", action="store_true", help="skip the Gaussian gradient filter timing")
     ap.add_argument("--no-lbfgs", action="store_true", help="skip the l-BFGS two-loop timing")
+    ap.add_argument("--split-shots", type=int, default=None,
+                    help="split shots per mixed launch (-1 = every free CU, 0 = none); default: tuned")
     ap.add_argument("--no-split", action="store_true",
                     help="one workgroup per shot in every launch (RAJZEL_AMD_SPLIT=0)")
     ap.add_argument("--concurrency-scan", action="store_true",
@@ -374,7 +376,11 @@ def main():
     ctx.gradient()
     # shots per CU: how many of the slowest shots run split in the mixed
     # launches, timed on this evaluation (every candidate gives the same bits)
-    split_shots, tune_ms = (-1, None) if args.no_split else ctx.tune_split()
+    if args.no_split or args.split_shots is not None:
+        split_shots, tune_ms = (-1 if args.split_shots is None else args.split_shots), None
+        ctx.set_split_shots(split_shots)
+    else:
+        split_shots, tune_ms = ctx.tune_split()
     log(f"[bench] setup done, R={ctx.rows_per_lane}, {ctx.device_bytes / 2**30:.2f} GiB on device, "
         f"split {ctx.split_state()}, split shots {split_shots} ({tune_ms} ms)")
 
