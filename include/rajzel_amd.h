@@ -326,8 +326,10 @@ int rz_smooth2(int device, void *hip_stream, const float *in, float *out, int n2
  * two paired sweeps in two workgroups on two CUs of one XCD, coupled
  * through L2, instead of one workgroup on one CU.  If a launch finds the two
  * workgroups of a shot on different XCDs, or a wait for the partner times
- * out, the context repeats the work unsplit and stays unsplit; failure then
- * reads 1 or 2.  Results are identical either way.  on = 2 also marks the
+ * out, the context repeats the work unsplit; failure then reads 1 or 2.  It
+ * stays unsplit for 16 runs (evaluations or forwards), then tries split
+ * launches again; each further failure doubles that wait (32, 64, ...).
+ * rz_set_split resets the wait.  Results are identical either way.  on = 2 also marks the
  * next split launch as failed before it starts (a test of that fallback).
  * No counterpart in the reference (its sweeps run one after another on one
  * core). */

@@ -83,6 +83,9 @@ struct rz_ctx {
     int ncu = 0;
     int split_fail = 0;                 // 1: workgroups of a shot on different XCDs, 2: a wait timed out
     int split_shots = -1;               // split shots per mixed launch: -1 all the CUs allow, 0 none
+    // after a failed split launch: runs until split is tried again, and the
+    // wait after the next failure (doubled each time: 16, 32, 64, ...)
+    long long split_wait = 0, split_backoff = 16;
     SplitSync ss{nullptr, nullptr, nullptr};
     // Mixed launches: with more shots than half the CUs (and fewer than
     // the CUs), the shots that took the most outer iterations in the
@@ -436,6 +439,8 @@ int check_convergence(rz_ctx *c, bool adj)
     if (serr != 0) {
         c->split = false;
         c->split_fail = serr;
+        c->split_wait = c->split_backoff;
+        c->split_backoff = std::min<long long>(2 * c->split_backoff, 1 << 20);
         HIPCHK(c, hipMemsetAsync(c->ss.err, 0, sizeof(int), c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return kSplitRetry;
@@ -696,6 +701,13 @@ int run(rz_ctx *c, bool grad, int cap, bool direct = false)
                 return fail(c, RZ_ERR_STATE, "model %zu: residuals not set", i);
     }
     TRY(ready(c, grad && !direct));
+    // a context unsplit by a failed launch (its two workgroups of a shot
+    // apart, or a wait timed out, e.g. with another process on the GPU)
+    // tries split launches again after a back-off
+    if (!c->split && c->split_fail && c->split_wait > 0 && --c->split_wait == 0) {
+        c->split = true;
+        drop_graphs(c);
+    }
     // forward fields of every shot are still resident and current: the
     // adjoint-only run does not recompute them
     const bool reuse = direct && c->fields_epoch == c->s_epoch && c->nslots >= c->total_shots;
@@ -1472,6 +1484,8 @@ int rz_set_split(rz_ctx *c, int on)
     if (!c || on < 0 || on > 2) return RZ_ERR_INVALID;
     if (c->split != (on != 0)) drop_graphs(c);
     c->split = on != 0;
+    c->split_wait = 0;
+    c->split_backoff = 16;
     if (on) c->split_fail = 0;
     if (on == 2) {   // the next split launch finds its failure flag already raised
         const int two = 2;

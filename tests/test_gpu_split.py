@@ -143,3 +143,27 @@ def test_split_shot_counts_and_tuner_keep_results():
     ref = outs[-1]
     for k, v in outs.items():
         assert bits_equal(v[0], ref[0]) and bits_equal(v[1], ref[1]) and v[2] == ref[2], k
+
+
+def test_failed_split_launch_retries_after_backoff():
+    """After a failed split launch the context runs unsplit for 16 runs,
+    then tries split launches again (the results are the same bits either
+    way)."""
+    from rajzel_amd import native as N
+    sv = W.small_case(nx=64, ny=130, nshots=4, seed=6, rec_row=(2, 129))
+    sv.tobs = np.full((sv.nshots, sv.ntr), 0.3, np.float32)
+    print("[case] split retry 130x64", flush=True)
+    ref = _run(N, sv, 2, False)
+    with N.Context([model_dict(sv, max_iter=200)], rows_per_lane=2) as ctx:
+        ctx.set_split(2)
+        ctx.set_slowness(0, sv.S)
+        ctx.set_picks(0, sv.tobs)
+        ctx.gradient()
+        assert ctx.split_state() == (0, 2)
+        states = []
+        for _ in range(17):
+            ctx.gradient()
+            states.append(ctx.split_state()[0])
+            assert bits_equal(ctx.grad(0), ref["grad"])
+        # the failed run's own unsplit repetition counts as one of the 16
+        assert 1 in states and states.index(1) >= 14 and states[-1] == 1, states
