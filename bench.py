@@ -7,6 +7,9 @@ assembly, summed over shots.  Workload: 1001 x 3001 grid (DH = 5 m), 200
 surface shots, 1500 receivers at row 2; current model = 12-layer model,
 picks = times in the model with 20 Gaussian anomalies + N(0, 1 ms) noise
 (synthetic, seeded).  Inputs are resident in HBM before the timed region.
+``--workload c5``: the 2000 x 6000 grid (DH = 2.5 m), 400 shots, 3000
+receivers, current model the 1-D gradient 1500 + 0.6 z, picks from it plus
+40 Gaussian anomalies.
 
 Printed: one JSON line with the shot-gradients/s of the batched launch,
 the plain one-shot-per-launch form on the same GPU, the HBM roofline of the
@@ -303,7 +306,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--shots", type=int, default=200)
+    ap.add_argument("--workload", choices=("c4", "c5"), default="c4",
+                    help="c4: 1001 x 3001, DH 5 m, 200 shots (the default); c5: 2000 x 6000, DH 2.5 m, 400 shots")
+    ap.add_argument("--shots", type=int, default=0, help="shots of the workload (default: all of it)")
     ap.add_argument("--plain-shots", type=int, default=10,
                     help="shots timed in the one-shot-per-launch form (rate is per shot)")
     ap.add_argument("--cpu-shots", type=int, default=0,
@@ -332,6 +337,12 @@ def main():
               f"(--gpus {args.gpus}) is not part of this build", file=sys.stderr)
         sys.exit(2)
 
+    # The one JSON line goes to this process's stdout; everything else that
+    # writes to file descriptor 1 (the reference's C routines print there,
+    # and so do the CPU baseline's processes) goes to stderr.
+    sys.stdout.flush()
+    result = os.fdopen(os.dup(1), "w")
+    os.dup2(2, 1)
     if args.no_split:
         os.environ["RAJZEL_AMD_SPLIT"] = "0"
     from rajzel_amd import formats as F
@@ -339,7 +350,8 @@ def main():
     from rajzel_amd import workloads as W
 
     t_setup = time.perf_counter()
-    sv = W.c4_gradient(nshots=args.shots)
+    make = W.c4_gradient if args.workload == "c4" else W.c5_gradient
+    sv = make(nshots=args.shots) if args.shots else make()
     S = sv.S
     S_true = W.slowness(sv.vp_true)
     log(f"[bench] workload {sv.name}: {sv.nshots} shots, {sv.ntr} receivers "
@@ -358,7 +370,7 @@ def main():
             log(f"reference CPU build unavailable ({e})")
     if args.concurrency_scan:
         print(json.dumps({"workload": f"{sv.ny}x{sv.nx} grid, {sv.ntr} receivers, picks 0.5 s",
-                          "concurrency": concurrency_scan(sv, args.rows_per_lane)}), flush=True)
+                          "concurrency": concurrency_scan(sv, args.rows_per_lane)}), file=result, flush=True)
         return
 
     ctx = N.Context([model_dict(sv)], rows_per_lane=args.rows_per_lane)
@@ -410,7 +422,8 @@ def main():
     achieved = phase_bytes[dom] / (phase_ms[dom] * 1e-3) / 1e9
     eval_gbs = (fwd_b + adj_b + grd_b) / (ms * 1e-3) / 1e9
     workload = f"FATT gradient evaluation, {sv.ny}x{sv.nx} grid, DH={sv.dh} m, " \
-               f"{sv.nshots} shots, {sv.ntr} receivers, 12-layer model"
+               f"{sv.nshots} shots, {sv.ntr} receivers, " + \
+               ("12-layer model" if args.workload == "c4" else "1-D gradient model")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
                 "traffic": measured_traffic(args.traffic, dom, workload),
@@ -418,7 +431,8 @@ def main():
                 "bytes_per_launch": phase_bytes[dom], "avg_launch_ms": phase_ms[dom]}
 
     out = {
-        "metric": METRIC, "value": value, "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC if args.workload == "c4" else METRIC.replace("1001x3001", f"{sv.ny}x{sv.nx}"),
+        "value": value, "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload,
@@ -486,7 +500,7 @@ def main():
         log(f"[bench] lbfgs: {out['lbfgs']}")
     if args.copy_probe:
         copy_probe(args.copy_probe)
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out), file=result, flush=True)
 
 
 if __name__ == "__main__":

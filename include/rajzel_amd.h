@@ -81,9 +81,11 @@ int rz_grid_index(float pos_m, float dh);
  *                  the band count; RZ_ROWS_AUTOTUNE = measured by
  *                  rz_autotune_rows()
  *   hbm_budget     bytes of device memory for per-shot fields; 0 = enough
- *                  for every shot at once.  Below that, shots run in
- *                  batches that fit and forward fields are recomputed per
- *                  batch (results are unchanged). */
+ *                  for every shot at once when that fits the device's free
+ *                  memory less 1/16 of the device, else what does fit.
+ *                  Below all shots, shots run in batches that fit and
+ *                  forward fields are recomputed per batch (results are
+ *                  unchanged). */
 #define RZ_ROWS_AUTOTUNE (-1)
 int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models, int rows_per_lane,
               size_t hbm_budget);

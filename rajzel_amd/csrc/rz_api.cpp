@@ -966,6 +966,17 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     // as many as the budget allows
     const size_t per_slot = (size_t)c->slot_floats * 5 * sizeof(float);
     int nslots = std::max(1, shots);
+    if (hbm_budget == 0) {
+        // no budget given: every shot if the fields fit the device's free
+        // memory less a reserve of 1/16 of the device (the tables below,
+        // the caller's own buffers), else as many shots as do fit
+        size_t mfree = 0, mtotal = 0;
+        if (hipMemGetInfo(&mfree, &mtotal) == hipSuccess && mtotal > 0) {
+            const size_t reserve = mtotal / 16;
+            const size_t room = mfree > reserve ? mfree - reserve : 0;
+            if ((size_t)nslots * per_slot > room) hbm_budget = std::max(room, per_slot);
+        }
+    }
     if (hbm_budget > 0) {
         const size_t fit = hbm_budget / per_slot;
         if (fit < 1) return bail(fail(c, RZ_ERR_NOMEM, "HBM budget below one shot's fields (%zu bytes)", per_slot));

@@ -191,8 +191,13 @@ static rz_ctx *rzs_context(float **srcpos, int s1, int s2, int **recpos, int ntr
     d.nshots = ns; d.src_ix = sx; d.src_iy = sy;
     d.ntr = ntr; d.rec_ix = rx; d.rec_iy = ry;
     const char *dev = getenv("RAJZEL_AMD_DEVICE");
+    /* RAJZEL_AMD_HBM_MIB: device memory for the shots' fields in MiB; without
+     * it every shot is resident when the device has room, else rz_create
+     * takes what fits.  Under a budget the shots run in batches. */
+    const char *mib = getenv("RAJZEL_AMD_HBM_MIB");
+    const size_t budget = (mib && *mib) ? (size_t)strtoull(mib, NULL, 10) << 20 : 0;
     rz_ctx *c = NULL;
-    if (rz_create(&c, dev ? atoi(dev) : 0, 1, &d, 0, 0) != RZ_OK) {
+    if (rz_create(&c, dev ? atoi(dev) : 0, 1, &d, 0, budget) != RZ_OK) {
         snprintf(g_msg, sizeof g_msg, " rajzel_amd context: %s", rz_last_error(NULL));
         err(g_msg);
     }
@@ -326,17 +331,25 @@ void forward(float **S, float **TT, float *Tmod, float **srcpos, int nshots, int
     rzs_check(rz_forward(c), "forward");
     float *tm = rzs_alloc((size_t)ns * (ntr > 0 ? ntr : 1) * sizeof(float));
     if (ntr > 0) rzs_check(rz_get_tmod(c, 0, tm, 0), "receiver times");
+    /* under an HBM budget only the last batch's fields stay resident: the
+     * snapshots of the others are solved again one shot at a time (the
+     * same bits), which leaves the last shot's field in TT */
+    const int resident = rz_batch_shots(c) >= ns;
     for (int s = 0; s < ns; ++s) {
         const int ishot = NSHOT1 + s;
         if (ntr > 0) memcpy(&Tmod[1], tm + (size_t)s * ntr, ntr * sizeof(float));
         if (SNAP == 1) {
-            rzs_check(rz_get_field(c, 0, s, 0, &TT[1][1], 0), "traveltime field");
+            if (resident)
+                rzs_check(rz_get_field(c, 0, s, 0, &TT[1][1], 0), "traveltime field");
+            else
+                rzs_check(rz_eikonal_shot(c, 0, g_rzs.src_ix[s], g_rzs.src_iy[s], &TT[1][1], NULL, NULL),
+                          "traveltime field");
             snprintf(filename, sizeof filename, "%s_shot_%d.tt", SNAP_FILE, ishot);
             writemod(filename, TT, 3);
         }
         if (SEISMO == 1) write_picks(Tmod, ntr, ishot);
     }
-    rzs_last_shot(ns, ntr, TT, NULL, Tmod, tm);
+    rzs_last_shot(ns, ntr, (SNAP == 1 && !resident) ? NULL : TT, NULL, Tmod, tm);
     free(tm);
 }
 

@@ -22,7 +22,7 @@
 namespace rz {
 
 constexpr int kWave = 64;
-constexpr int kPF = 8;                // prefetch depth == steps between barriers
+constexpr int kPF = 8;                // steps between barriers (an epoch); prefetch depth at R <= 4
 constexpr int kLag = kPF;             // extra lag between consecutive bands (>= kPF-1)
 constexpr int kBandOff = kWave + kLag;// step offset between consecutive bands
 constexpr int kRingQ = 32;            // LDS ring slots per band (>= kLag+1+kPF, power of 2)
@@ -33,6 +33,13 @@ constexpr int kMaxBands = 8;          // bands (waves) per group
 // Bands a group may hold at R rows per lane: 8 for R <= 2, 4 above (with
 // two groups: 16 resp. 8 waves, a budget of 128 resp. 256 VGPRs per lane).
 constexpr int max_bands_for(int R) { return R <= 2 ? 8 : 4; }
+// Prefetch slots of a sweep: a step's blocks are loaded this many steps
+// before it (one less in the adjoint sweep).  At R = 8 a step holds twice
+// the rows of one at R = 4, so four slots keep the bytes in flight per wave
+// and about the same lead in time, and the sweep within 256 VGPRs (eight
+// slots spilled 32 to 60 registers to scratch).
+constexpr int pf_slots(int R) { return R >= 8 ? 4 : kPF; }
+static_assert(kPF % pf_slots(8) == 0 && kPF % pf_slots(4) == 0, "an epoch holds whole rounds of slots");
 
 // INF of the reference (fd.h:29-31): an int compared against float fields.
 constexpr float kInf = 9999999.0f;

@@ -944,6 +944,7 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
     constexpr int SP = Stage<R>::P, SQ = Stage<R>::Q;
+    constexpr int PS = pf_slots(R);
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
     const int nkeys = nx + 63;          // data keys 0..nx+62; key nkeys is the dummy block
     const int gend = sweep_steps(nx, nb, Stage<R>::HOLD);
@@ -986,7 +987,7 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
     lfloat *ring = (lfloat *)w.ring;
     lfloat *ring_dummy = ring + kMaxBands * kRingQ + l;
 
-    float qt[kPF][R], qs[kPF][R], qe[kPF];
+    float qt[PS][R], qs[PS][R], qe[PS];
 
 #define RZ_FWD_LOAD(SLOT, TAU)                                                              \
     do {                                                                                    \
@@ -1007,10 +1008,10 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
     } while (0)
 
 #pragma unroll
-    for (int u = 0; u < kPF; ++u) RZ_FWD_LOAD(u, u - off);
+    for (int u = 0; u < PS; ++u) RZ_FWD_LOAD(u, u - off);
     // The set-up ends in a full wait: the waits the compiler derives at the
     // loop head then come from the loop's own pattern of prefetches (a slot
-    // is waited for about seven steps after its load), not from a set-up
+    // is waited for PS - 1 steps after its load), not from a set-up
     // load just before the loop, which would cost a full memory latency in
     // every epoch.
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
@@ -1025,7 +1026,7 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
     for (int g0 = 0; g0 < gend; g0 += kPF) {
 #pragma unroll
         for (int u = 0; u < kPF; ++u) {
-            const int un = (u + 1) % kPF;
+            const int sl = u % PS, un = (u + 1) % PS;   // g0 is a multiple of PS
             const int tau = g0 + u - off;
             const bool active = tau >= 0 && tau < nkeys;   // wave-uniform
             const int c = SX > 0 ? tau - lag : nx - 1 - tau + lag;
@@ -1039,7 +1040,7 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
             }
             float nv[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) nv[r] = qt[u][r];
+            for (int r = 0; r < R; ++r) nv[r] = qt[sl][r];
             // staged store, part 1: the piece of step tau - dl (written at
             // least one step ago), read before this step's arithmetic
             const int tp = tau - dl;
@@ -1053,12 +1054,12 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
                     up_new = lane_from_above(pnew[R - 1]);
                     dn_old = lane_from_below(qt[un][0]);
                     if (l == 0) up_new = ringv;
-                    if (l == 63) dn_old = qe[u];
+                    if (l == 63) dn_old = qe[sl];
                 } else {
                     up_new = lane_from_below(pnew[0]);
                     dn_old = lane_from_above(qt[un][R - 1]);
                     if (l == 63) up_new = ringv;
-                    if (l == 0) dn_old = qe[u];
+                    if (l == 0) dn_old = qe[sl];
                 }
                 // The R cells of a lane form a chain along y; all of them are
                 // first updated with the fast root.  If any cell of the wave
@@ -1066,11 +1067,11 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
                 // redone with the correctly rounded root.
                 float fsum = 0.f;
                 bool need = false;
-                fwd_cells<R, SY, false>(qt[u], qt[un], qs[u], pnew, up_new, dn_old, first, last, top, bot,
+                fwd_cells<R, SY, false>(qt[sl], qt[un], qs[sl], pnew, up_new, dn_old, first, last, top, bot,
                                         rvalid, cvalid, dh, two_dh2, nv, fsum, need);
                 if (__builtin_amdgcn_ballot_w64(need)) {
                     fsum = 0.f;
-                    fwd_cells<R, SY, true>(qt[u], qt[un], qs[u], pnew, up_new, dn_old, first, last, top, bot,
+                    fwd_cells<R, SY, true>(qt[sl], qt[un], qs[sl], pnew, up_new, dn_old, first, last, top, bot,
                                            rvalid, cvalid, dh, two_dh2, nv, fsum, need);
                 }
                 acc += (double)fsum;
@@ -1086,7 +1087,7 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
             }
             lfloat *rw = (ring_writer && cvalid) ? &ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
-            RZ_FWD_LOAD(u, tau + kPF);
+            RZ_FWD_LOAD(sl, tau + PS);
         }
         pace_epoch<PACE>(pc, g0 / kPF);
     }
@@ -1166,6 +1167,7 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     constexpr bool FA = (SX == SY);
     constexpr int BR = kWave * R;
     constexpr int SP = Stage<R>::P, SQ = Stage<R>::Q;
+    constexpr int PS = pf_slots(R);
     const int nx = w.nx, ny = w.ny, nb = w.nb, beta = w.beta, l = w.lane;
     const int nkeys = nx + 63;
     const int gend = sweep_steps(nx, nb, Stage<R>::HOLD);   // the same for both kinds of output
@@ -1208,7 +1210,7 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     lfloat *ring = (lfloat *)w.ring;
     lfloat *ring_dummy = ring + kMaxBands * kRingQ + l;
 
-    float qt[kPF][R], ql[kPF][R], qeu[kPF], qed[kPF], qel[kPF];
+    float qt[PS][R], ql[PS][R], qeu[PS], qed[PS], qel[PS];
 
 #define RZ_ADJ_LOAD(SLOT, TAU)                                                              \
     do {                                                                                    \
@@ -1225,20 +1227,20 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
         qel[SLOT] = ldb1(rL, SY > 0 ? ed_ : eu_);                                           \
     } while (0)
 
-    // Slot u holds the blocks of step u (mod kPF).  A step reads its own
+    // Slot u holds the blocks of step u (mod PS).  A step reads its own
     // slot, the next one (T and lambda at the next key) and the previous one
     // (T at the previous key), then reloads the previous slot with the
-    // blocks of kPF - 1 steps ahead: no value outlives its slot, so the
+    // blocks of PS - 1 steps ahead: no value outlives its slot, so the
     // steady-state loop needs no register copies, and its waits are counted
-    // ones (a slot is waited for about five steps after its load).  The
-    // set-up ends in a full wait, so that the loop head's waits do not
-    // depend on set-up loads.  Slot kPF-1 is first loaded by step 0; before
-    // that it stands for T at the key before the first, which only cells
-    // that keep their value read.
+    // ones (a slot is waited for a few steps after its load).  The set-up
+    // ends in a full wait, so that the loop head's waits do not depend on
+    // set-up loads.  Slot PS-1 is first loaded by step 0; before that it
+    // stands for T at the key before the first, which only cells that keep
+    // their value read.
 #pragma unroll
-    for (int u = 0; u < kPF - 1; ++u) RZ_ADJ_LOAD(u, u - off);
+    for (int u = 0; u < PS - 1; ++u) RZ_ADJ_LOAD(u, u - off);
 #pragma unroll
-    for (int r = 0; r < R; ++r) qt[kPF - 1][r] = 0.f;
+    for (int r = 0; r < R; ++r) qt[PS - 1][r] = 0.f;
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
 
     // xcp/xcn: positive and negative parts of the x quotient carried to the
@@ -1252,7 +1254,8 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
     for (int g0 = 0; g0 < gend; g0 += kPF) {
 #pragma unroll
         for (int u = 0; u < kPF; ++u) {
-            const int un = (u + 1) % kPF, up = (u + kPF - 1) % kPF;
+            // g0 is a multiple of PS: step g0 + u uses slot u % PS
+            const int sl = u % PS, un = (u + 1) % PS, up = (u + PS - 1) % PS;
             const float (&tprev)[R] = qt[up];   // T at the previous key
             const int tau = g0 + u - off;
             const bool active = tau >= 0 && tau < nkeys;   // wave-uniform
@@ -1265,7 +1268,7 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
             }
             float nv[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) nv[r] = ql[u][r];
+            for (int r = 0; r < R; ++r) nv[r] = ql[sl][r];
 
             // Every step is computed: an inactive step only has lanes that
             // keep their value (cvalid is false).  Without a branch the eight
@@ -1283,19 +1286,19 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
                     t_above = lane_from_above(qt[un][R - 1]);
                     t_below = lane_from_below(tprev[0]);
                 }
-                if (l == 0) t_above = qeu[u];
-                if (l == 63) t_below = qed[u];
+                if (l == 0) t_above = qeu[sl];
+                if (l == 63) t_below = qed[sl];
                 // lambda across the lane boundary: new upstream, old downstream
                 float l_up, l_dn;   // row k-1 for r = 0, row k+1 for r = R-1
                 if (SY > 0) {
                     l_up = lane_from_above(pnew[R - 1]);
                     l_dn = lane_from_below(ql[un][0]);
                     if (l == 0) l_up = ringv;
-                    if (l == 63) l_dn = qel[u];
+                    if (l == 63) l_dn = qel[sl];
                 } else {
                     l_up = lane_from_above(ql[un][R - 1]);
                     l_dn = lane_from_below(pnew[0]);
-                    if (l == 0) l_up = qel[u];
+                    if (l == 0) l_up = qel[sl];
                     if (l == 63) l_dn = ringv;
                 }
                 // T-only part of the cell equations (no dependence on lambda).
@@ -1306,12 +1309,12 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
                 bool keep[R], infv[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const float traw = qt[u][r];
+                    const float traw = qt[sl][r];
                     const float tc = fabsf(traw);
                     const float tl = fabsf(SX > 0 ? tprev[r] : qt[un][r]);   // T(k, c-1)
                     const float tr = fabsf(SX > 0 ? qt[un][r] : tprev[r]);   // T(k, c+1)
-                    const float tu = fabsf(r == 0 ? t_above : qt[u][r > 0 ? r - 1 : 0]);
-                    const float td = fabsf(r == R - 1 ? t_below : qt[u][r < R - 1 ? r + 1 : r]);
+                    const float tu = fabsf(r == 0 ? t_above : qt[sl][r > 0 ? r - 1 : 0]);
+                    const float td = fabsf(r == R - 1 ? t_below : qt[sl][r < R - 1 ? r + 1 : r]);
                     float app, amm;   // pos_part(ap), neg_part(am)
                     if (SX > 0) {
                         const float ap = qdh_t<FD>(-(tr - tc), dh, rdh, rh, rl);
@@ -1351,15 +1354,15 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
                     const int r = SY > 0 ? i : R - 1 - i;
-                    const float lc = ql[u][r];
+                    const float lc = ql[sl][r];
                     const float lxm = SX > 0 ? pnew[r] : ql[un][r];            // lambda(k, c-1)
                     const float lxp = SX > 0 ? ql[un][r] : pnew[r];            // lambda(k, c+1)
                     float lym, lyp;
                     if (SY > 0) {
                         lym = (r == 0) ? l_up : nv[r > 0 ? r - 1 : 0];
-                        lyp = (r == R - 1) ? l_dn : ql[u][r < R - 1 ? r + 1 : r];
+                        lyp = (r == R - 1) ? l_dn : ql[sl][r < R - 1 ? r + 1 : r];
                     } else {
-                        lym = (r == 0) ? l_up : ql[u][r > 0 ? r - 1 : 0];
+                        lym = (r == 0) ? l_up : ql[sl][r > 0 ? r - 1 : 0];
                         lyp = (r == R - 1) ? l_dn : nv[r < R - 1 ? r + 1 : r];
                     }
                     const float rhs = qdh<FD>(amp[r] * lxm - apm[r] * lxp, dh, rdh)
@@ -1396,7 +1399,7 @@ __device__ __attribute__((noinline)) double adj_sweep(const WaveCtx w_, float dh
             }
             lfloat *rw = (ring_writer && cvalid) ? &ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
             *rw = SY > 0 ? nv[R - 1] : nv[0];
-            RZ_ADJ_LOAD(up, tau + kPF - 1);
+            RZ_ADJ_LOAD(up, tau + PS - 1);
             // keeps the reloads in their step (the scheduler would otherwise sink
             // them toward their use, shortening the prefetch distance)
             __builtin_amdgcn_sched_barrier(0);

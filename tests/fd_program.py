@@ -69,15 +69,23 @@ def make_workspace(sv, vp_model, invmat, picks=None, spatfilter=0, stages=None, 
 MPIEXEC = next((p for p in ("/opt/conda/bin/mpiexec", "/usr/bin/mpiexec") if os.path.exists(p)), None)
 
 
-def run(ws, lib, timeout=300, env=None, ranks=1):
+def run(ws, lib, timeout=300, env=None, ranks=1, log=None):
     """Run the program in ``ws`` (``env``: extra environment variables) as
     one process, or as ``ranks`` MPI ranks under mpiexec; returns the
-    CompletedProcess."""
+    CompletedProcess.  With ``log`` (a file path) the program writes its
+    output there as it runs, and stdout and stderr of the result both hold
+    that file's text."""
     env = dict(os.environ, **(env or {}))
     cmd = [LAUNCHER, lib, "rz.inp", "wf.inp"]
     if ranks > 1:
         cmd = [MPIEXEC, "-n", str(ranks)] + cmd
-    return subprocess.run(cmd, cwd=ws, env=env, capture_output=True, text=True, timeout=timeout)
+    if log is None:
+        return subprocess.run(cmd, cwd=ws, env=env, capture_output=True, text=True, timeout=timeout)
+    with open(log, "w") as fh:
+        r = subprocess.run(cmd, cwd=ws, env=env, stdout=fh, stderr=subprocess.STDOUT, timeout=timeout)
+    with open(log) as fh:
+        text = fh.read()
+    return subprocess.CompletedProcess(cmd, r.returncode, text, text)
 
 
 def outputs(ws):
@@ -281,6 +289,81 @@ def time_fatt_c4(itermax=20):
     return res
 
 
+def digests(ws):
+    """SHA-256 of every output file of the run (logs excluded)."""
+    import hashlib
+    out = {}
+    for base, _, files in os.walk(ws):
+        rel = os.path.relpath(base, ws)
+        if rel.split(os.sep)[0] == "log":
+            continue
+        for f in files:
+            h = hashlib.sha256()
+            with open(os.path.join(base, f), "rb") as fh:
+                for chunk in iter(lambda: fh.read(1 << 24), b""):
+                    h.update(chunk)
+            out[os.path.normpath(os.path.join(rel, f))] = h.hexdigest()
+    return out
+
+
+def time_fatt_c5(itermax=30, batch=100, log_dir=None):
+    """The 2000 x 6000 configuration's inversion: 400 shots, 3000 receivers,
+    DH = 2.5 m, l-BFGS (NLBFGS = 20) with the Wolfe line search for
+    `itermax` iterations, the Gaussian gradient filter (SPATFILTER = 4, an
+    81 x 41 cell window) and the horizontal gradient taper in precond, on
+    the MI355X program; once with every shot's fields resident (default)
+    and once under an HBM budget of `batch` shots (RAJZEL_AMD_HBM_MIB), so
+    that each evaluation runs in shot batches.  Picks: the forward in the
+    true model plus N(0, 1 ms).  Returns the wall times, the grad_obj
+    calls, the misfit log and whether the two runs wrote the same files.
+    ``log_dir``: where the program's output of each run goes as it runs."""
+    import time
+    from rajzel_amd import native as N
+    t0 = time.perf_counter()
+    sv = W.c5_gradient()
+    with N.Context([dict(nx=sv.nx, ny=sv.ny, dh=sv.dh, src_ix=sv.src_ix, src_iy=sv.src_iy, rec_ix=sv.rec_ix,
+                         rec_iy=sv.rec_iy)]) as ctx:
+        ctx.set_slowness(0, W.slowness(sv.vp_true))
+        ctx.forward()
+        tm = ctx.tmod(0)
+        budget_mib = -(-ctx.slot_bytes * batch // (1 << 20))
+        slot_mib = ctx.slot_bytes / 2**20
+    obs = F.roundtrip_picks(tm + np.random.default_rng(sv.meta["seed"] + 1000).normal(0.0, sv.noise_s, tm.shape))
+    print(f"[step] workload and picks {time.perf_counter() - t0:.1f} s", flush=True)
+    res = {"grid": f"{sv.ny}x{sv.nx}", "dh": sv.dh, "shots": sv.nshots, "receivers": sv.ntr, "itermax": itermax,
+           "method": "l-BFGS, NLBFGS = 20, Wolfe line search, SPATFILTER = 4 (81 x 41), horizontal taper",
+           "field_mib_per_shot": slot_mib, "budget_mib": budget_mib}
+    stage = [dict(PRO=0.0, SPATFILTER=4, WD_DAMP=1e-3, WD_DAMP1=1e-3, FILT_SIZE_GRAD=80, FILT_SIZE_GRAD1=40)]
+    spaces, outs = [], []
+    try:
+        for key, env in (("resident_s", {}), ("batched_s", {"RAJZEL_AMD_HBM_MIB": str(budget_mib)})):
+            ws = make_workspace(sv, sv.vp, invmat=1, picks=obs, stages=stage, ITERMAX=itermax, SEISMO=0,
+                                LINESEARCH=1, NLBFGS=20, GRAD_METHOD=2, SWS_TAPER_GRAD_HOR=1, GRADT1=4, GRADT2=8,
+                                EXP_TAPER_GRAD_HOR=0.0)
+            spaces.append(ws)
+            t0 = time.perf_counter()
+            r = run(ws, GPU_LIB, timeout=1000, env=dict(env, RAJZEL_AMD_SPECULATE_STATS="1"),
+                    log=os.path.join(log_dir, f"fatt_c5_{key[:-2]}.log") if log_dir else None)
+            res[key] = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            line = [t for t in r.stderr.splitlines() if t.startswith("rajzel_amd line search:")][-1]
+            res.setdefault("grad_obj_calls", int(line.split()[3]))
+            outs.append(digests(ws))
+            if key == "resident_s":
+                with open(os.path.join(ws, "misfit.dat")) as fh:
+                    res["misfit_log"] = [float(l.split()[1]) for l in fh if l.strip()]
+            print(f"[step] {key} {res[key]:.1f} s", flush=True)
+            remove(ws)
+        res["iterations_logged"] = len(res["misfit_log"])
+        res["files"] = len(outs[0])
+        res["outputs_identical"] = outs[0] == outs[1]
+    finally:
+        for ws in spaces:
+            remove(ws)
+    return res
+
+
 if __name__ == "__main__":
     import json
     import sys
@@ -290,6 +373,8 @@ if __name__ == "__main__":
         print(json.dumps(time_fatt_picks()), flush=True)
     elif sys.argv[1:] == ["--time-fatt-c4"]:
         print(json.dumps(time_fatt_c4()), flush=True)
+    elif sys.argv[1:2] == ["--time-fatt-c5"]:
+        print(json.dumps(time_fatt_c5(log_dir=sys.argv[2] if len(sys.argv) > 2 else None)), flush=True)
     elif sys.argv[1:2] == ["--time-line-search"]:
         ls = int(sys.argv[2]) if len(sys.argv) > 2 else 2
         wl = sys.argv[3] if len(sys.argv) > 3 else "c2"
@@ -297,4 +382,4 @@ if __name__ == "__main__":
               flush=True)
     else:
         sys.exit("usage: python -m tests.fd_program --time-grid-search | --time-fatt-picks | "
-                 "--time-line-search [2|1] [c2|c4_<shots>] | --time-fatt-c4")
+                 "--time-line-search [2|1] [c2|c4_<shots>] | --time-fatt-c4 | --time-fatt-c5 [log dir]")

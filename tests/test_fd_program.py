@@ -105,6 +105,25 @@ def test_forward_program_with_snapshots():
 
 
 @pytest.mark.gpu
+def test_programs_under_an_hbm_budget():
+    """RAJZEL_AMD_HBM_MIB=1 leaves room for the fields of some of 16 shots:
+    the forward with snapshots (the shots outside the last batch solved
+    again one at a time) and three l-BFGS iterations run in shot batches and
+    write the reference's files byte for byte."""
+    from rajzel_amd import native as N
+    sv = P.survey(nshots=16)
+    with N.Context([dict(nx=sv.nx, ny=sv.ny, dh=sv.dh, src_ix=sv.src_ix, src_iy=sv.src_iy, rec_ix=sv.rec_ix,
+                         rec_iy=sv.rec_iy)], hbm_budget=1 << 20) as ctx:
+        assert 1 <= ctx.batch_shots < sv.nshots, (ctx.batch_shots, ctx.slot_bytes)
+    env = {"RAJZEL_AMD_HBM_MIB": "1"}
+    out = _compare(lambda: P.make_workspace(sv, sv.vp_true, invmat=0, SNAP=1), env=env)
+    assert sum(k.startswith("tt/snap_shot_") for k in out) == sv.nshots
+    obs = _observed(sv)
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=1, picks=obs, ITERMAX=3), env=env)
+    assert len([m for m in out["misfit.dat"].decode().split("\n") if m.strip()]) == 3
+
+
+@pytest.mark.gpu
 def test_fatt_lbfgs_wolfe():
     sv = P.survey()
     obs = _observed(sv)
