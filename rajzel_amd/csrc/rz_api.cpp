@@ -522,7 +522,9 @@ int plan_mixed(rz_ctx *c, int per)
 {
     const int T = c->total_shots;
     c->plan_on = false;
-    if (!c->split || c->split_shots == 0 || per < T || T <= 0 || 2 * T <= c->ncu || T >= c->ncu) return RZ_OK;
+    if (!c->split || c->split_shots == 0 || per < T || T <= 0 || 2 * T <= c->ncu || T >= c->ncu ||
+        !paired_for(c->R, c->nb_max))
+        return RZ_OK;
     if ((int)c->hist_f.size() != T || (int)c->hist_a.size() != T) return RZ_OK;
     for (int i = 0; i < T; ++i)
         if (c->hist_f[i] < 0 || c->hist_a[i] < 0) return RZ_OK;
@@ -586,10 +588,11 @@ int plan_mixed(rz_ctx *c, int per)
 // Split shots (two workgroups each) while every workgroup can have a CU of
 // its own: at most one shot per two CUs.  With more shots the workgroups
 // would share CUs (two fit one), which measured slower than one workgroup
-// per shot.
+// per shot.  Unpaired grids (more bands than a pair holds) have no pair to
+// split.
 bool use_split(const rz_ctx *c, int nshots)
 {
-    return c->split && c->split_shots != 0 && nshots > 0 && 2 * nshots <= c->ncu;
+    return c->split && c->split_shots != 0 && nshots > 0 && 2 * nshots <= c->ncu && paired_for(c->R, c->nb_max);
 }
 
 // Enqueue the launches of one batch of shots [b0, b1) on the work stream:
@@ -844,9 +847,9 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     c->R = rows_per_lane ? rows_per_lane : choose_R(ny_max);
     if (c->R != 1 && c->R != 2 && c->R != 4 && c->R != 8)
         return bail(fail(c, RZ_ERR_INVALID, "rows_per_lane must be 1, 2, 4 or 8"));
-    if (bands_for(ny_max, c->R) > max_bands_for(c->R))
+    if (bands_for(ny_max, c->R) > max_bands_unpaired(c->R))
         return bail(fail(c, RZ_ERR_INVALID, "NY=%d needs more than %d bands at %d rows per lane", ny_max,
-                         max_bands_for(c->R), c->R));
+                         max_bands_unpaired(c->R), c->R));
 
     long long rec_off = 0, cells = 0;
     int shots = 0, carry = 0;

@@ -33,6 +33,11 @@ constexpr int kMaxBands = 8;          // bands (waves) per group
 // Bands a group may hold at R rows per lane: 8 for R <= 2, 4 above (with
 // two groups: 16 resp. 8 waves, a budget of 128 resp. 256 VGPRs per lane).
 constexpr int max_bands_for(int R) { return R <= 2 ? 8 : 4; }
+// Taller grids at R = 8 (5 to 8 bands: 2049 to 4096 rows) run unpaired: all
+// 8 waves form one group, one wave per band, and the four sweeps of an
+// iteration run one after the other.
+constexpr int max_bands_unpaired(int R) { return R == 8 ? 8 : max_bands_for(R); }
+constexpr bool paired_for(int R, int nb) { return nb <= max_bands_for(R); }
 // Prefetch slots of a sweep: a step's blocks are loaded this many steps
 // before it (one less in the adjoint sweep).  At R = 8 a step holds twice
 // the rows of one at R = 4, so four slots keep the bytes in flight per wave

@@ -1483,10 +1483,11 @@ __device__ void mark_kept_cells(const ModelDev &md, float *TA, float *TB)
 // Wave context of a two-group workgroup: waves [0, nbg) form group 0 (the
 // leading sweep of a pair), waves [nbg, 2 nbg) group 1 (the trailing one);
 // within a group, wave b handles band b.  Each group has its own LDS ring.
-__device__ __forceinline__ WaveCtx make_wave(const ModelDev &md, float *ring_all)
+// An unpaired workgroup (groups = 1) is group 0 alone.
+__device__ __forceinline__ WaveCtx make_wave(const ModelDev &md, float *ring_all, int groups)
 {
     WaveCtx w;
-    const int nbg = blockDim.x >> 7;                                     // bands per group
+    const int nbg = (blockDim.x >> 6) / groups;                          // bands per group
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     w.nx = md.nx; w.ny = md.ny; w.nb = md.nb;
     w.group = wave / nbg;
@@ -1498,18 +1499,19 @@ __device__ __forceinline__ WaveCtx make_wave(const ModelDev &md, float *ring_all
     return w;
 }
 
-template <int R>
+template <int R, bool PAIRED>
 __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
     const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, Slots slots,
     const long long *__restrict__ rec_off, const int *__restrict__ shot_base, ShotOut out)
 {
+    static_assert(PAIRED || max_bands_unpaired(R) <= kGroups * max_bands_for(R), "unpaired bands fit the block");
     __shared__ float ring[kGroups * (kMaxBands * kRingQ + kWave)];
     __shared__ double red[kGroups * kMaxBands];
     __shared__ float stage_all[kGroups * max_bands_for(R) * Stage<R>::WAVE_FLOATS];
     const ShotDev sh = shots[blockIdx.x];
     const ModelDev md = mds[sh.model];
     const int nwaves = blockDim.x >> 6;
-    const WaveCtx w = make_wave(md, ring);
+    const WaveCtx w = make_wave(md, ring, PAIRED ? kGroups : 1);
     float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
     const int nep = sweep_steps(md.nx, md.nb, Stage<R>::HOLD) / kPF;
     const int D = pair_delay(md.nb, Stage<R>::HOLD);
@@ -1526,12 +1528,27 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
     // simply dropped if the loop ends.
     int it = 0, last = -1, nord = 0;
     float osum = -1.f;   // ordered sum of the last decision, if it took one
-    Dec acc_next{0.0, 0.0};  // group 1: the sweep 1 run ahead
+    [[maybe_unused]] Dec acc_next{0.0, 0.0};  // group 1: the sweep 1 run ahead
     while (cont) {
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
         float *Wo = W[it & 1];
         const float *Wi = it > 0 ? W[(it - 1) & 1] : nullptr;   // T at the start (null: the initial field)
         Dec acc{0.0, 0.0};
+        if constexpr (!PAIRED) {
+            // one group: the four sweeps of the iteration one after the other
+            acc = it == 0 ? fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k,
+                                                       sh.src_c, tinit, lockstep(0, 0))
+                          : fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, Wi, md.SA, TB, stage, sh.src_k,
+                                                        sh.src_c, tinit, lockstep(0, 0));
+            const Dec a2 = fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k,
+                                                       sh.src_c, tinit, lockstep(0, 0));
+            const Dec a3 = fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k,
+                                                       sh.src_c, tinit, lockstep(0, 0));
+            const Dec a4 = fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k,
+                                                       sh.src_c, tinit, lockstep(0, 0));
+            acc.sum += a2.sum + a3.sum + a4.sum;
+            acc.cnt += a2.cnt + a3.cnt + a4.cnt;
+        } else {
         if (it == 0) {
             if (w.group == 0)
                 acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
@@ -1557,6 +1574,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         } else {
             acc_next = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k, sh.src_c,
                                                    tinit, lockstep(D, 0));
+        }
         }
         const Dec tot = block_sum2(acc, red, nwaves);
         last = it++;
@@ -1599,12 +1617,13 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
 // Adjoint kernel: one workgroup per shot (eikonal_adj.c:10-59 with
 // init_travel_adj.c as an O(ntr) scatter).
 // ---------------------------------------------------------------------------
-template <int R, int FD, bool CHKINF>
+template <int R, int FD, bool CHKINF, bool PAIRED>
 __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
     const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, Slots slots,
     const long long *__restrict__ rec_off, const int *__restrict__ shot_base, const float *__restrict__ tres,
     ShotOut out)
 {
+    static_assert(PAIRED || max_bands_unpaired(R) <= kGroups * max_bands_for(R), "unpaired bands fit the block");
     constexpr int BR = kWave * R;
     __shared__ float ring[kGroups * (kMaxBands * kRingQ + kWave)];
     __shared__ double red[kGroups * kMaxBands];
@@ -1612,7 +1631,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
     const ShotDev sh = shots[blockIdx.x];
     const ModelDev md = mds[sh.model];
     const int nwaves = blockDim.x >> 6;
-    const WaveCtx w = make_wave(md, ring);
+    const WaveCtx w = make_wave(md, ring, PAIRED ? kGroups : 1);
     float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
     const int D = pair_delay(md.nb, Stage<R>::HOLD);
     const long long so = (long long)sh.slot * slots.slot_stride;
@@ -1649,7 +1668,12 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
         // reads B and writes A, sweep 4 updates A in place; sweeps 1+2 and
         // 3+4 run as pairs (same x direction).
         double acc;
-        if (w.group == 0) {
+        if constexpr (!PAIRED) {   // one group: the four sweeps one after the other
+            acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wi, LB, stage, lockstep(0, 0));
+            acc += adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LB, stage, lockstep(0, 0));
+            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, Wo, stage, lockstep(0, 0));
+            acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wo, Wo, stage, lockstep(0, 0));
+        } else if (w.group == 0) {
             acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wi, LB, stage, lockstep(0, D));
             acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, Wo, stage, lockstep(0, D));
         } else {
@@ -2250,8 +2274,14 @@ hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *
                           const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st)
 {
     if (nshots <= 0) return hipSuccess;
-    RZ_DISPATCH_R(R, hipLaunchKernelGGL(k_forward<RR>, dim3(nshots), dim3(kGroups * nb_max * kWave), 0, st, md,
-                                        shots, slots, rec_off, shot_base, out));
+    if (!paired_for(R, nb_max)) {   // unpaired: only at R = 8, one wave per band
+        if (R != 8 || nb_max > max_bands_unpaired(R)) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_forward<8, false>), dim3(nshots), dim3(nb_max * kWave), 0, st, md, shots, slots, rec_off,
+                           shot_base, out);
+        return hipGetLastError();
+    }
+    RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_forward<RR, true>), dim3(nshots), dim3(kGroups * nb_max * kWave), 0, st,
+                                        md, shots, slots, rec_off, shot_base, out));
     return hipGetLastError();
 }
 
@@ -2301,10 +2331,18 @@ hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *
                           int variant, hipStream_t st)
 {
     if (nshots <= 0) return hipSuccess;
-    const dim3 g(nshots), b(kGroups * nb_max * kWave);
-#define RZ_ADJ_LAUNCH(FD_, CI_)                                                                    \
-    RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, FD_, CI_>), g, b, 0, st, md, shots, slots, \
-                                        rec_off, shot_base, tres, out))
+    const bool paired = paired_for(R, nb_max);
+    if (!paired && (R != 8 || nb_max > max_bands_unpaired(R))) return hipErrorInvalidValue;
+    const dim3 g(nshots), b((paired ? kGroups : 1) * nb_max * kWave);
+    // unpaired: only at R = 8, one wave per band
+#define RZ_ADJ_LAUNCH(FD_, CI_)                                                                         \
+    if (!paired) {                                                                                      \
+        hipLaunchKernelGGL((k_adjoint<8, FD_, CI_, false>), g, b, 0, st, md, shots, slots, rec_off,    \
+                           shot_base, tres, out);                                                       \
+    } else {                                                                                            \
+        RZ_DISPATCH_R(R, hipLaunchKernelGGL((k_adjoint<RR, FD_, CI_, true>), g, b, 0, st, md, shots,   \
+                                            slots, rec_off, shot_base, tres, out))                      \
+    }
     switch ((variant & kAdjFastDiv) ? ((variant & kAdjFloatT) ? 2 : 1) : 0) {   // DH division mode
     case 0:
         if (variant & kAdjInfGuard) { RZ_ADJ_LAUNCH(0, true); } else { RZ_ADJ_LAUNCH(0, false); }

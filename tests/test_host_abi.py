@@ -52,6 +52,7 @@ def _model(**kw):
     (dict(rec_iy=[1, 15]), "outside the grid"),
     (dict(time=float("inf")), "must be finite"),
     (dict(ny=10000), "bands"),
+    (dict(ny=4097), "bands"),                    # 9 bands of 512 rows: one more than a workgroup holds
     (dict(nx=9000000, ny=5), "exceeds 2 GiB"),   # 45M cells, but one band of 64 rows: 2.3 GB per field
 ])
 def test_invalid_models_rejected_before_device(kw, msg):
