@@ -1816,7 +1816,6 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
     if (!pair_handshake(cnt, role, ss.err)) return;
     const WaveCtx w = make_wave_split(md, ring, role);
     float *stage = stage_all + (threadIdx.x >> 6) * Stage<R>::WAVE_FLOATS;
-    const int nep = sweep_steps(md.nx, md.nb, Stage<R>::HOLD) / kPF;
     Pace pc;
     pc.pre = pc.post = 0; pc.lead = cnt + kSyncLead; pc.err = ss.err; pc.base = 0;
     pc.D = pair_delay(md.nb, Stage<R>::HOLD);
