@@ -79,7 +79,11 @@ int rz_grid_index(float pos_m, float dh);
  * On failure *out is NULL and rz_last_error(NULL) gives the reason.
  *   rows_per_lane  1, 2, 4 or 8 rows of a band per lane; 0 = chosen by
  *                  the band count; RZ_ROWS_AUTOTUNE = measured by
- *                  rz_autotune_rows()
+ *                  rz_autotune_rows().  A band is 64*R rows; a grid may
+ *                  have at most 8 bands at R <= 2, 4 at R = 4 and 8 at
+ *                  R = 8 (NY <= 512, 1024, 1024, 4096).  Above 4 bands
+ *                  at R = 8 the four sweeps of an iteration run one after
+ *                  the other instead of two at a time.
  *   hbm_budget     bytes of device memory for per-shot fields; 0 = enough
  *                  for every shot at once when that fits the device's free
  *                  memory less 1/16 of the device, else what does fit.
