@@ -83,7 +83,10 @@ int rz_grid_index(float pos_m, float dh);
  *                  have at most 8 bands at R <= 2, 4 at R = 4 and 8 at
  *                  R = 8 (NY <= 512, 1024, 1024, 4096).  Above 4 bands
  *                  at R = 8 the four sweeps of an iteration run one after
- *                  the other instead of two at a time.
+ *                  the other instead of two at a time.  A grid with
+ *                  NY > 4096 and NX <= 4096 is stored and solved
+ *                  transposed (R = 8 only; fields, gradients and receiver
+ *                  times keep the grid's orientation and values).
  *   hbm_budget     bytes of device memory for per-shot fields; 0 = enough
  *                  for every shot at once when that fits the device's free
  *                  memory less 1/16 of the device, else what does fit.

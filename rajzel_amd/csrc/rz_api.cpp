@@ -32,6 +32,10 @@ struct ModelHost {
     int carry_off = 0;       // offset in the carry buffer
     bool have_S = false, have_picks = false, have_res = false;
     bool fastdiv = false;
+    // stored transposed (NY above the rows a workgroup holds, NX within
+    // them): d.nx and d.ny, the sources' and receivers' rows (k) and columns
+    // (c) are those of the transposed grid
+    bool trans = false;
     bool ft_dh = false;     // the float FMA quotient is exact for this DH (ft_div)
     bool ft_s = false;      // ... and the slowness keeps its inputs clear of underflow
     float rdh_hi = 0.f, rdh_lo = 0.f, s_min_ft = 0.f;
@@ -58,6 +62,7 @@ struct rz_ctx {
     int device = 0;
     hipStream_t own = nullptr, stream = nullptr;
     int R = 4, nb_max = 1;
+    bool paired = true;    // two sweeps at a time; false: tall or transposed grids (R = 8)
     bool fastdiv = true;
     bool chkinf = false;   // INF guard of sweep_adj.c:68 can fire only if 2*TIME >= INF
     bool ft = false;       // every model allows the float T-only quotients (adjoint variant 2)
@@ -198,6 +203,17 @@ constexpr size_t kTaperMaxEntries = (size_t)1 << 28;
     } while (0)
 
 int bands_for(int ny, int R) { return (ny + kWave * R - 1) / (kWave * R); }
+
+// Rows a workgroup holds (8 bands of 512 rows, R = 8).  A taller grid whose
+// NX fits them is stored and solved transposed: the Godunov and adjoint
+// updates treat x and y alike, so with the sweep directions transposed too
+// (and norm1's sum still taken in the grid's order) every value is the same.
+constexpr int kMaxRowsStored = kWave * 8 * max_bands_unpaired(8);
+bool transposed(const rz_model_desc &d) { return d.ny > kMaxRowsStored && d.nx <= kMaxRowsStored; }
+
+// NX and NY of the grid as the caller sees it
+int grid_nx(const ModelHost &h) { return h.trans ? h.d.ny : h.d.nx; }
+int grid_ny(const ModelHost &h) { return h.trans ? h.d.nx : h.d.ny; }
 
 // The adjoint divides by DH as (float)(x * RN64(1/DH)).  That equals the IEEE
 // quotient x/DH for every float x (a float quotient is never within 2^-49
@@ -374,6 +390,7 @@ ModelDev make_dev(const rz_ctx *c, const ModelHost &h)
     d.rec_k = h.rec_k_d; d.rec_c = h.rec_c_d;
     d.uq_k = h.uq_k_d; d.uq_c = h.uq_c_d; d.uq_r = h.uq_r_d;
     d.max_iter = h.d.max_iter > 0 ? h.d.max_iter : 1000;
+    d.trans = h.trans ? 1 : 0;
     d.rdh_hi = h.rdh_hi; d.rdh_lo = h.rdh_lo; d.s_min_ft = h.s_min_ft;
     d.stop_s = h.stop_s; d.band_lo = h.band_lo; d.band_hi = h.band_hi; d.stop_mode = h.stop_mode;
     return d;
@@ -522,8 +539,7 @@ int plan_mixed(rz_ctx *c, int per)
 {
     const int T = c->total_shots;
     c->plan_on = false;
-    if (!c->split || c->split_shots == 0 || per < T || T <= 0 || 2 * T <= c->ncu || T >= c->ncu ||
-        !paired_for(c->R, c->nb_max))
+    if (!c->split || c->split_shots == 0 || per < T || T <= 0 || 2 * T <= c->ncu || T >= c->ncu || !c->paired)
         return RZ_OK;
     if ((int)c->hist_f.size() != T || (int)c->hist_a.size() != T) return RZ_OK;
     for (int i = 0; i < T; ++i)
@@ -588,11 +604,11 @@ int plan_mixed(rz_ctx *c, int per)
 // Split shots (two workgroups each) while every workgroup can have a CU of
 // its own: at most one shot per two CUs.  With more shots the workgroups
 // would share CUs (two fit one), which measured slower than one workgroup
-// per shot.  Unpaired grids (more bands than a pair holds) have no pair to
-// split.
+// per shot.  Unpaired grids (more bands than a pair holds, or transposed)
+// have no pair to split.
 bool use_split(const rz_ctx *c, int nshots)
 {
-    return c->split && c->split_shots != 0 && nshots > 0 && 2 * nshots <= c->ncu && paired_for(c->R, c->nb_max);
+    return c->split && c->split_shots != 0 && nshots > 0 && 2 * nshots <= c->ncu && c->paired;
 }
 
 // Enqueue the launches of one batch of shots [b0, b1) on the work stream:
@@ -612,7 +628,7 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
         TRY(fork_side(c));
         HIPCHK(c, launch_forward_split(c->R, c->nb_max, c->md_d, c->plan_d, c->plan_n[0], sl, c->rec_off_d,
                                        c->shot_base_d, so, c->ss, c->side));
-        HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->plan_d + T, c->plan_n[1], sl, c->rec_off_d,
+        HIPCHK(c, launch_forward(c->R, c->nb_max, c->paired, c->md_d, c->plan_d + T, c->plan_n[1], sl, c->rec_off_d,
                                  c->shot_base_d, so, c->stream));
         TRY(join_side(c));
     } else if (!reuse) {
@@ -620,7 +636,7 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
             HIPCHK(c, launch_forward_split(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
                                            c->shot_base_d, so, c->ss, c->stream));
         else
-            HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+            HIPCHK(c, launch_forward(c->R, c->nb_max, c->paired, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
                                      c->shot_base_d, so, c->stream));
     }
     TRY(mark(c, &m.e[1]));
@@ -640,14 +656,14 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
         TRY(fork_side(c));
         HIPCHK(c, launch_adjoint_split(c->R, c->nb_max, c->md_d, c->plan_d + 2 * T, c->plan_n[2], sl, c->rec_off_d,
                                        c->shot_base_d, c->tres_d, so, adj_variant(c), c->ss, c->side));
-        HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->plan_d + 3 * T, c->plan_n[3], sl, c->rec_off_d,
+        HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->paired, c->md_d, c->plan_d + 3 * T, c->plan_n[3], sl, c->rec_off_d,
                                  c->shot_base_d, c->tres_d, so, adj_variant(c), c->stream));
         TRY(join_side(c));
     } else if (split)
         HIPCHK(c, launch_adjoint_split(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
                                        c->shot_base_d, c->tres_d, so, adj_variant(c), c->ss, c->stream));
     else
-        HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
+        HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->paired, c->md_d, c->shots_d + b0, b1 - b0, sl, c->rec_off_d,
                                  c->shot_base_d, c->tres_d, so, adj_variant(c), c->stream));
     TRY(mark(c, &m.e[3]));
     for (size_t mi = 0; mi < c->m.size(); ++mi) {
@@ -663,7 +679,8 @@ int enqueue_batch(rz_ctx *c, bool grad, bool direct, bool reuse, int b0, int b1,
     }
     if (b1 == c->total_shots)   // grad_obj.c:70: with a per-shot taper the sum's frame is copied again
         for (const ModelHost &h : c->m)
-            if (h.tap_mode != 0) HIPCHK(c, launch_grad_frame(h.d.nx, h.d.ny, c->grad_d + h.grad_off, c->stream));
+            if (h.tap_mode != 0)
+                HIPCHK(c, launch_grad_frame(grid_nx(h), grid_ny(h), c->grad_d + h.grad_off, c->stream));
     TRY(mark(c, &m.e[4]));
     return RZ_OK;
 }
@@ -773,6 +790,16 @@ int copy_out(rz_ctx *c, void *dst, const void *src, size_t bytes, int on_device)
     return RZ_OK;
 }
 
+// A row-major field of model h as stored (f): in the caller's orientation,
+// transposed into s_row2 if the grid is stored transposed (null if that
+// launch fails).
+const float *to_grid(rz_ctx *c, const ModelHost &h, const float *f)
+{
+    if (!h.trans) return f;
+    if (launch_transpose(f, c->s_row2, h.d.ny, h.d.nx, c->stream) != hipSuccess) return nullptr;
+    return c->s_row2;
+}
+
 }  // namespace
 
 extern "C" {
@@ -825,7 +852,8 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     };
     c->device = device;
     if (nmodels <= 0 || !models) return bail(fail(c, RZ_ERR_INVALID, "need at least one model"));
-    int ny_max = 0;
+    int ny_max = 0;          // rows of the tallest grid as stored
+    bool any_trans = false;
     for (int i = 0; i < nmodels; ++i) {
         const rz_model_desc &d = models[i];
         if (d.nx < 5 || d.ny < 5) return bail(fail(c, RZ_ERR_INVALID, "model %d: NX, NY must be >= 5", i));
@@ -836,7 +864,8 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
             return bail(fail(c, RZ_ERR_INVALID, "model %d: negative count", i));
         if ((long long)d.nx * d.ny > (1LL << 28))
             return bail(fail(c, RZ_ERR_INVALID, "model %d: grid too large", i));
-        ny_max = std::max(ny_max, d.ny);
+        ny_max = std::max(ny_max, transposed(d) ? d.nx : d.ny);
+        any_trans = any_trans || transposed(d);
     }
     if (rows_per_lane == RZ_ROWS_AUTOTUNE) {
         int best = 0;
@@ -844,9 +873,12 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
             return bail(fail(c, RZ_ERR_INVALID, "autotune: %s", g_create_err.c_str()));
         rows_per_lane = best;
     }
-    c->R = rows_per_lane ? rows_per_lane : choose_R(ny_max);
+    c->R = rows_per_lane ? rows_per_lane : (any_trans ? 8 : choose_R(ny_max));
     if (c->R != 1 && c->R != 2 && c->R != 4 && c->R != 8)
         return bail(fail(c, RZ_ERR_INVALID, "rows_per_lane must be 1, 2, 4 or 8"));
+    if (any_trans && c->R != 8)
+        return bail(fail(c, RZ_ERR_INVALID, "NY above %d is solved transposed, at 8 rows per lane only (not %d)",
+                         kMaxRowsStored, c->R));
     if (bands_for(ny_max, c->R) > max_bands_unpaired(c->R))
         return bail(fail(c, RZ_ERR_INVALID, "NY=%d needs more than %d bands at %d rows per lane", ny_max,
                          max_bands_unpaired(c->R), c->R));
@@ -859,21 +891,26 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         ModelHost h;
         h.d = d;
         h.d.src_ix = h.d.src_iy = h.d.rec_ix = h.d.rec_iy = nullptr;
-        h.nb = bands_for(d.ny, c->R);
-        h.bstride = (long long)(d.nx + 64) * kWave * c->R;   // keys 0..nx+62 + one dummy block
+        h.trans = transposed(d);
+        if (h.trans) std::swap(h.d.nx, h.d.ny);
+        const int nx = h.d.nx, ny = h.d.ny;   // as stored
+        h.nb = bands_for(ny, c->R);
+        h.bstride = (long long)(nx + 64) * kWave * c->R;   // keys 0..nx+62 + one dummy block
         field_max = std::max(field_max, (long long)h.nb * h.bstride);
         // the adjoint reads a field through one buffer descriptor, whose
         // byte range is a 31-bit count
         if (4 * (long long)h.nb * h.bstride > 0x7fffffffLL)
             return bail(fail(c, RZ_ERR_INVALID, "model %d: a %d x %d field exceeds 2 GiB in the band layout", i,
                              d.nx, d.ny));
+        // a cell (x, y) of the grid is (row y, column x) as stored, or (row
+        // x, column y) transposed
         for (int s = 0; s < d.nshots; ++s) {
             const int x = d.src_ix[s], y = d.src_iy[s];
             if (x < 0 || x >= d.nx || y < 0 || y >= d.ny)
                 return bail(fail(c, RZ_ERR_INVALID, "model %d shot %d: source cell (%d,%d) outside the grid", i, s,
                                  x, y));
-            h.src_c.push_back(x);
-            h.src_k.push_back(y);
+            h.src_c.push_back(h.trans ? y : x);
+            h.src_k.push_back(h.trans ? x : y);
         }
         std::unordered_map<long long, int> last;
         for (int r = 0; r < d.ntr; ++r) {
@@ -881,16 +918,17 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
             if (x < 0 || x >= d.nx || y < 0 || y >= d.ny)
                 return bail(fail(c, RZ_ERR_INVALID, "model %d receiver %d: cell (%d,%d) outside the grid", i, r,
                                  x, y));
-            h.rec_c.push_back(x);
-            h.rec_k.push_back(y);
-            if (x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1) last[(long long)y * d.nx + x] = r;
+            const int kk = h.trans ? x : y, cc = h.trans ? y : x;
+            h.rec_c.push_back(cc);
+            h.rec_k.push_back(kk);
+            if (cc > 0 && cc < nx - 1 && kk > 0 && kk < ny - 1) last[(long long)kk * nx + cc] = r;
         }
         // interior receiver cells, last receiver wins (init_travel_adj.c:49-55)
         std::vector<std::pair<long long, int>> uq(last.begin(), last.end());
         std::sort(uq.begin(), uq.end());
         for (auto &p : uq) {
-            h.uq_k.push_back((int)(p.first / d.nx));
-            h.uq_c.push_back((int)(p.first % d.nx));
+            h.uq_k.push_back((int)(p.first / nx));
+            h.uq_c.push_back((int)(p.first % nx));
             h.uq_r.push_back(p.second);
         }
         set_stop(h);
@@ -911,6 +949,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         c->nb_max = std::max(c->nb_max, h.nb);
         c->m.push_back(std::move(h));
     }
+    c->paired = paired_for(c->R, c->nb_max) && !any_trans;
     c->total_shots = shots;
     c->total_rec = rec_off;
     c->total_cells = cells;
@@ -1077,8 +1116,13 @@ int rz_set_slowness(rz_ctx *c, int model, const float *S, int on_device)
     if (!S) return fail(c, RZ_ERR_INVALID, "null slowness");
     ModelHost &h = c->m[model];
     const size_t n = (size_t)h.d.nx * h.d.ny;
-    HIPCHK(c, hipMemcpyAsync(h.S, S, n * sizeof(float), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                             c->stream));
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (h.trans) {   // the caller's NY x NX rows become the stored grid's columns
+        HIPCHK(c, hipMemcpyAsync(c->s_row2, S, n * sizeof(float), kind, c->stream));
+        HIPCHK(c, launch_transpose(c->s_row2, h.S, grid_ny(h), grid_nx(h), c->stream));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(h.S, S, n * sizeof(float), kind, c->stream));
+    }
     HIPCHK(c, hipMemsetAsync(c->err_d, 0, sizeof(int), c->stream));
     ModelDev mh = make_dev(c, h);
     HIPCHK(c, launch_prepare_slowness(&mh, model, c->md_d, h.SA, h.SB, c->err_d, c->stream));
@@ -1270,7 +1314,7 @@ int rz_get_field(rz_ctx *c, int model, int shot, int which, float *out, int on_d
         HIPCHK(c, launch_field_to_rowmajor(&mh, c->TA + so, nullptr, nullptr, c->s_row, 1, c->stream));
     else
         HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA + so, c->LC + so, c->lsel_d + g, c->s_row, 0, c->stream));
-    return copy_out(c, out, c->s_row, n * sizeof(float), on_device);
+    return copy_out(c, out, to_grid(c, h, c->s_row), n * sizeof(float), on_device);
 }
 
 int rz_eikonal_shot(rz_ctx *c, int model, int src_ix, int src_iy, float *T, float *tmod, int *iters)
@@ -1278,22 +1322,24 @@ int rz_eikonal_shot(rz_ctx *c, int model, int src_ix, int src_iy, float *T, floa
     TRY(check_model(c, model));
     ModelHost &h = c->m[model];
     if (!h.have_S) return fail(c, RZ_ERR_STATE, "slowness not set");
-    if (src_ix < 0 || src_ix >= h.d.nx || src_iy < 0 || src_iy >= h.d.ny)
+    if (src_ix < 0 || src_ix >= grid_nx(h) || src_iy < 0 || src_iy >= grid_ny(h))
         return fail(c, RZ_ERR_INVALID, "source cell (%d,%d) outside the grid", src_ix, src_iy);
     ShotDev sd{};
-    sd.model = model; sd.shot = 0; sd.src_k = src_iy; sd.src_c = src_ix; sd.slot = 0;
+    sd.model = model; sd.shot = 0; sd.slot = 0;
+    sd.src_k = h.trans ? src_ix : src_iy;
+    sd.src_c = h.trans ? src_iy : src_ix;
     HIPCHK(c, hipMemcpyAsync(c->s_shot, &sd, sizeof sd, hipMemcpyHostToDevice, c->stream));
     ShotOut o;
     o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1; o.lsel = c->s_it + 2;
     o.ord_f = c->s_it + 3; o.ord_a = c->s_it + 4;
     o.osum_f = c->s_osum; o.osum_a = c->s_osum + 1;
-    HIPCHK(c, launch_forward(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d, o,
+    HIPCHK(c, launch_forward(c->R, c->nb_max, c->paired, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d, o,
                              c->stream));
     ModelDev mh = make_dev(c, h);
     HIPCHK(c, launch_field_to_rowmajor(&mh, c->TA, nullptr, nullptr, c->s_row, 1, c->stream));
     int it = 0;
     const size_t n = (size_t)h.d.nx * h.d.ny;
-    if (T) HIPCHK(c, hipMemcpyAsync(T, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (T) HIPCHK(c, hipMemcpyAsync(T, to_grid(c, h, c->s_row), n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (tmod && h.d.ntr)
         HIPCHK(c, hipMemcpyAsync(tmod, c->s_tmod, h.d.ntr * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(&it, c->s_it, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1311,6 +1357,14 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     ModelHost &h = c->m[model];
     if (!T) return fail(c, RZ_ERR_INVALID, "null T");
     const size_t n = (size_t)h.d.nx * h.d.ny;
+    std::vector<float> Tt;   // T as stored, for a grid stored transposed
+    if (h.trans) {
+        const int gx = grid_nx(h), gy = grid_ny(h);
+        Tt.resize(n);
+        for (int y = 0; y < gy; ++y)
+            for (int x = 0; x < gx; ++x) Tt[(size_t)x * gy + y] = T[(size_t)y * gx + x];
+        T = Tt.data();
+    }
     bool big = false;
     float tmin = INFINITY, tmax = 0.f;   // smallest nonzero and largest T (float T-only quotients)
     for (size_t i = 0; i < n; ++i) {
@@ -1350,11 +1404,12 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1; o.lsel = c->s_it + 2;
     o.ord_f = c->s_it + 3; o.ord_a = c->s_it + 4;
     o.osum_f = c->s_osum; o.osum_a = c->s_osum + 1;
-    HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d,
+    HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->paired, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d,
                              c->s_tres, o, variant, c->stream));
     HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA, c->LC, c->s_it + 2, c->s_row, 0, c->stream));
     int it = 0;
-    if (lam) HIPCHK(c, hipMemcpyAsync(lam, c->s_row, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (lam)
+        HIPCHK(c, hipMemcpyAsync(lam, to_grid(c, h, c->s_row), n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(&it, c->s_it + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->last_b0 = c->last_b1 = -1;
@@ -1374,7 +1429,12 @@ static int ass_grad_shot(rz_ctx *c, int model, int src, const float *lam, float 
     if (!lam) return fail(c, RZ_ERR_INVALID, "null lambda");
     const size_t n = (size_t)h.d.nx * h.d.ny;
     ModelDev mh = make_dev(c, h);
-    HIPCHK(c, hipMemcpyAsync(c->s_row, lam, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (h.trans) {   // lambda as stored; the gradient comes out in the caller's orientation
+        HIPCHK(c, hipMemcpyAsync(c->s_row2, lam, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_transpose(c->s_row2, c->s_row, grid_ny(h), grid_nx(h), c->stream));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->s_row, lam, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(c, launch_rowmajor_to_field(&mh, c->s_row, c->LA, 0, c->stream));
     // slot 0, lambda in LA: s_slot0 holds the 0 for both
     HIPCHK(c, launch_gradient(&mh, model, c->md_d, c->s_slot0, c->s_slot0, 1, slots_of(c), c->s_row2, 0,
@@ -1423,8 +1483,10 @@ int rz_set_shot_taper(rz_ctx *c, int model, int mode, float radius_m, int filtsi
         return RZ_OK;   // unchanged geometry: keep the tables
     TaperTables t;
     if (mode != 0) {
-        const std::string why = build_taper(mode, h.d.nx, h.d.ny, h.d.dh, radius_m, filtsize, nsrc, sx.data(),
-                                            sz.data(), h.d.ntr, h.rec_c.data(), h.rec_k.data(), kTaperMaxEntries, t);
+        // in the grid's orientation, as k_gradient applies the tables
+        const std::string why = build_taper(mode, grid_nx(h), grid_ny(h), h.d.dh, radius_m, filtsize, nsrc, sx.data(),
+                                            sz.data(), h.d.ntr, h.trans ? h.rec_k.data() : h.rec_c.data(),
+                                            h.trans ? h.rec_c.data() : h.rec_k.data(), kTaperMaxEntries, t);
         if (!why.empty()) return fail(c, RZ_ERR_INVALID, "per-shot taper: %s", why.c_str());
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));

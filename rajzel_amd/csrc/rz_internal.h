@@ -62,7 +62,7 @@ struct ModelDev {
     const int *rec_k, *rec_c;      // receivers, 0-based
     const int *uq_k, *uq_c, *uq_r; // interior receiver cells and their winning receiver
     int max_iter;
-    int pad_;
+    int trans;                     // stored transposed: rows here are the grid's columns (x), columns its rows
     // T-only quotients of the adjoint as fma(x, rdh_hi, x * rdh_lo) in float
     // (adjoint variant 2); slowness below s_min_ft turns that variant off.
     float rdh_hi, rdh_lo;
@@ -122,11 +122,15 @@ struct SplitSync {
 // Launchers (rz_kernels.hip).  All return hipError_t of the launch.
 hipError_t launch_prepare_slowness(const ModelDev *md_host, int model, const ModelDev *md_dev, float *SA,
                                    float *SB, int *err_flag, hipStream_t st);
-hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
-                          const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st);
-hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
-                          const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
+// paired: two sweeps at a time (two wave groups); else one group of waves
+// runs the four sweeps in turn (R = 8 only; tall and transposed grids)
+hipError_t launch_forward(int R, int nb_max, bool paired, const ModelDev *md, const ShotDev *shots, int nshots,
+                          Slots slots, const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st);
+hipError_t launch_adjoint(int R, int nb_max, bool paired, const ModelDev *md, const ShotDev *shots, int nshots,
+                          Slots slots, const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
                           int variant, hipStream_t st);
+// out (cols x rows) = the transpose of in (rows x cols), both row-major
+hipError_t launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t st);
 // adjoint variant bits: 0 DH division by a double multiply, 1 INF guard,
 // 2 (with bit 0) T-only quotients by the float FMA of ModelDev::rdh_hi/lo
 constexpr int kAdjFastDiv = 1, kAdjInfGuard = 2, kAdjFloatT = 4;

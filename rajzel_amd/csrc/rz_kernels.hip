@@ -508,37 +508,57 @@ __device__ int block_min(int v)
 
 template <int R>
 __device__ __attribute__((noinline)) float ordered_norm1(int nx_, int ny_, long long bstride_, const float *Tend_,
-                                                         const float *Tstart__, float tinit_, int src_k_, int src_c_)
+                                                         const float *Tstart__, float tinit_, int src_k_, int src_c_,
+                                                         int trans_)
 {
     constexpr int BR = kWave * R;
     static_assert(kNormPer % R == 0, "whole row groups per thread");
     __shared__ float s_s;
-    const int nx = uni(nx_), ny = uni(ny_), src_k = uni(src_k_), src_c = uni(src_c_);
+    const int nx = uni(nx_), ny = uni(ny_), src_k = uni(src_k_), src_c = uni(src_c_), trans = uni(trans_);
     const long long bstride = uni(bstride_);
     const float tinit = uni(tinit_);
     const float *Tstart_ = uni(Tstart__);
     const gfloat *Tend = gp(uni(Tend_)), *Tstart = gp(Tstart_);
-    const int nseg = (ny + kNormPer - 1) / kNormPer;   // row blocks per column
-    const int nblk = nx * nseg, t = threadIdx.x, nt = blockDim.x;
+    // the grid's x outer: columns here, or rows of a transposed grid
+    const int nseg = ((trans ? nx : ny) + kNormPer - 1) / kNormPer;   // blocks of terms per x
+    const int nblk = (trans ? ny : nx) * nseg, t = threadIdx.x, nt = blockDim.x;
     float s = 0.f;                                      // the running sum, the same in every thread
     for (int g0 = 0; g0 < nblk; g0 += nt) {
-        // this thread's terms: rows k0 .. k0 + kNormPer - 1 of column c
-        const int g = g0 + t, c = g / nseg, k0 = (g - c * nseg) * kNormPer;
+        const int g = g0 + t;
         const bool live = g < nblk;
         float d[kNormPer];
+        if (!trans) {
+            // this thread's terms: rows k0 .. k0 + kNormPer - 1 of column c
+            const int c = g / nseg, k0 = (g - c * nseg) * kNormPer;
 #pragma unroll
-        for (int j = 0; j < kNormPer; j += R) {
-            const int k = k0 + j, bb = k / BR, l = (k % BR) / R;
-            float te[R], ts[R];
-            if (live && k < ny) {
-                const long long a = bb * bstride + (long long)(c + l) * BR + l * R;
-                ldv<R>(Tend + a, te);
-                if (Tstart_) ldv<R>(Tstart + a, ts);
+            for (int j = 0; j < kNormPer; j += R) {
+                const int k = k0 + j, bb = k / BR, l = (k % BR) / R;
+                float te[R], ts[R];
+                if (live && k < ny) {
+                    const long long a = bb * bstride + (long long)(c + l) * BR + l * R;
+                    ldv<R>(Tend + a, te);
+                    if (Tstart_) ldv<R>(Tstart + a, ts);
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float t0 = Tstart_ ? ts[r] : ((k + r == src_k && c == src_c) ? 0.f : tinit);
+                    d[j + r] = (live && k + r < ny) ? fabsf(te[r] - t0) : 0.f;
+                }
             }
+        } else {
+            // transposed: columns c0 .. c0 + kNormPer - 1 of row k
+            const int k = g / nseg, c0 = (g - k * nseg) * kNormPer;
+            const int bb = k / BR, l = (k % BR) / R, r = k % R;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float t0 = Tstart_ ? ts[r] : ((k + r == src_k && c == src_c) ? 0.f : tinit);
-                d[j + r] = (live && k + r < ny) ? fabsf(te[r] - t0) : 0.f;
+            for (int j = 0; j < kNormPer; ++j) {
+                const int c = c0 + j;
+                float te = 0.f, t0 = 0.f;
+                if (live && c < nx) {
+                    const long long a = bb * bstride + (long long)(c + l) * BR + l * R + r;
+                    te = Tend[a];
+                    t0 = Tstart_ ? Tstart[a] : ((k == src_k && c == src_c) ? 0.f : tinit);
+                }
+                d[j] = (live && c < nx) ? fabsf(te - t0) : 0.f;
             }
         }
         const int mine = t * kNormPer;                 // chunk index of this thread's first term
@@ -735,7 +755,7 @@ __device__ bool decide(const ModelDev &md, double G, double mbound, const float 
     }
     if (d >= 0) return d != 0;
     nord += 1 << 16;
-    osum = ordered_norm1<R>(md.nx, md.ny, md.bstride, Wo, Wi, tinit, src_k, src_c);
+    osum = ordered_norm1<R>(md.nx, md.ny, md.bstride, Wo, Wi, tinit, src_k, src_c, md.trans);
     return stop_exact(md, osum);
 }
 
@@ -1540,12 +1560,20 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
                                                        sh.src_c, tinit, lockstep(0, 0))
                           : fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, Wi, md.SA, TB, stage, sh.src_k,
                                                         sh.src_c, tinit, lockstep(0, 0));
-            const Dec a2 = fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k,
-                                                       sh.src_c, tinit, lockstep(0, 0));
+            // a transposed grid swaps the second and fourth directions: the
+            // grid's (-,+) and (+,-) are (+,-) and (-,+) here
+            const Dec a2 = md.trans
+                ? fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c, tinit,
+                                              lockstep(0, 0))
+                : fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c, tinit,
+                                              lockstep(0, 0));
             const Dec a3 = fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k,
                                                        sh.src_c, tinit, lockstep(0, 0));
-            const Dec a4 = fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k,
-                                                       sh.src_c, tinit, lockstep(0, 0));
+            const Dec a4 = md.trans
+                ? fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c, tinit,
+                                              lockstep(0, 0))
+                : fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c, tinit,
+                                              lockstep(0, 0));
             acc.sum += a2.sum + a3.sum + a4.sum;
             acc.cnt += a2.cnt + a3.cnt + a4.cnt;
         } else {
@@ -1600,9 +1628,11 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
     const long long ob = rec_off[sh.model] + (long long)sh.shot * md.ntr;
     for (int i = threadIdx.x; i < md.ntr; i += blockDim.x) {
         const int k = md.rec_k[i], c = md.rec_c[i];
-        const int kn = (k <= md.ny - 2) ? k + 1 : (k == md.ny - 1 ? k - 1 : k);
+        int kn = k, cn = c;   // the next row of the grid (a column here if transposed)
+        if (md.trans) cn = (c <= md.nx - 2) ? c + 1 : (c == md.nx - 1 ? c - 1 : c);
+        else kn = (k <= md.ny - 2) ? k + 1 : (k == md.ny - 1 ? k - 1 : k);
         out.tmod[ob + i] = TA[addr_a<R>(k, c, md.bstride)];
-        out.tnb[ob + i] = TA[addr_a<R>(kn, c, md.bstride)];
+        out.tnb[ob + i] = TA[addr_a<R>(kn, cn, md.bstride)];
     }
     __syncthreads();
     mark_kept_cells<R>(md, TA, TB);
@@ -1670,8 +1700,14 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_adjoint(
         double acc;
         if constexpr (!PAIRED) {   // one group: the four sweeps one after the other
             acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wi, LB, stage, lockstep(0, 0));
-            acc += adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LB, stage, lockstep(0, 0));
-            acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, Wo, stage, lockstep(0, 0));
+            // a transposed grid swaps the second and third directions
+            if (md.trans) {
+                acc += adj_sweep<R, -1, +1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LB, stage, lockstep(0, 0));
+                acc += adj_sweep<R, +1, -1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, Wo, stage, lockstep(0, 0));
+            } else {
+                acc += adj_sweep<R, +1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, LB, stage, lockstep(0, 0));
+                acc += adj_sweep<R, -1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TB, LB, Wo, stage, lockstep(0, 0));
+            }
             acc += adj_sweep<R, -1, -1, true, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wo, Wo, stage, lockstep(0, 0));
         } else if (w.group == 0) {
             acc = adj_sweep<R, +1, +1, false, FD, CHKINF>(w, md.dh, md.rdh, md.rdh_hi, md.rdh_lo, md.eps_adj, TA, Wi, LB, stage, lockstep(0, D));
@@ -2015,13 +2051,14 @@ __global__ void k_split_reset(int *cnt, long long n)
 // direct != nullptr: the residual Tmod - Tobs is given (an adjoint source
 // dL/dTmod); no pick is muted and the slope scaling is applied as usual.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float residual_of(const ModelDev &md, int k, float tm, float tn, float r)
+__device__ __forceinline__ float residual_of(const ModelDev &md, int k, int ny, float tm, float tn, float r)
 {
     // calc_FA_res.c:52-69: divide by the vertical slope at the receiver
+    // (k: its row, ny: the grid's rows)
     float g = 0.f;
     bool have = true;
-    if (k <= md.ny - 2) g = (tn - tm) / md.dh;
-    else if (k == md.ny - 1) g = (tm - tn) / md.dh;
+    if (k <= ny - 2) g = (tn - tm) / md.dh;
+    else if (k == ny - 1) g = (tm - tn) / md.dh;
     else have = false;
     return have ? ((fabsf(g) > 0.f) ? r / g : 0.f) : r;
 }
@@ -2034,7 +2071,7 @@ __global__ void k_residual(const ModelDev *__restrict__ mds, int model, int ntr,
     const ModelDev md = mds[model];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntr) return;
-    const int k = md.rec_k[i];
+    const int k = md.trans ? md.rec_c[i] : md.rec_k[i], ny = md.trans ? md.nx : md.ny;   // the grid's row
     float cr = carry[i];
     constexpr int U = 8;
     for (int s = s0; s < s1; s += U) {
@@ -2052,7 +2089,7 @@ __global__ void k_residual(const ModelDev *__restrict__ mds, int model, int ntr,
             if (s + u >= s1) break;
             if (to[u] >= 0.f) {                                   // calc_FA_res.c:39
                 const float r = direct ? dd[u] : tm[u] - to[u];   // calc_FA_res.c:42
-                cr = residual_of(md, k, tm[u], tn[u], r);
+                cr = residual_of(md, k, ny, tm[u], tn[u], r);
             }
             tres[rec_off + (long long)(s + u) * ntr + i] = cr;
         }
@@ -2137,7 +2174,10 @@ __global__ void k_gradient(const ModelDev *__restrict__ mds, int model, const in
         const int pos = (kk == k && cc == c) ? (int)p : addr_a<R>(kk, cc, md.bstride);
         const double s = (double)md.SA[pos];
         const double s3 = s * s * s;
-        const long long cell = (long long)k * md.nx + c;
+        // the gradient is row-major in the grid's orientation: (ko, co) is
+        // the cell's row and column there
+        const int ko = md.trans ? c : k, co = md.trans ? k : c;
+        const long long cell = (long long)ko * (md.trans ? md.ny : md.nx) + co;
         float g = accumulate ? grad[cell] : 0.f;
         if (tp.mode == 0) {
             for (int j = 0; j < nsh; ++j) {
@@ -2156,9 +2196,9 @@ __global__ void k_gradient(const ModelDev *__restrict__ mds, int model, const in
                     if (r1 > r0) v = 1e-5f;
                 } else {
                     const int *bx = tp.box + 4 * (tp.src0 + j);
-                    if (k >= bx[0] && k <= bx[1] && c >= bx[2] && c <= bx[3])
-                        v *= tp.fsrc[tp.off[tp.src0 + j] + (long long)(k - bx[0]) * (bx[3] - bx[2] + 1) +
-                                     (c - bx[2])];
+                    if (ko >= bx[0] && ko <= bx[1] && co >= bx[2] && co <= bx[3])
+                        v *= tp.fsrc[tp.off[tp.src0 + j] + (long long)(ko - bx[0]) * (bx[3] - bx[2] + 1) +
+                                     (co - bx[2])];
                     for (unsigned i = r0; i < r1; ++i) v *= tp.frec[i];
                 }
                 g += v;
@@ -2240,6 +2280,18 @@ __global__ void k_rowmajor_to_field(int nx, int ny, long long bstride, const flo
     }
 }
 
+// out (cols x rows) = the transpose of in (rows x cols), row-major: the
+// fields of a grid stored transposed, to and from the caller's orientation.
+__global__ void k_transpose(const float *__restrict__ in, float *__restrict__ out, int rows, int cols)
+{
+    const long long n = (long long)rows * cols;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(i / cols), c = (int)(i - (long long)r * cols);
+        out[(long long)c * rows + r] = in[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -2260,6 +2312,13 @@ static int grid_for(long long n, int block)
     return (int)g;
 }
 
+hipError_t launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_transpose, dim3(grid_for((long long)rows * cols, 256)), dim3(256), 0, st, in, out, rows,
+                       cols);
+    return hipGetLastError();
+}
+
 hipError_t launch_prepare_slowness(const ModelDev *mh, int model, const ModelDev *md, float *SA, float *SB,
                                    int *err_flag, hipStream_t st)
 {
@@ -2269,11 +2328,12 @@ hipError_t launch_prepare_slowness(const ModelDev *mh, int model, const ModelDev
     return hipGetLastError();
 }
 
-hipError_t launch_forward(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
-                          const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st)
+hipError_t launch_forward(int R, int nb_max, bool paired, const ModelDev *md, const ShotDev *shots, int nshots,
+                          Slots slots, const long long *rec_off, const int *shot_base, ShotOut out, hipStream_t st)
 {
     if (nshots <= 0) return hipSuccess;
-    if (!paired_for(R, nb_max)) {   // unpaired: only at R = 8, one wave per band
+    if (paired && !paired_for(R, nb_max)) return hipErrorInvalidValue;
+    if (!paired) {   // unpaired: only at R = 8, one wave per band
         if (R != 8 || nb_max > max_bands_unpaired(R)) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_forward<8, false>), dim3(nshots), dim3(nb_max * kWave), 0, st, md, shots, slots, rec_off,
                            shot_base, out);
@@ -2325,12 +2385,12 @@ hipError_t launch_adjoint_split(int R, int nb_max, const ModelDev *md, const Sho
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(int R, int nb_max, const ModelDev *md, const ShotDev *shots, int nshots, Slots slots,
-                          const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
+hipError_t launch_adjoint(int R, int nb_max, bool paired, const ModelDev *md, const ShotDev *shots, int nshots,
+                          Slots slots, const long long *rec_off, const int *shot_base, const float *tres, ShotOut out,
                           int variant, hipStream_t st)
 {
     if (nshots <= 0) return hipSuccess;
-    const bool paired = paired_for(R, nb_max);
+    if (paired && !paired_for(R, nb_max)) return hipErrorInvalidValue;
     if (!paired && (R != 8 || nb_max > max_bands_unpaired(R))) return hipErrorInvalidValue;
     const dim3 g(nshots), b((paired ? kGroups : 1) * nb_max * kWave);
     // unpaired: only at R = 8, one wave per band

@@ -51,8 +51,8 @@ def _model(**kw):
     (dict(rec_ix=[1, 20]), "outside the grid"),
     (dict(rec_iy=[1, 15]), "outside the grid"),
     (dict(time=float("inf")), "must be finite"),
-    (dict(ny=10000), "bands"),
-    (dict(ny=4097), "bands"),                    # 9 bands of 512 rows: one more than a workgroup holds
+    (dict(ny=10000, nx=5000), "bands"),          # too tall stored either way
+    (dict(ny=4097, nx=4097), "bands"),           # 9 bands of 512 rows: one more than a workgroup holds
     (dict(nx=9000000, ny=5), "exceeds 2 GiB"),   # 45M cells, but one band of 64 rows: 2.3 GB per field
 ])
 def test_invalid_models_rejected_before_device(kw, msg):
@@ -60,6 +60,14 @@ def test_invalid_models_rejected_before_device(kw, msg):
     with pytest.raises(N.RzError) as e:
         N.Context([_model(**kw)])
     assert e.value.code == N.RZ_ERR_INVALID and msg in str(e.value)
+
+
+def test_transposed_grids_need_eight_rows_per_lane():
+    """NY = 10000 with NX = 20 is stored transposed, which only R = 8 runs."""
+    from rajzel_amd import native as N
+    with pytest.raises(N.RzError) as e:
+        N.Context([_model(ny=10000)], rows_per_lane=4)
+    assert e.value.code == N.RZ_ERR_INVALID and "transposed" in str(e.value)
 
 
 def test_invalid_rows_per_lane():
@@ -72,10 +80,10 @@ def test_invalid_rows_per_lane():
 def test_autotune_of_infeasible_model_fails_before_device():
     from rajzel_amd import native as N
     with pytest.raises(N.RzError) as e:
-        N.autotune_rows([_model(ny=10000)])
+        N.autotune_rows([_model(ny=10000, nx=5000)])
     assert e.value.code == N.RZ_ERR_INVALID and "rows-per-lane" in str(e.value)
     with pytest.raises(N.RzError) as e:
-        N.Context([_model(ny=10000)], rows_per_lane=N.ROWS_AUTOTUNE)
+        N.Context([_model(ny=10000, nx=5000)], rows_per_lane=N.ROWS_AUTOTUNE)
     assert "autotune" in str(e.value)
 
 
