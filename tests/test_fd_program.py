@@ -124,6 +124,21 @@ def test_programs_under_an_hbm_budget():
 
 
 @pytest.mark.gpu
+def test_programs_on_a_transposed_grid():
+    """A 4500 x 18 grid (more rows than a workgroup holds, solved
+    transposed): the forward with snapshots, and the grid search over 9
+    (vp0, grad0) models in one launch, write the reference's files byte for
+    byte."""
+    sv = P.survey(nx=18, ny=4500, nshots=3, seed=23)
+    out = _compare(lambda: P.make_workspace(sv, sv.vp_true, invmat=0, SNAP=1))
+    assert sum(k.startswith("tt/snap_shot_") for k in out) == sv.nshots
+    obs = _observed(sv)
+    grid = dict(VP0_1=1400.0, VP0_2=1650.0, DVP0=100.0, GRAD0_1=0.3, GRAD0_2=0.8, DGRAD0=0.2)
+    out = _compare(lambda: P.make_workspace(sv, P.start_model(sv), invmat=2, picks=obs, **grid))
+    assert len([m for m in out["gs.dat"].decode().split("\n") if m.strip()]) == 10   # header + 9 models
+
+
+@pytest.mark.gpu
 def test_fatt_lbfgs_wolfe():
     sv = P.survey()
     obs = _observed(sv)
