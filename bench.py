@@ -329,8 +329,9 @@ def main():
                     help="only print per-iteration kernel times versus shots in flight")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="finish with one device-to-device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic_fatt_gradient_1001x3001_200shots.json"),
-                    help="counter summary giving the HBM bytes per launch of the dominant kernel")
+    ap.add_argument("--traffic", default=None,
+                    help="counter summary giving the HBM bytes per launch of the dominant kernel "
+                         "(default: profiles/hbm_traffic_fatt_gradient_<ny>x<nx>_<shots>shots.json)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the gradient path shards by shot, and a multi-GPU run "
@@ -426,7 +427,9 @@ def main():
                ("12-layer model" if args.workload == "c4" else "1-D gradient model")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(args.traffic, dom, workload),
+                "traffic": measured_traffic(args.traffic or os.path.join(
+                    ROOT, "profiles", f"hbm_traffic_fatt_gradient_{sv.ny}x{sv.nx}_{sv.nshots}shots.json"),
+                    dom, workload),
                 "kernel": f"{dom} phase: k_{dom} (+ k_{dom}_split in mixed launches)",
                 "bytes_per_launch": phase_bytes[dom], "avg_launch_ms": phase_ms[dom]}
 
