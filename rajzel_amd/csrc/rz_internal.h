@@ -112,7 +112,10 @@ struct ShotOut {
 // becomes nonzero when a launch cannot run split (1: the two workgroups of
 // a shot were on different XCDs, 2: a wait timed out).
 constexpr int kSyncInts = 64;
-constexpr int kSyncPart = 4 * kMaxBands;   // doubles per split shot: partial sums, then counts
+// doubles per split shot: partial sums, counts, then the forward's flags of
+// sweeps that changed no cell (two per pair phase)
+constexpr int kPartFixed = 4 * kMaxBands;
+constexpr int kSyncPart = kPartFixed + 4;
 struct SplitSync {
     int *cnt;
     double *part;

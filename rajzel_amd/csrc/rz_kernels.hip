@@ -1519,6 +1519,58 @@ __device__ __forceinline__ WaveCtx make_wave(const ModelDev &md, float *ring_all
     return w;
 }
 
+// A forward sweep that changes no cell leaves every later sweep nothing to
+// change: a cell's update (sweep.c:37-97) is min(T, G) with G a function of
+// its four neighbours, the same in every direction, so when none of them
+// changed since the cell's last update, G is what it was then and T <= G
+// already.  With no change in one sweep, induction over the next sweep's
+// order gives none in it, and so on.
+//
+// Whether this workgroup's sweep that just ended changed no cell (cnt: a
+// lane's count of steps with a change).
+__device__ bool block_unchanged(double cnt)
+{
+    __shared__ int changed_s;
+    if (threadIdx.x == 0) changed_s = 0;
+    __syncthreads();
+    if (__builtin_amdgcn_ballot_w64(cnt > 0.0) != 0 && (threadIdx.x & 63) == 0) changed_s = 1;
+    __syncthreads();
+    const bool r = changed_s == 0;
+    __syncthreads();
+    return r;
+}
+
+// Whether the sweep of one of the two wave groups in the phase that just
+// ended changed no cell.
+__device__ bool pair_unchanged(double cnt, int group)
+{
+    __shared__ int changed_s[kGroups];
+    if (threadIdx.x < kGroups) changed_s[threadIdx.x] = 0;
+    __syncthreads();
+    if (__builtin_amdgcn_ballot_w64(cnt > 0.0) != 0 && (threadIdx.x & 63) == 0) changed_s[group] = 1;
+    __syncthreads();
+    const bool r = changed_s[0] == 0 || changed_s[1] == 0;
+    __syncthreads();
+    return r;
+}
+
+// After an iteration with a sweep that changed no cell and a decision to go
+// on: the next iteration changes no cell, its norm1 is 0, and 0 < TTNORM
+// ends the loop (eikonal.c:44).  Counts that iteration (or stops at the
+// iteration limit, as the loop's head would) without sweeping.  Returns
+// whether the loop goes on, which it does only when 0 >= TTNORM.
+__device__ __forceinline__ bool fixed_point(const ModelDev &md, int &it, float &osum)
+{
+    if (0.f >= md.ttnorm) return true;
+    if (md.max_iter > 0 && it >= md.max_iter) {
+        it = -1;
+    } else {
+        ++it;
+        osum = -1.f;   // decided by the zero sum, without an ordered sum
+    }
+    return false;
+}
+
 template <int R, bool PAIRED>
 __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
     const ModelDev *__restrict__ mds, const ShotDev *__restrict__ shots, Slots slots,
@@ -1549,6 +1601,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
     int it = 0, last = -1, nord = 0;
     float osum = -1.f;   // ordered sum of the last decision, if it took one
     [[maybe_unused]] Dec acc_next{0.0, 0.0};  // group 1: the sweep 1 run ahead
+    bool fixed = false;                       // a sweep of this iteration changed no cell
     while (cont) {
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
         float *Wo = W[it & 1];
@@ -1593,21 +1646,30 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
                                           lockstep(D, 0));
         acc.sum += a2.sum;
         acc.cnt += a2.cnt;
-        if (w.group == 0) {
+        // Sweeps 2 and 3 left the field in Wo (A) and TB (B).  If one of
+        // them changed no cell, sweep 4 and the next sweep 1 change none
+        // either (see pair_unchanged) and are skipped.
+        fixed = pair_unchanged(a2.cnt, w.group);
+        if (fixed) {
+            acc_next = Dec{0.0, 0.0};
+        } else if (w.group == 0) {
             const Dec a4 = fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k,
                                                        sh.src_c, tinit, lockstep(0, D));
             acc.sum += a4.sum;
             acc.cnt += a4.cnt;
             acc_next = Dec{0.0, 0.0};
+            fixed = pair_unchanged(a4.cnt, w.group);
         } else {
             acc_next = fwd_sweep<R, +1, +1, false>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k, sh.src_c,
                                                    tinit, lockstep(D, 0));
+            fixed = pair_unchanged(acc_next.cnt, w.group);
         }
         }
         const Dec tot = block_sum2(acc, red, nwaves);
         last = it++;
         bool passes;                                              // norm1.c:27, eikonal.c:44
         cont = decide<R, false>(md, tot.sum, tot.cnt, Wo, Wi, tinit, sh.src_k, sh.src_c, passes, nord, osum);
+        if (cont && fixed) cont = fixed_point(md, it, osum);
     }
     const int nfield = md.nb * (int)md.bstride;
     if (last >= 0) {  // the last sweep wrote W[last & 1]; the adjoint reads TA and TB
@@ -1891,24 +1953,37 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
                                                       sh.src_c, tinit, pc);
         acc.sum += a2.sum;
         acc.cnt += a2.cnt;
+        // as in k_forward: if sweep 2 or 3 changed no cell, sweep 4 and the
+        // next sweep 1 are skipped (each workgroup tells the other, in part,
+        // whether its sweep changed a cell)
+        bool mine = block_unchanged(a2.cnt);
+        if (threadIdx.x == 0) part[kPartFixed + role] = mine ? 1.0 : 0.0;
         ok = rendezvous(cnt, role, phase++, ss.err);
         if (!ok) break;
+        bool fixed = mine || part[kPartFixed + 1 - role] != 0.0;
         pc.base = phase * kPhaseStride;
-        if (role == 0) {
+        if (fixed) {
+            acc_next = Dec{0.0, 0.0};
+        } else if (role == 0) {
             const Dec a4 = fwd_sweep<R, +1, -1, false, kPaceLead>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage,
                                                                   sh.src_k, sh.src_c, tinit, pc);
             acc.sum += a4.sum;
             acc.cnt += a4.cnt;
             acc_next = Dec{0.0, 0.0};
+            mine = block_unchanged(a4.cnt);
         } else {
             acc_next = fwd_sweep<R, +1, +1, false, kPaceTrail>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage,
                                                                sh.src_k, sh.src_c, tinit, pc);
+            mine = block_unchanged(acc_next.cnt);
         }
+        if (!fixed && threadIdx.x == 0) part[kPartFixed + 2 + role] = mine ? 1.0 : 0.0;
         const Dec tot = pair_sum2(acc, red, nwaves, part, cnt, role, phase++, ss.err, ok);
         if (!ok) break;
+        if (!fixed) fixed = mine || part[kPartFixed + 2 + 1 - role] != 0.0;
         last = it++;
         bool passes;                                              // norm1.c:27, eikonal.c:44
         cont = decide<R, false>(md, tot.sum, tot.cnt, Wo, Wi, tinit, sh.src_k, sh.src_c, passes, nord, osum);
+        if (cont && fixed) cont = fixed_point(md, it, osum);
         if (passes) {
             // both workgroups read the whole grid alike; they meet before the
             // next iteration's sweep 2 overwrites Wi
