@@ -100,6 +100,7 @@ struct rz_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<int> hist_f, hist_a;    // outer iterations per shot in the last run (-1 unknown)
     std::vector<int> hist_pf, hist_pa;  // stopping-test passes per shot in the last run (ord_f/ord_a words)
+    std::vector<int> hist_wf;           // sweep phases of each shot's forward in the last run
     ShotDev *plan_d = nullptr;          // [4][total_shots]: forward split / rest, adjoint split / rest
     int plan_n[4] = {0, 0, 0, 0};
     std::vector<int> plan_ids;          // shot order of the plan now on the device
@@ -113,10 +114,12 @@ struct rz_ctx {
     int *lsel_d = nullptr;              // per shot: the converged lambda is in LC (1) or LA (0)
     int *ord_f_d = nullptr, *ord_a_d = nullptr;   // per shot: passes of the stopping test over the fields
     float *osum_f_d = nullptr, *osum_a_d = nullptr; // per shot: ordered sum of the last decision
+    int *phases_f_d = nullptr;          // per shot: sweep phases of the forward
     int *err_d = nullptr;
     // single-shot scratch
     float *s_tmod = nullptr, *s_tnb = nullptr, *s_tres = nullptr, *s_row = nullptr, *s_row2 = nullptr;
-    int *s_it = nullptr;                // iterations forward, adjoint; lambda location; stop passes f, a
+    int *s_it = nullptr;                // iterations forward, adjoint; lambda location; stop passes f, a;
+                                        // forward phases
     float *s_osum = nullptr;            // ordered sums of the last decisions, forward and adjoint
     ShotDev *s_shot = nullptr;
     int *s_slot0 = nullptr;
@@ -423,6 +426,7 @@ ShotOut outs_of(const rz_ctx *c)
     ShotOut o;
     o.tmod = c->tmod_d; o.tnb = c->tnb_d; o.it_f = c->it_f_d; o.it_a = c->it_a_d; o.lsel = c->lsel_d;
     o.ord_f = c->ord_f_d; o.ord_a = c->ord_a_d; o.osum_f = c->osum_f_d; o.osum_a = c->osum_a_d;
+    o.phases_f = c->phases_f_d;
     return o;
 }
 
@@ -445,8 +449,9 @@ int check_convergence(rz_ctx *c, bool adj)
     if (c->total_shots == 0) return RZ_OK;
     int serr = 0;
     HIPCHK(c, hipMemcpyAsync(&serr, c->ss.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    std::vector<int> pf(c->total_shots), pa(c->total_shots);
+    std::vector<int> pf(c->total_shots), pa(c->total_shots), wf(c->total_shots);
     HIPCHK(c, hipMemcpyAsync(itf.data(), c->it_f_d, itf.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(wf.data(), c->phases_f_d, wf.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(pf.data(), c->ord_f_d, pf.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if (adj) {
         HIPCHK(c, hipMemcpyAsync(ita.data(), c->it_a_d, ita.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -464,6 +469,7 @@ int check_convergence(rz_ctx *c, bool adj)
     }
     c->hist_f = itf;
     c->hist_pf = pf;
+    c->hist_wf = wf;
     if (adj) {
         c->hist_a = ita;
         c->hist_pa = pa;
@@ -552,10 +558,13 @@ int plan_mixed(rz_ctx *c, int per)
         // a shot's expected time in sixteenths of an outer iteration: an
         // ordered-sum pass of the stopping test costs about half an
         // iteration, a count pass about a sixteenth
+        // (the forward's sweeps after a sweep that changed no cell are
+        // skipped: its time is its sweep phases, two per iteration)
         std::vector<int> cost(T);
         for (int i = 0; i < T; ++i) {
             const int w = (int)hp.size() == T ? hp[i] : 0;
-            cost[i] = 16 * h[i] + 8 * (w >> 16) + (w & 0xFFFF);
+            const int run = (ph == 0 && (int)c->hist_wf.size() == T) ? 8 * c->hist_wf[i] : 16 * h[i];
+            cost[i] = run + 8 * (w >> 16) + (w & 0xFFFF);
         }
         std::vector<int> ord(T);
         for (int i = 0; i < T; ++i) ord[i] = i;
@@ -991,6 +1000,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
         (rc = dalloc(c, &c->it_a_d, shots)) || (rc = dalloc(c, &c->lsel_d, shots)) ||
         (rc = dalloc(c, &c->ord_f_d, shots)) || (rc = dalloc(c, &c->ord_a_d, shots)) ||
         (rc = dalloc(c, &c->osum_f_d, shots)) || (rc = dalloc(c, &c->osum_a_d, shots)) ||
+        (rc = dalloc(c, &c->phases_f_d, shots)) ||
         (rc = dalloc(c, &c->err_d, 1)))
         return bail(rc);
     int ntr_max = 0, ncell_max = 0;
@@ -1000,7 +1010,7 @@ int rz_create(rz_ctx **out, int device, int nmodels, const rz_model_desc *models
     }
     if ((rc = dalloc(c, &c->s_tmod, ntr_max)) || (rc = dalloc(c, &c->s_tnb, ntr_max)) ||
         (rc = dalloc(c, &c->s_tres, ntr_max)) || (rc = dalloc(c, &c->s_row, ncell_max)) ||
-        (rc = dalloc(c, &c->s_row2, ncell_max)) || (rc = dalloc(c, &c->s_it, 5)) || (rc = dalloc(c, &c->s_osum, 2)) ||
+        (rc = dalloc(c, &c->s_row2, ncell_max)) || (rc = dalloc(c, &c->s_it, 6)) || (rc = dalloc(c, &c->s_osum, 2)) ||
         (rc = dalloc(c, &c->s_shot, 1)) || (rc = dalloc(c, &c->s_slot0, 1)))
         return bail(rc);
 
@@ -1332,7 +1342,7 @@ int rz_eikonal_shot(rz_ctx *c, int model, int src_ix, int src_iy, float *T, floa
     ShotOut o;
     o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1; o.lsel = c->s_it + 2;
     o.ord_f = c->s_it + 3; o.ord_a = c->s_it + 4;
-    o.osum_f = c->s_osum; o.osum_a = c->s_osum + 1;
+    o.osum_f = c->s_osum; o.osum_a = c->s_osum + 1; o.phases_f = c->s_it + 5;
     HIPCHK(c, launch_forward(c->R, c->nb_max, c->paired, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d, o,
                              c->stream));
     ModelDev mh = make_dev(c, h);
@@ -1403,7 +1413,7 @@ int rz_eikonal_adj_shot(rz_ctx *c, int model, const float *T, const float *tres,
     ShotOut o;
     o.tmod = c->s_tmod; o.tnb = c->s_tnb; o.it_f = c->s_it; o.it_a = c->s_it + 1; o.lsel = c->s_it + 2;
     o.ord_f = c->s_it + 3; o.ord_a = c->s_it + 4;
-    o.osum_f = c->s_osum; o.osum_a = c->s_osum + 1;
+    o.osum_f = c->s_osum; o.osum_a = c->s_osum + 1; o.phases_f = c->s_it + 5;
     HIPCHK(c, launch_adjoint(c->R, c->nb_max, c->paired, c->md_d, c->s_shot, 1, slots_of(c), c->rec_off0_d, c->shot_base0_d,
                              c->s_tres, o, variant, c->stream));
     HIPCHK(c, launch_field_to_rowmajor(&mh, c->LA, c->LC, c->s_it + 2, c->s_row, 0, c->stream));

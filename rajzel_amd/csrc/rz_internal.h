@@ -105,6 +105,8 @@ struct ShotOut {
     int *ord_f, *ord_a;  // passes of the stopping test over the fields (indexed as it_f): the
                          // counts of changed cells in bits 0-15, the ordered sums above
     float *osum_f, *osum_a;  // norm1's ordered float sum of the last decision, -1 if not taken
+    int *phases_f;   // sweep phases the forward ran (a pair of sweeps or a lone sweep each;
+                     // indexed as it_f): its time, in sweeps, for planning the next launch
 };
 
 // Coupling of the two workgroups of split shots (rz_kernels.hip): per shot

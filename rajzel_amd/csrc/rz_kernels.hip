@@ -1602,6 +1602,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
     float osum = -1.f;   // ordered sum of the last decision, if it took one
     [[maybe_unused]] Dec acc_next{0.0, 0.0};  // group 1: the sweep 1 run ahead
     bool fixed = false;                       // a sweep of this iteration changed no cell
+    int phases = 0;
     while (cont) {
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
         float *Wo = W[it & 1];
@@ -1629,7 +1630,9 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
                                               lockstep(0, 0));
             acc.sum += a2.sum + a3.sum + a4.sum;
             acc.cnt += a2.cnt + a3.cnt + a4.cnt;
+            phases += 4;
         } else {
+        phases += (it == 0 ? 2 : 1) + 1;   // sweep 1 alone, pair 2+3, pair 4+1 (less one if skipped)
         if (it == 0) {
             if (w.group == 0)
                 acc = fwd_sweep<R, +1, +1, true>(w, md.dh, md.two_dh2, TA, md.SA, TB, stage, sh.src_k, sh.src_c,
@@ -1652,6 +1655,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         fixed = pair_unchanged(a2.cnt, w.group);
         if (fixed) {
             acc_next = Dec{0.0, 0.0};
+            --phases;
         } else if (w.group == 0) {
             const Dec a4 = fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k,
                                                        sh.src_c, tinit, lockstep(0, D));
@@ -1702,6 +1706,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         out.it_f[shot_base[sh.model] + sh.shot] = it;
         out.ord_f[shot_base[sh.model] + sh.shot] = nord;
         out.osum_f[shot_base[sh.model] + sh.shot] = osum;
+        out.phases_f[shot_base[sh.model] + sh.shot] = phases;
     }
 }
 
@@ -1929,11 +1934,13 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
     float osum = -1.f;
     bool ok = true;
     Dec acc_next{0.0, 0.0};
+    int phases = 0;
     while (cont) {
         if (md.max_iter > 0 && it >= md.max_iter) { it = -1; break; }
         float *Wo = W[it & 1];
         const float *Wi = it > 0 ? W[(it - 1) & 1] : nullptr;
         Dec acc{0.0, 0.0};
+        phases += (it == 0 ? 2 : 1) + 1;
         if (it == 0) {
             if (role == 0) {
                 pc.base = phase * kPhaseStride;
@@ -1964,6 +1971,7 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
         pc.base = phase * kPhaseStride;
         if (fixed) {
             acc_next = Dec{0.0, 0.0};
+            --phases;
         } else if (role == 0) {
             const Dec a4 = fwd_sweep<R, +1, -1, false, kPaceLead>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage,
                                                                   sh.src_k, sh.src_c, tinit, pc);
@@ -2020,6 +2028,7 @@ __global__ __launch_bounds__(max_bands_for(R) * kWave, max_bands_for(R) / 2) voi
         out.it_f[shot_base[sh.model] + sh.shot] = it;
         out.ord_f[shot_base[sh.model] + sh.shot] = nord;
         out.osum_f[shot_base[sh.model] + sh.shot] = osum;
+        out.phases_f[shot_base[sh.model] + sh.shot] = phases;
     }
 }
 
