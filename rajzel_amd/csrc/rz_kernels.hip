@@ -1621,16 +1621,32 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
                                               lockstep(0, 0))
                 : fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c, tinit,
                                               lockstep(0, 0));
-            const Dec a3 = fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k,
-                                                       sh.src_c, tinit, lockstep(0, 0));
-            const Dec a4 = md.trans
-                ? fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c, tinit,
-                                              lockstep(0, 0))
-                : fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c, tinit,
-                                              lockstep(0, 0));
-            acc.sum += a2.sum + a3.sum + a4.sum;
-            acc.cnt += a2.cnt + a3.cnt + a4.cnt;
-            phases += 4;
+            acc.sum += a2.sum;
+            acc.cnt += a2.cnt;
+            phases += 2;
+            // After a sweep that changed no cell the rest are skipped (see
+            // block_unchanged; an unchanged sweep 1 leaves sweep 2 unchanged
+            // too): Wo holds the field of sweep 2, which sweep 3 kept.
+            fixed = block_unchanged(a2.cnt);
+            if (!fixed) {
+                const Dec a3 = fwd_sweep<R, -1, -1, false>(w, md.dh, md.two_dh2, Wo, md.SA, TB, stage, sh.src_k,
+                                                           sh.src_c, tinit, lockstep(0, 0));
+                acc.sum += a3.sum;
+                acc.cnt += a3.cnt;
+                ++phases;
+                fixed = block_unchanged(a3.cnt);
+            }
+            if (!fixed) {
+                const Dec a4 = md.trans
+                    ? fwd_sweep<R, -1, +1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c,
+                                                  tinit, lockstep(0, 0))
+                    : fwd_sweep<R, +1, -1, false>(w, md.dh, md.two_dh2, TB, md.SB, Wo, stage, sh.src_k, sh.src_c,
+                                                  tinit, lockstep(0, 0));
+                acc.sum += a4.sum;
+                acc.cnt += a4.cnt;
+                ++phases;
+                fixed = block_unchanged(a4.cnt);
+            }
         } else {
         phases += (it == 0 ? 2 : 1) + 1;   // sweep 1 alone, pair 2+3, pair 4+1 (less one if skipped)
         if (it == 0) {
@@ -1651,7 +1667,7 @@ __global__ __launch_bounds__(kGroups * max_bands_for(R) * kWave) void k_forward(
         acc.cnt += a2.cnt;
         // Sweeps 2 and 3 left the field in Wo (A) and TB (B).  If one of
         // them changed no cell, sweep 4 and the next sweep 1 change none
-        // either (see pair_unchanged) and are skipped.
+        // either (see block_unchanged) and are skipped.
         fixed = pair_unchanged(a2.cnt, w.group);
         if (fixed) {
             acc_next = Dec{0.0, 0.0};

@@ -19,23 +19,20 @@ from tests.helpers import bits_equal, model_dict, ndiff
 
 pytestmark = pytest.mark.gpu
 
-CASES = [  # (NX, NY, sigma, R)
+CASES = [  # (NX, NY, sigma, R): paired sweeps
     (300, 120, 2.0, 2),
     (400, 150, 6.0, 4),
     (64, 700, 1.0, 4),
 ]
+# five bands of 512 rows (one wave group, sweeps one after the other), and a
+# grid of more than 4096 rows, solved transposed
+TALL = [(48, 2300, 3.0, 8), (20, 4500, 1.0, 8)]
 
 
-@pytest.mark.parametrize("split", [False, True])
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[1]}x{c[0]}-s{c[2]:g}-R{c[3]}")
-def test_forward_iteration_limits_around_the_fixed_point(ora, case, split):
+def _check(ora, sv, R, split, limits):
     from rajzel_amd import native as N
-    nx, ny, sigma, R = case
-    sv = W.edge_case(nx, ny, nshots=3, seed=nx + 3 * ny, sigma=sigma, ttnorm=1e-10)
-    _, _, it_ref = ora.forward(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.dh, sv.time, sv.ttnorm)
-    assert (it_ref > 1).all(), it_ref
-    for maxit in range(1, int(it_ref.max()) + 2):
-        print(f"[case] fixed point {ny}x{nx} R={R} split={split} max_iter={maxit}", flush=True)
+    for maxit in limits:
+        print(f"[case] fixed point {sv.ny}x{sv.nx} R={R} split={split} max_iter={maxit}", flush=True)
         tm_o, _, it_o = ora.forward(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.dh, sv.time,
                                     sv.ttnorm, maxit)
         with N.Context([model_dict(sv, max_iter=maxit)], rows_per_lane=R) as ctx:
@@ -54,3 +51,29 @@ def test_forward_iteration_limits_around_the_fixed_point(ora, case, split):
                 T = ctx.field(0, s, "T")
                 assert bits_equal(T, T_o), f"max_iter {maxit} shot {s}: {ndiff(T, T_o)} cells differ"
             assert ctx.split_state()[1] == 0
+
+
+def _survey(nx, ny, sigma):
+    return W.edge_case(nx, ny, nshots=3, seed=nx + 3 * ny, sigma=sigma, ttnorm=1e-10)
+
+
+@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[1]}x{c[0]}-s{c[2]:g}-R{c[3]}")
+def test_forward_iteration_limits_around_the_fixed_point(ora, case, split):
+    nx, ny, sigma, R = case
+    sv = _survey(nx, ny, sigma)
+    _, _, it_ref = ora.forward(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.dh, sv.time, sv.ttnorm)
+    assert (it_ref > 1).all(), it_ref
+    _check(ora, sv, R, split, range(1, int(it_ref.max()) + 2))
+
+
+@pytest.mark.parametrize("case", TALL, ids=lambda c: f"{c[1]}x{c[0]}-s{c[2]:g}-R{c[3]}")
+def test_forward_fixed_point_one_wave_group(ora, case):
+    """The same with the sweeps run one after the other (more bands than a
+    pair of wave groups holds), at each shot's own count and one either
+    side of it."""
+    nx, ny, sigma, R = case
+    sv = _survey(nx, ny, sigma)
+    _, _, it_ref = ora.forward(sv.S, sv.src_ix, sv.src_iy, sv.rec_ix, sv.rec_iy, sv.dh, sv.time, sv.ttnorm)
+    limits = sorted({int(i) + d for i in it_ref for d in (-1, 0, 1)} | {0})
+    _check(ora, sv, R, False, limits)
