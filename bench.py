@@ -215,6 +215,18 @@ def copy_probe(mib):
         hip.hipFree(b)
 
 
+def workload_name(sv, kind):
+    """The workload named in the bench line and in the traffic summaries
+    (kind "c4": the 12-layer model, "c5": the 1-D gradient model)."""
+    return (f"FATT gradient evaluation, {sv.ny}x{sv.nx} grid, DH={sv.dh} m, {sv.nshots} shots, "
+            f"{sv.ntr} receivers, " + ("12-layer model" if kind == "c4" else "1-D gradient model"))
+
+
+def traffic_path(sv):
+    """The committed counter summary of this workload's evaluation."""
+    return os.path.join(ROOT, "profiles", f"hbm_traffic_fatt_gradient_{sv.ny}x{sv.nx}_{sv.nshots}shots.json")
+
+
 def measured_traffic(path, phase, workload):
     """HBM bytes of one evaluation's ``phase`` ("forward" or "adjoint": the
     one-workgroup and the split kernel of the phase together) from the
@@ -422,14 +434,10 @@ def main():
     dom = max(("forward", "adjoint"), key=lambda k: phase_ms[k])
     achieved = phase_bytes[dom] / (phase_ms[dom] * 1e-3) / 1e9
     eval_gbs = (fwd_b + adj_b + grd_b) / (ms * 1e-3) / 1e9
-    workload = f"FATT gradient evaluation, {sv.ny}x{sv.nx} grid, DH={sv.dh} m, " \
-               f"{sv.nshots} shots, {sv.ntr} receivers, " + \
-               ("12-layer model" if args.workload == "c4" else "1-D gradient model")
+    workload = workload_name(sv, args.workload)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(args.traffic or os.path.join(
-                    ROOT, "profiles", f"hbm_traffic_fatt_gradient_{sv.ny}x{sv.nx}_{sv.nshots}shots.json"),
-                    dom, workload),
+                "traffic": measured_traffic(args.traffic or traffic_path(sv), dom, workload),
                 "kernel": f"{dom} phase: k_{dom} (+ k_{dom}_split in mixed launches)",
                 "bytes_per_launch": phase_bytes[dom], "avg_launch_ms": phase_ms[dom]}
 

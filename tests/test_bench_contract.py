@@ -1,13 +1,14 @@
 """bench.py's inputs that are committed files: the HBM-traffic summary it
 reads for roofline.traffic must name the workload the bench runs (the
 match is exact, and a mismatch silently reports traffic = null), and hold
-both phases' byte counts."""
+both phases' byte counts, for each of its two workloads."""
 import json
 import os
 import sys
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic_fatt_gradient_1001x3001_200shots.json")
 
 
 def _bench():
@@ -16,17 +17,18 @@ def _bench():
     return bench
 
 
-def test_traffic_summary_matches_the_bench_workload():
+@pytest.mark.parametrize("kind", ["c4", "c5"])
+def test_traffic_summary_matches_the_bench_workload(kind):
     bench = _bench()
     from rajzel_amd import workloads as W
-    sv = W.c4_gradient()
-    workload = (f"FATT gradient evaluation, {sv.ny}x{sv.nx} grid, DH={sv.dh} m, "
-                f"{sv.nshots} shots, {sv.ntr} receivers, 12-layer model")
-    rec = json.load(open(TRAFFIC))
+    sv = W.c4_gradient() if kind == "c4" else W.c5_gradient()
+    workload = bench.workload_name(sv, kind)
+    path = bench.traffic_path(sv)
+    rec = json.load(open(path))
     assert rec["workload"] == workload
     for phase in ("forward", "adjoint"):
-        b = bench.measured_traffic(TRAFFIC, phase, workload)
-        assert b is not None and 1e10 < b < 1e12, (phase, b)
+        b = bench.measured_traffic(path, phase, workload)
+        assert b is not None and 1e10 < b < 1e13, (phase, b)
     # calibration of the counters by the 1 GiB copy
     assert 1.9 < rec["fetch_scale"] < 2.1 and 0.9 < rec["write_scale"] < 1.1
 
