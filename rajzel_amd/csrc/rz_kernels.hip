@@ -1027,8 +1027,49 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
         }                                                                                   \
     } while (0)
 
+    // The first sweep from the initial field (INIT, always +x) changes no
+    // cell left of column src_c - 1: such a cell's neighbours all hold the
+    // initial value (the source's column is further right), and the update
+    // of a cell among initial values returns at least that value.  Keys
+    // below src_c - 1 hold only such cells (key = column + lane in layout
+    // A), so the epochs of this band whose steps all lie there just store
+    // the initial value: no loads, no arithmetic.  gbeg is the first epoch
+    // computed in full (0 in every other sweep).
+    int gbeg = 0;
+    if constexpr (INIT && SX > 0) gbeg = min(max(src_c - 1 + off, 0) / kPF, nep) * kPF;
+    float pnew[R];
 #pragma unroll
-    for (int u = 0; u < PS; ++u) RZ_FWD_LOAD(u, u - off);
+    for (int r = 0; r < R; ++r) pnew[r] = tinit;
+    double acc = 0.0;
+    unsigned chg = 0;        // steps at which some of this lane's cells changed
+    float ring_next = 0.f;   // first consumer step is >= kBandOff steps away
+    if constexpr (INIT && SX > 0) {
+        for (int g0 = 0; g0 < gbeg; g0 += kPF) {
+#pragma unroll
+            for (int u = 0; u < kPF; ++u) {
+                const int tau = g0 + u - off;
+                const bool cvalid = tau >= 0 && tau < nkeys && (unsigned)(tau - lag) < (unsigned)nx;
+                const int c = tau - lag;
+                {   // read one step ahead, as below
+                    const int cn = tau + 1 - lag;
+                    ring_next = ring[ering * kRingQ + (clampi(cn, 0, nx - 1) & (kRingQ - 1))];
+                }
+                const int tp = tau - dl;
+                float piece[R];
+                ldl<R>(lstage + (tp & (SQ - 1)) * BR, piece);
+                stl<R>(lstage + (tau & (SQ - 1)) * BR, pnew);
+                int ko = tp + kshift;
+                ko = (unsigned)ko < (unsigned)nkeys ? ko : nkeys;
+                stv<R>(gTout + ko * BR + l * R, piece);
+                lfloat *rw = (ring_writer && cvalid) ? &ring[beta * kRingQ + (c & (kRingQ - 1))] : ring_dummy;
+                *rw = tinit;
+            }
+            pace_epoch<PACE>(pc, g0 / kPF);
+        }
+    }
+
+#pragma unroll
+    for (int u = 0; u < PS; ++u) RZ_FWD_LOAD(u, gbeg + u - off);
     // The set-up ends in a full wait: the waits the compiler derives at the
     // loop head then come from the loop's own pattern of prefetches (a slot
     // is waited for PS - 1 steps after its load), not from a set-up
@@ -1036,14 +1077,7 @@ __device__ __attribute__((noinline)) Dec fwd_sweep(const WaveCtx w_, float dh_, 
     // every epoch.
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
 
-    float pnew[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) pnew[r] = tinit;
-    double acc = 0.0;
-    unsigned chg = 0;        // steps at which some of this lane's cells changed
-    float ring_next = 0.f;   // first consumer step is >= kBandOff steps away
-
-    for (int g0 = 0; g0 < gend; g0 += kPF) {
+    for (int g0 = gbeg; g0 < gend; g0 += kPF) {
 #pragma unroll
         for (int u = 0; u < kPF; ++u) {
             const int sl = u % PS, un = (u + 1) % PS;   // g0 is a multiple of PS
