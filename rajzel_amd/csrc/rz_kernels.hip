@@ -2318,6 +2318,9 @@ __global__ void k_gradient(const ModelDev *__restrict__ mds, int model, const in
         const long long cell = (long long)ko * (md.trans ? md.ny : md.nx) + co;
         float g = accumulate ? grad[cell] : 0.f;
         if (tp.mode == 0) {
+            // unrolled so that several shots' loads are in flight; the sum
+            // stays in shot order
+#pragma unroll 8
             for (int j = 0; j < nsh; ++j) {
                 const float *L = lsel[j] ? slots.LC : slots.LA;   // where the shot's lambda converged
                 const float lam = L[(long long)slot_of[j] * slots.slot_stride + pos];
